@@ -1,0 +1,89 @@
+// Internal declarations shared by the HIP sources of libalgotom_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/algotom_hip.h"
+
+namespace ath {
+
+// ---- error state (thread-local; read through ath_last_error) -------------
+int set_error(int status, const std::string& msg);
+int check_hip(hipError_t err, const char* what);
+void clear_error();
+
+// ---- element conversions ---------------------------------------------------
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(__half v) { return __half2float(v); }
+__device__ __forceinline__ float to_f32(__hip_bfloat16 v) {
+    return __bfloat162float(v);
+}
+
+__device__ __forceinline__ void store_elem(float* p, double v) {
+    *p = (float)v;
+}
+__device__ __forceinline__ void store_elem(double* p, double v) { *p = v; }
+__device__ __forceinline__ void store_elem(__half* p, double v) {
+    *p = __float2half((float)v);
+}
+__device__ __forceinline__ void store_elem(__hip_bfloat16* p, double v) {
+    *p = __float2bfloat16((float)v);
+}
+
+// numpy.pad index maps: i = padded index - pad; returns the source column,
+// or -1 where the padded sample is zero (mode "constant").
+__host__ __device__ __forceinline__ int pad_source(int i, int n, int mode) {
+    if (i >= 0 && i < n) return i;
+    switch (mode) {
+    case ATH_PAD_EDGE: return i < 0 ? 0 : n - 1;
+    case ATH_PAD_WRAP: { int j = i % n; return j < 0 ? j + n : j; }
+    case ATH_PAD_REFLECT: {
+        if (n == 1) return 0;
+        const int p = 2 * (n - 1);
+        int j = i % p; if (j < 0) j += p;
+        return j < n ? j : p - j; }
+    case ATH_PAD_SYMMETRIC: {
+        const int p = 2 * n;
+        int j = i % p; if (j < 0) j += p;
+        return j < n ? j : p - 1 - j; }
+    default: return -1;
+    }
+}
+
+// ---- ramp filter (ramp_filter.hip) ----------------------------------------
+int fft_length(int ncol, int pad);
+size_t plan_bytes(int fft_len, int nwin);
+int plan_build(const double* taps, int nwin, int width, int fft_len,
+               void* plan, hipStream_t stream);
+int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
+                int ncol, int pad, int pad_mode, int apply_log,
+                const void* plan, int fft_len, int nwin, void* out,
+                int out_dtype, hipStream_t stream);
+
+// ---- back-projection (backproject.hip) -------------------------------------
+struct BpArgs {
+    const void* sino;      // (nrow, nslice, ncol)
+    float* recon;          // (ncol, nslice, ncol)
+    const float* angles;   // nrow float32 radians
+    int nrow, nslice, ncol;
+    double center;         // float32 centre widened to double
+    double icenter;        // 0.5 * (ncol - 1)
+    int edge_pad;
+    int use_mask;
+    double mask_r2;        // (ratio * (ncol // 2))^2
+    int mask_c;            // ncol // 2
+    int apply_scale;
+    float pi32, denom;     // float32(pi), float32(nrow - 1)
+};
+
+int back_projection(const BpArgs& a, int sino_dtype, int acc,
+                    hipStream_t stream);
+int back_projection_naive(const BpArgs& a, int acc, hipStream_t stream);
+
+}  // namespace ath

@@ -1,0 +1,283 @@
+// extern "C" entry points of libalgotom_hip (declared in include/algotom_hip.h).
+// Argument checks mirror the reference's own (reconstruction.py:412-419);
+// nothing here allocates device memory or copies between host and device.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ath_internal.h"
+
+namespace ath {
+
+namespace {
+thread_local int g_status = ATH_OK;
+thread_local std::string g_message;
+}  // namespace
+
+int set_error(int status, const std::string& msg) {
+    g_status = status;
+    g_message = msg;
+    return status;
+}
+
+void clear_error() {
+    g_status = ATH_OK;
+    g_message.clear();
+}
+
+int check_hip(hipError_t err, const char* what) {
+    if (err == hipSuccess) return ATH_OK;
+    return set_error(ATH_ERR_HIP, std::string(what) + ": " +
+                                      hipGetErrorName(err) + " (" +
+                                      hipGetErrorString(err) + ")");
+}
+
+namespace {
+
+bool valid_dtype(int t) {
+    return t == ATH_F32 || t == ATH_F64 || t == ATH_F16 || t == ATH_BF16;
+}
+
+size_t dtype_bytes(int t) {
+    return t == ATH_F64 ? 8 : (t == ATH_F32 ? 4 : 2);
+}
+
+int check_geometry(int nrow, int nslice, int ncol) {
+    if (nrow < 1 || nslice < 1 || ncol < 1)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "nrow, nslice and ncol must be positive");
+    return ATH_OK;
+}
+
+int fill_bp_args(BpArgs& a, const void* sino, int nrow, int nslice, int ncol,
+                 const float* angles, float center, int edge_pad,
+                 double mask_radius, int apply_scale, float* recon) {
+    if (!sino || !angles || !recon)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    int st = check_geometry(nrow, nslice, ncol);
+    if (st != ATH_OK) return st;
+    // The reference checks 0 <= center < ncol in Python (reconstruction.py
+    // :412-413); its kernels accept any value, as this boundary does.
+    if (!std::isfinite(center))
+        return set_error(ATH_ERR_INVALID_ARG, "center must be finite");
+    if (apply_scale && nrow < 2)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "the pi/(nrow-1) scale needs at least two angles");
+    a.sino = sino;
+    a.recon = recon;
+    a.angles = angles;
+    a.nrow = nrow;
+    a.nslice = nslice;
+    a.ncol = ncol;
+    a.center = (double)center;
+    a.icenter = 0.5 * (double)(ncol - 1);
+    a.edge_pad = edge_pad ? 1 : 0;
+    a.use_mask = mask_radius >= 0.0 ? 1 : 0;
+    a.mask_r2 = mask_radius * mask_radius;
+    a.mask_c = ncol / 2;
+    a.apply_scale = apply_scale ? 1 : 0;
+    a.pi32 = (float)M_PI;
+    a.denom = (float)(nrow - 1);
+    return ATH_OK;
+}
+
+}  // namespace
+}  // namespace ath
+
+using namespace ath;
+
+extern "C" {
+
+int ath_version(void) { return 100; }
+
+int ath_last_error(char* buf, size_t len) {
+    if (buf && len > 0) {
+        std::snprintf(buf, len, "%s", g_message.c_str());
+    }
+    return g_status;
+}
+
+int ath_device_count(int* count) {
+    clear_error();
+    if (!count) return set_error(ATH_ERR_INVALID_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        (void)hipGetLastError();
+        return set_error(ATH_ERR_NO_DEVICE,
+                         std::string("no HIP device: ") + hipGetErrorName(e));
+    }
+    *count = n;
+    return ATH_OK;
+}
+
+int ath_filter_fft_length(int ncol, int pad) {
+    clear_error();
+    if (ncol < 1 || pad < 0)
+        return set_error(ATH_ERR_INVALID_ARG, "ncol must be > 0, pad >= 0");
+    const int L = fft_length(ncol, pad);
+    if (L < 0 || L > 16384)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "ramp filter: padded width needs an FFT above 16384");
+    return L;
+}
+
+size_t ath_filter_plan_bytes(int fft_len, int nwin) {
+    if (fft_len < 64 || (fft_len & (fft_len - 1)) || nwin < 1) return 0;
+    return plan_bytes(fft_len, nwin);
+}
+
+int ath_filter_plan_build(const double* taps, int nwin, int width,
+                          int fft_len, void* plan, size_t plan_bytes_given,
+                          void* stream) {
+    clear_error();
+    if (!taps || !plan) return set_error(ATH_ERR_INVALID_ARG, "null pointer");
+    if (nwin < 1 || width < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "nwin and width must be > 0");
+    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)))
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "fft_len must be a power of two in [64, 16384]");
+    if (plan_bytes_given < plan_bytes(fft_len, nwin))
+        return set_error(ATH_ERR_WORKSPACE, "filter plan buffer too small");
+    return plan_build(taps, nwin, width, fft_len, plan,
+                      static_cast<hipStream_t>(stream));
+}
+
+int ath_filter_taps_from_window(const double* win_re, const double* win_im,
+                                int width, double* taps_out) {
+    clear_error();
+    if (!win_re || !win_im || !taps_out || width < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "bad arguments");
+    // R' = ifftshift(R); r' = ifft(R'); g = Re(ifftshift(r'))
+    const int W = width, h = W / 2;
+    std::vector<long double> cw(W), sw(W);
+    const long double two_pi = 6.283185307179586476925286766559L;
+    for (int k = 0; k < W; ++k) {
+        cw[k] = std::cos(two_pi * (long double)k / (long double)W);
+        sw[k] = std::sin(two_pi * (long double)k / (long double)W);
+    }
+    for (int m = 0; m < W; ++m) {
+        const int n = (m + h) % W;  // ifftshift of the inverse transform
+        long double acc = 0.0L;
+        for (int k = 0; k < W; ++k) {
+            const int src = (k + h) % W;  // ifftshift of the window
+            const int t = (int)(((long long)k * n) % W);
+            acc += (long double)win_re[src] * cw[t] -
+                   (long double)win_im[src] * sw[t];
+        }
+        taps_out[m] = (double)(acc / (long double)W);
+    }
+    return ATH_OK;
+}
+
+int ath_ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
+                    int ncol, int pad, int pad_mode, int apply_log,
+                    const void* plan, int fft_len, int nwin, void* out,
+                    int out_dtype, void* stream) {
+    clear_error();
+    if (!sino || !plan || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    int st = check_geometry(nrow, nslice, ncol);
+    if (st != ATH_OK) return st;
+    if (pad < 0) return set_error(ATH_ERR_INVALID_ARG, "pad must be >= 0");
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+        return set_error(ATH_ERR_INVALID_ARG, "unsupported pad mode");
+    if (!valid_dtype(out_dtype) ||
+        (in_dtype != ATH_F32 && in_dtype != ATH_F64))
+        return set_error(ATH_ERR_INVALID_ARG, "bad dtype");
+    if (nwin != 1 && nwin != nrow)
+        return set_error(ATH_ERR_INVALID_ARG, "nwin must be 1 or nrow");
+    const long long width = (long long)ncol + 2LL * pad;
+    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)) ||
+        (long long)fft_len < 2 * width - pad)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "fft_len must be a power of two >= 2*(ncol+2*pad)-pad "
+                         "(see ath_filter_fft_length)");
+    return ramp_filter(sino, in_dtype, nrow, nslice, ncol, pad, pad_mode,
+                       apply_log, plan, fft_len, nwin, out, out_dtype,
+                       static_cast<hipStream_t>(stream));
+}
+
+int ath_back_projection(const void* sino, int sino_dtype, int nrow,
+                        int nslice, int ncol, const float* angles,
+                        float center, int edge_pad, int acc,
+                        double mask_radius, int apply_scale, float* recon,
+                        void* stream) {
+    clear_error();
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center,
+                          edge_pad, mask_radius, apply_scale, recon);
+    if (st != ATH_OK) return st;
+    return back_projection(a, sino_dtype, acc,
+                           static_cast<hipStream_t>(stream));
+}
+
+int ath_back_projection_naive(const float* sino, int nrow, int nslice,
+                              int ncol, const float* angles, float center,
+                              int edge_pad, int acc, double mask_radius,
+                              int apply_scale, float* recon, void* stream) {
+    clear_error();
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center,
+                          edge_pad, mask_radius, apply_scale, recon);
+    if (st != ATH_OK) return st;
+    if (nslice > 65535)
+        return set_error(ATH_ERR_UNSUPPORTED, "naive kernel: nslice > 65535");
+    return back_projection_naive(a, acc, static_cast<hipStream_t>(stream));
+}
+
+size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
+                               int nwin, int filt_dtype) {
+    if (nrow < 1 || nslice < 1 || ncol < 1 || pad < 0 || nwin < 1 ||
+        !valid_dtype(filt_dtype))
+        return 0;
+    const int L = fft_length(ncol, pad);
+    if (L < 0) return 0;
+    const size_t plan = plan_bytes(L, nwin);
+    const size_t plan_al = (plan + 255) & ~(size_t)255;
+    return plan_al + (size_t)nrow * nslice * ncol * dtype_bytes(filt_dtype);
+}
+
+int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
+                           int nslice, int ncol, const float* angles,
+                           float center, double mask_radius,
+                           const double* taps, int nwin, int pad,
+                           int pad_mode, int apply_log, int acc,
+                           int filt_dtype, void* workspace,
+                           size_t workspace_bytes, float* recon,
+                           void* stream) {
+    clear_error();
+    if (!workspace || !taps)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    const size_t need = ath_fbp_workspace_bytes(nrow, nslice, ncol, pad, nwin,
+                                                filt_dtype);
+    if (need == 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry or dtype");
+    if (workspace_bytes < need)
+        return set_error(ATH_ERR_WORKSPACE, "workspace too small");
+    if (filt_dtype != ATH_F32 && acc != ATH_ACC_F32)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "reduced-precision filtered storage needs "
+                         "acc = ATH_ACC_F32");
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center, 0,
+                          mask_radius, 1, recon);
+    if (st != ATH_OK) return st;
+    const int L = ath_filter_fft_length(ncol, pad);
+    if (L < 0) return L;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t plan = plan_bytes(L, nwin);
+    char* ws = static_cast<char*>(workspace);
+    void* filt = ws + ((plan + 255) & ~(size_t)255);
+    st = plan_build(taps, nwin, ncol + 2 * pad, L, ws, s);
+    if (st != ATH_OK) return st;
+    st = ath_ramp_filter(sino, in_dtype, nrow, nslice, ncol, pad, pad_mode,
+                         apply_log, ws, L, nwin, filt, filt_dtype, stream);
+    if (st != ATH_OK) return st;
+    a.sino = filt;
+    return back_projection(a, filt_dtype, acc, s);
+}
+
+}  // extern "C"

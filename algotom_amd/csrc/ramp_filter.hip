@@ -1,0 +1,408 @@
+// Ramp filter stage of FBP on gfx950.
+//
+// algotom's apply_ramp_filter (reconstruction.py:118-153) pads every row to
+// W = ncol + 2*pad columns, multiplies its shifted spectrum by the ramp
+// window and keeps the real part of the shifted inverse.  That is exactly a
+// length-W circular convolution of the padded row with a real kernel g
+// (g = Re(ifftshift(ifft(ifftshift(R)))); W is mostly 4 x prime here, so no
+// fast length-W transform exists).  This kernel evaluates it exactly as a
+// LINEAR convolution with the doubled kernel [g, g] on a power-of-two
+// length L >= 2W - pad, where the outputs W+pad .. W+pad+ncol-1 are free of
+// wrap-around.  Everything is float64: one workgroup owns one row, keeps the
+// row as L/2 packed complex samples (x[2n] + i x[2n+1]) in LDS, runs a
+// radix-4 Stockham FFT, multiplies by the kernel's half spectrum (the real
+// FFT split and the inverse merge fused in one pass) and runs the inverse.
+// The pass is HBM-bound: one read and one write of every sample.
+#include "ath_internal.h"
+
+namespace ath {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 conjg(double2 a) {
+    return make_double2(a.x, -a.y);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) {
+    return make_double2(a.x * s, a.y * s);
+}
+// multiply by -i (forward) or +i (inverse)
+template <int DIR>
+__device__ __forceinline__ double2 rot(double2 a) {
+    return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+}
+
+// In-LDS complex FFT of length M = 2^m, natural order in and out (Stockham
+// autosort: each pass reads all operands into registers, barriers, then
+// writes them to their sorted place in the same buffer).  DIR = -1 is the
+// forward transform, +1 the unscaled inverse.  tw[n] = exp(-2 pi i n / L),
+// L = 2M.  Each of the 256 threads handles up to MAXB butterflies a pass.
+template <int DIR, int MAXB>
+__device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw) {
+    const int tid = threadIdx.x;
+    const int L = 2 * M;
+    int ns = 1;
+    while (ns * 4 <= M) {
+        const int nb = M >> 2;
+        const int tstep = L / (ns * 4);
+        double2 v[MAXB][4];
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int j = tid + b * kThreads;
+            if (j < nb) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * nb];
+                const int k = j & (ns - 1);
+                if (k != 0) {
+#pragma unroll
+                    for (int r = 1; r < 4; ++r) {
+                        double2 w = tw[r * k * tstep];
+                        if (DIR > 0) w = conjg(w);
+                        v[b][r] = cmul(v[b][r], w);
+                    }
+                }
+                const double2 a0 = cadd(v[b][0], v[b][2]);
+                const double2 a1 = csub(v[b][0], v[b][2]);
+                const double2 a2 = cadd(v[b][1], v[b][3]);
+                const double2 a3 = rot<DIR>(csub(v[b][1], v[b][3]));
+                v[b][0] = cadd(a0, a2);
+                v[b][1] = cadd(a1, a3);
+                v[b][2] = csub(a0, a2);
+                v[b][3] = csub(a1, a3);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int j = tid + b * kThreads;
+            if (j < nb) {
+                const int k = j & (ns - 1);
+                const int base = (j - k) * 4 + k;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) buf[base + r * ns] = v[b][r];
+            }
+        }
+        __syncthreads();
+        ns *= 4;
+    }
+    if (ns < M) {  // one radix-2 pass, ns = M / 2
+        const int nb = M >> 1;
+        double2 v[2 * MAXB][2];
+#pragma unroll
+        for (int b = 0; b < 2 * MAXB; ++b) {
+            const int j = tid + b * kThreads;
+            if (j < nb) {
+                v[b][0] = buf[j];
+                double2 x1 = buf[j + nb];
+                if (j != 0) {
+                    double2 w = tw[2 * j];
+                    if (DIR > 0) w = conjg(w);
+                    x1 = cmul(x1, w);
+                }
+                v[b][1] = csub(v[b][0], x1);
+                v[b][0] = cadd(v[b][0], x1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 2 * MAXB; ++b) {
+            const int j = tid + b * kThreads;
+            if (j < nb) {
+                buf[j] = v[b][0];
+                buf[j + ns] = v[b][1];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Z = FFT_M(x[2n] + i x[2n+1]) -> X[k] = rfft_L(x)[k] for k and M-k.
+__device__ __forceinline__ void split_real(double2 zk, double2 zm, double2 wk,
+                                           double2& xk, double2& xm) {
+    const double2 e = cscale(cadd(zk, conjg(zm)), 0.5);
+    const double2 o = rot<-1>(cscale(csub(zk, conjg(zm)), 0.5));
+    const double2 wo = cmul(wk, o);
+    xk = cadd(e, wo);
+    xm = conjg(csub(e, wo));
+}
+
+template <typename Tin>
+__device__ __forceinline__ double load_sample(const Tin* row, int src,
+                                              int apply_log);
+template <>
+__device__ __forceinline__ double load_sample<float>(const float* row, int src,
+                                                     int apply_log) {
+    const float x = row[src];
+    // -np.log of a float32 array is float32 (reconstruction.py:436)
+    return apply_log ? (double)(float)(-log((double)x)) : (double)x;
+}
+template <>
+__device__ __forceinline__ double load_sample<double>(const double* row,
+                                                      int src, int apply_log) {
+    const double x = row[src];
+    return apply_log ? -log(x) : x;
+}
+
+// One workgroup per row (angle i, slice j) of a (nrow, nslice, ncol) stack.
+template <int MAXB, typename Tin, typename Tout>
+__global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
+    const Tin* __restrict__ sino, Tout* __restrict__ out, int nslice,
+    int ncol, int pad, int pad_mode, int apply_log,
+    const double2* __restrict__ tw, const double2* __restrict__ spec,
+    int nwin, int M) {
+    extern __shared__ double2 buf[];
+    const int tid = threadIdx.x;
+    const size_t row = blockIdx.x;
+    const int width = ncol + 2 * pad;
+    const Tin* src_row = sino + row * (size_t)ncol;
+    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+
+    // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
+    for (int n = tid; n < M; n += kThreads) {
+        double v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * n + h;
+            v[h] = 0.0;
+            if (k < width) {
+                const int s = pad_source(k - pad, ncol, pad_mode);
+                if (s >= 0) v[h] = load_sample<Tin>(src_row, s, apply_log);
+            }
+        }
+        buf[n] = make_double2(v[0], v[1]);
+    }
+    __syncthreads();
+    lds_fft<-1, MAXB>(buf, M, tw);
+
+    // rfft split, multiply by the kernel spectrum, merge for the inverse
+    for (int k = tid; k <= M / 2; k += kThreads) {
+        const int km = (M - k) & (M - 1);
+        const double2 wk = tw[k];
+        double2 xk, xm;
+        split_real(buf[k], buf[km], wk, xk, xm);
+        const double2 yk = cmul(xk, G[k]);
+        const double2 ym = cmul(xm, G[M - k]);
+        // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
+        const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
+        const double2 ok = rot<1>(cmul(conjg(wk),
+                                       cscale(csub(yk, conjg(ym)), 0.5)));
+        if (k == 0 || k == km) {
+            buf[k] = cadd(ek, ok);
+        } else {
+            // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
+            const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
+            const double2 om = rot<-1>(cmul(wk,
+                                            cscale(csub(ym, conjg(yk)), 0.5)));
+            buf[k] = cadd(ek, ok);
+            buf[km] = cadd(em, om);
+        }
+    }
+    __syncthreads();
+    lds_fft<1, MAXB>(buf, M, tw);
+
+    const double* y = reinterpret_cast<const double*>(buf);
+    Tout* dst = out + row * (size_t)ncol;
+    const int off = width + pad;
+    for (int c = tid; c < ncol; c += kThreads) store_elem(dst + c, y[off + c]);
+}
+
+// tw[n] = exp(-2 pi i n / L)
+__global__ void twiddle_kernel(double2* tw, int L) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < L) {
+        double s, c;
+        sincospi(2.0 * (double)n / (double)L, &s, &c);
+        tw[n] = make_double2(c, -s);
+    }
+}
+
+// spec[r][k] = rfft_L([g_r, g_r, 0...])[k] / M, k = 0..M, one block per row.
+template <int MAXB>
+__global__ __launch_bounds__(kThreads) void kernel_spectrum_kernel(
+    const double* __restrict__ taps, int width, const double2* __restrict__ tw,
+    double2* __restrict__ spec, int M) {
+    extern __shared__ double2 buf[];
+    const int tid = threadIdx.x;
+    const double* g = taps + (size_t)blockIdx.x * width;
+    double2* G = spec + (size_t)blockIdx.x * (M + 1);
+    for (int n = tid; n < M; n += kThreads) {
+        double v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * n + h;
+            v[h] = k < 2 * width ? g[k < width ? k : k - width] : 0.0;
+        }
+        buf[n] = make_double2(v[0], v[1]);
+    }
+    __syncthreads();
+    lds_fft<-1, MAXB>(buf, M, tw);
+    const double inv = 1.0 / (double)M;
+    for (int k = tid; k <= M / 2; k += kThreads) {
+        const int km = (M - k) & (M - 1);
+        double2 xk, xm;
+        split_real(buf[k], buf[km], tw[k], xk, xm);
+        G[k] = cscale(xk, inv);
+        G[M - k] = cscale(xm, inv);
+    }
+}
+
+int maxb_for(int M) {
+    if (M <= 1024) return 1;
+    if (M <= 2048) return 2;
+    if (M <= 4096) return 4;
+    if (M <= 8192) return 8;
+    return 0;
+}
+
+template <typename K>
+int prepare_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(kernel),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+        }
+    }
+    return ATH_OK;
+}
+
+template <int MAXB, typename Tin, typename Tout>
+int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
+                  int pad, int pad_mode, int apply_log, const void* plan,
+                  int fft_len, int nwin, hipStream_t stream) {
+    const int M = fft_len / 2;
+    const double2* tw = static_cast<const double2*>(plan);
+    const double2* spec = tw + fft_len;
+    const size_t lds = (size_t)M * sizeof(double2);
+    auto kern = ramp_filter_kernel<MAXB, Tin, Tout>;
+    prepare_lds(kern, lds);
+    const size_t rows = (size_t)nrow * nslice;
+    hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kThreads), lds, stream,
+                       static_cast<const Tin*>(sino), static_cast<Tout*>(out),
+                       nslice, ncol, pad, pad_mode, apply_log, tw, spec, nwin,
+                       M);
+    return check_hip(hipGetLastError(), "ramp_filter_kernel launch");
+}
+
+template <int MAXB, typename Tin>
+int dispatch_out(int out_dtype, const void* sino, void* out, int nrow,
+                 int nslice, int ncol, int pad, int pad_mode, int apply_log,
+                 const void* plan, int fft_len, int nwin, hipStream_t s) {
+    switch (out_dtype) {
+    case ATH_F32:
+        return launch_filter<MAXB, Tin, float>(sino, out, nrow, nslice, ncol,
+                                               pad, pad_mode, apply_log, plan,
+                                               fft_len, nwin, s);
+    case ATH_F64:
+        return launch_filter<MAXB, Tin, double>(sino, out, nrow, nslice, ncol,
+                                                pad, pad_mode, apply_log, plan,
+                                                fft_len, nwin, s);
+    case ATH_F16:
+        return launch_filter<MAXB, Tin, __half>(sino, out, nrow, nslice, ncol,
+                                                pad, pad_mode, apply_log, plan,
+                                                fft_len, nwin, s);
+    case ATH_BF16:
+        return launch_filter<MAXB, Tin, __hip_bfloat16>(
+            sino, out, nrow, nslice, ncol, pad, pad_mode, apply_log, plan,
+            fft_len, nwin, s);
+    }
+    return set_error(ATH_ERR_INVALID_ARG, "ramp filter: bad output dtype");
+}
+
+template <int MAXB>
+int dispatch_in(int in_dtype, int out_dtype, const void* sino, void* out,
+                int nrow, int nslice, int ncol, int pad, int pad_mode,
+                int apply_log, const void* plan, int fft_len, int nwin,
+                hipStream_t s) {
+    if (in_dtype == ATH_F32)
+        return dispatch_out<MAXB, float>(out_dtype, sino, out, nrow, nslice,
+                                         ncol, pad, pad_mode, apply_log, plan,
+                                         fft_len, nwin, s);
+    if (in_dtype == ATH_F64)
+        return dispatch_out<MAXB, double>(out_dtype, sino, out, nrow, nslice,
+                                          ncol, pad, pad_mode, apply_log, plan,
+                                          fft_len, nwin, s);
+    return set_error(ATH_ERR_INVALID_ARG,
+                     "ramp filter: input dtype must be float32 or float64");
+}
+
+template <int MAXB>
+int launch_spectrum(const double* taps, int nwin, int width,
+                    const double2* tw, double2* spec, int M,
+                    hipStream_t stream) {
+    const size_t lds = (size_t)M * sizeof(double2);
+    auto kern = kernel_spectrum_kernel<MAXB>;
+    prepare_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwin), dim3(kThreads), lds,
+                       stream, taps, width, tw, spec, M);
+    return check_hip(hipGetLastError(), "kernel_spectrum_kernel launch");
+}
+
+}  // namespace
+
+int fft_length(int ncol, int pad) {
+    const long long width = (long long)ncol + 2LL * pad;
+    const long long need = 2 * width - pad;
+    long long L = 64;
+    while (L < need) L <<= 1;
+    return L > (1 << 30) ? -1 : (int)L;
+}
+
+size_t plan_bytes(int fft_len, int nwin) {
+    const size_t M = (size_t)fft_len / 2;
+    return sizeof(double2) * ((size_t)fft_len + (size_t)nwin * (M + 1));
+}
+
+int plan_build(const double* taps, int nwin, int width, int fft_len,
+               void* plan, hipStream_t stream) {
+    const int M = fft_len / 2;
+    double2* tw = static_cast<double2*>(plan);
+    double2* spec = tw + fft_len;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((fft_len + 255) / 256), dim3(256),
+                       0, stream, tw, fft_len);
+    int st = check_hip(hipGetLastError(), "twiddle_kernel launch");
+    if (st != ATH_OK) return st;
+    switch (maxb_for(M)) {
+    case 1: return launch_spectrum<1>(taps, nwin, width, tw, spec, M, stream);
+    case 2: return launch_spectrum<2>(taps, nwin, width, tw, spec, M, stream);
+    case 4: return launch_spectrum<4>(taps, nwin, width, tw, spec, M, stream);
+    case 8: return launch_spectrum<8>(taps, nwin, width, tw, spec, M, stream);
+    }
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "ramp filter: FFT length above 16384 is not supported");
+}
+
+int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
+                int ncol, int pad, int pad_mode, int apply_log,
+                const void* plan, int fft_len, int nwin, void* out,
+                int out_dtype, hipStream_t s) {
+    const int M = fft_len / 2;
+    switch (maxb_for(M)) {
+    case 1: return dispatch_in<1>(in_dtype, out_dtype, sino, out, nrow, nslice,
+                                  ncol, pad, pad_mode, apply_log, plan,
+                                  fft_len, nwin, s);
+    case 2: return dispatch_in<2>(in_dtype, out_dtype, sino, out, nrow, nslice,
+                                  ncol, pad, pad_mode, apply_log, plan,
+                                  fft_len, nwin, s);
+    case 4: return dispatch_in<4>(in_dtype, out_dtype, sino, out, nrow, nslice,
+                                  ncol, pad, pad_mode, apply_log, plan,
+                                  fft_len, nwin, s);
+    case 8: return dispatch_in<8>(in_dtype, out_dtype, sino, out, nrow, nslice,
+                                  ncol, pad, pad_mode, apply_log, plan,
+                                  fft_len, nwin, s);
+    }
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "ramp filter: FFT length above 16384 is not supported");
+}
+
+}  // namespace ath

@@ -1,0 +1,427 @@
+"""Filtered back-projection on AMD Instinct MI355X -- drop-in for algotom's
+``algotom.rec.reconstruction`` FBP functions (algotom 1.6.0).
+
+Same names, arguments, defaults, return shapes, warnings and errors as the
+reference; the filter and back-projection run as hand-written HIP kernels
+(libalgotom_hip.so) and nothing falls back to the CPU: without a HIP device
+every compute entry raises ``RuntimeError``.
+
+* numpy input gives numpy output (host copies in and out, as the reference's
+  numba launches do); a torch tensor on the GPU stays on the GPU and a
+  tensor comes back.
+* ``gpu`` keeps its meaning of "which reference kernel": ``gpu=True`` on a
+  single sinogram reproduces ``__back_projection_gpu_kernel`` (float64
+  accumulation, reconstruction.py:156-190); a stack with ``gpu=True``
+  reproduces ``__back_projection_gpu_chunk_kernel`` (:229-264) and
+  ``gpu=False`` reproduces ``back_projection_cpu`` (:307-359), both float32
+  storage after every angle.  All of them run on the GPU.
+* ``block`` and ``ncore`` are accepted and ignored: tiles are fixed by the
+  kernel and there is no CPU worker pool.
+* keyword-only ``precision`` (not in the reference): ``None`` keeps the
+  reference arithmetic; ``"fp32"`` interpolates and accumulates in float32
+  (float64 detector positions); ``"fp16"`` / ``"bf16"`` also store the
+  filtered sinogram in 16 bits.
+"""
+
+import functools
+import math
+import warnings
+
+import numpy as np
+from scipy.signal import windows as _windows
+
+from algotom_amd import _native as nat
+
+_WINDOWS = ("hann", "bartlett", "blackman", "hamming", "nuttall", "parzen",
+            "triang")
+
+
+# --------------------------------------------------------------------------
+# windows (host, float64; reconstruction.py:50-115)
+# --------------------------------------------------------------------------
+def make_smoothing_window(filter_name, width):
+    """Make a 1d smoothing window (reconstruction.py:50-83).
+
+    Parameters
+    ----------
+    filter_name : {"hann", "bartlett", "blackman", "hamming", "nuttall",
+                   "parzen", "triang"}
+        Window function; anything else gives ones.
+    width : int
+        Width of the window.
+
+    Returns
+    -------
+    array_like
+        1D float64 array (scipy.signal.windows, symmetric).
+    """
+    if filter_name in _WINDOWS:
+        return getattr(_windows, filter_name)(width)
+    return np.ones(width)
+
+
+def _ramp_kernel(width):
+    # discrete Ram-Lak on n = k - ceil((W-1)/2): 1/4 at 0, 0 at even n,
+    # -1/(n pi)^2 at odd n (reconstruction.py:106-111)
+    n = np.arange(0.0, width) - np.ceil((width - 1.0) / 2)
+    h = np.zeros(width)
+    h[n == 0.0] = 0.25
+    odd = np.mod(n, 2) == 1.0
+    h[odd] = -1.0 / (n[odd] * np.pi) ** 2
+    return h
+
+
+@functools.lru_cache(maxsize=64)
+def _ramp_row(width, filter_name):
+    row = np.fft.fftshift(np.fft.fft(_ramp_kernel(width)))
+    row = row * make_smoothing_window(filter_name, width)
+    row.setflags(write=False)
+    return row
+
+
+def make_2d_ramp_window(height, width, filter_name=None):
+    """Make the 2d ramp window in Fourier space (reconstruction.py:86-115).
+
+    Parameters
+    ----------
+    height : int
+        Height of the window.
+    width : int
+        Width of the window.
+    filter_name : {None, "hann", "bartlett", "blackman", "hamming",
+                   "nuttall", "parzen", "triang"}
+        Name of a smoothing window used.
+
+    Returns
+    -------
+    complex ndarray
+        2D array, every row the windowed ramp spectrum (fftshift order).
+    """
+    return np.tile(_ramp_row(int(width), filter_name), (height, 1))
+
+
+def _taps(rows):
+    """Real circular-convolution taps equivalent to multiplying the shifted
+    spectrum by ``rows`` and keeping the real part (reconstruction.py
+    :150-152): Re(ifftshift(ifft(ifftshift(R)))) per row."""
+    rows = np.asarray(rows)
+    g = np.fft.ifftshift(np.fft.ifft(np.fft.ifftshift(rows, axes=-1),
+                                     axis=-1), axes=-1)
+    return np.ascontiguousarray(np.real(g), dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# device plumbing
+# --------------------------------------------------------------------------
+def _is_tensor(x):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def _to_device(x, dev, allow_f64=True):
+    """Contiguous float32/float64 tensor on ``dev`` (no copy if already so)."""
+    import torch
+    if _is_tensor(x):
+        t = x
+    else:
+        a = np.asarray(x)
+        if a.dtype not in (np.float32, np.float64):
+            a = a.astype(np.float32)
+        t = torch.from_numpy(np.ascontiguousarray(a))
+    if t.dtype not in (torch.float32, torch.float64) or (
+            t.dtype == torch.float64 and not allow_f64):
+        t = t.to(torch.float32)
+    return t.to(dev).contiguous()
+
+
+def _angles_device(angles, dev):
+    import torch
+    if _is_tensor(angles):
+        return angles.detach().to(dev, torch.float32).contiguous()
+    a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64)
+                             .astype(np.float32))
+    return torch.from_numpy(a).to(dev)
+
+
+_plan_cache = {}
+
+
+def _filter_plan(taps, dev, fft_len):
+    """Device filter plan for ``taps`` (nwin, W) on FFT length ``fft_len``;
+    cached per device."""
+    import torch
+    nwin, width = taps.shape
+    key = (dev.index, width, fft_len, taps.tobytes())
+    hit = _plan_cache.get(key)
+    if hit is not None:
+        return hit
+    lib = nat.load()
+    nbytes = lib.ath_filter_plan_bytes(fft_len, nwin)
+    if nbytes == 0:
+        raise ValueError("ramp filter width %d is too large" % width)
+    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    taps_d = torch.from_numpy(taps).to(dev)
+    nat.check(lib.ath_filter_plan_build(nat.ptr(taps_d), nwin, width, fft_len,
+                                        nat.ptr(plan), nbytes,
+                                        nat.stream_ptr(dev)),
+              "ath_filter_plan_build")
+    if len(_plan_cache) > 32:
+        _plan_cache.clear()
+    entry = (plan, nwin, taps_d)
+    _plan_cache[key] = entry
+    return entry
+
+
+def _window_taps(ramp_win, nrow, width, filter_name):
+    """Taps (nwin, width) for the window the reference would use."""
+    if ramp_win is None or np.shape(ramp_win) != (nrow, width):
+        # reconstruction.py:148-149: regenerate when the shape mismatches
+        return _taps(_ramp_row(width, filter_name))[None, :]
+    win = np.asarray(ramp_win)
+    if np.all(win == win[0:1]):
+        return _taps(win[0])[None, :]
+    return _taps(win)
+
+
+def _clamp_nonpositive(sinogram):
+    """reconstruction.py:431-435: warn and set values <= 0 to 1, in place."""
+    if _is_tensor(sinogram):
+        bad = sinogram <= 0.0
+        if bool(bad.any()):
+            warnings.warn("!!!Applying logarithm is enabled but "
+                          "there are values <= 0.0 in the data!!!")
+            sinogram[bad] = 1.0
+    elif np.any(sinogram <= 0.0):
+        warnings.warn("!!!Applying logarithm is enabled but "
+                      "there are values <= 0.0 in the data!!!")
+        sinogram[sinogram <= 0.0] = np.float32(1.0)
+
+
+def _filter_stack(sino_t, pad, pad_mode, apply_log, taps, out_dtype):
+    """Run the ramp-filter kernel on a (nrow, nslice, ncol) device tensor."""
+    import torch
+    if pad_mode not in nat.PAD_MODES:
+        raise NotImplementedError(
+            "pad_mode %r is not supported by the GPU filter (supported: %s)"
+            % (pad_mode, ", ".join(nat.PAD_MODES)))
+    dev = sino_t.device
+    nrow, nslice, ncol = sino_t.shape
+    lib = nat.load()
+    fft_len = nat.check(lib.ath_filter_fft_length(ncol, pad),
+                        "ath_filter_fft_length")
+    plan, nwin, _ = _filter_plan(taps, dev, fft_len)
+    out = torch.empty((nrow, nslice, ncol), dtype=out_dtype, device=dev)
+    nat.check(lib.ath_ramp_filter(nat.ptr(sino_t), nat.dtype_code(sino_t.dtype),
+                                  nrow, nslice, ncol, int(pad),
+                                  nat.PAD_MODES[pad_mode], int(bool(apply_log)),
+                                  nat.ptr(plan), fft_len, nwin, nat.ptr(out),
+                                  nat.dtype_code(out_dtype),
+                                  nat.stream_ptr(dev)),
+              "ath_ramp_filter")
+    return out
+
+
+def _backproject_stack(filt_t, angles, center, acc, edge_pad=False,
+                       mask_radius=-1.0, apply_scale=False):
+    import torch
+    dev = filt_t.device
+    nrow, nslice, ncol = filt_t.shape
+    ang = _angles_device(angles, dev)
+    if ang.numel() != nrow:
+        raise ValueError("!!!Number of angles is not the same as the row "
+                         "number of the sinogram!!!")
+    out = torch.empty((ncol, nslice, ncol), dtype=torch.float32, device=dev)
+    lib = nat.load()
+    nat.check(lib.ath_back_projection(nat.ptr(filt_t),
+                                      nat.dtype_code(filt_t.dtype), nrow,
+                                      nslice, ncol, nat.ptr(ang),
+                                      float(np.float32(center)),
+                                      int(bool(edge_pad)), int(acc),
+                                      float(mask_radius), int(bool(apply_scale)),
+                                      nat.ptr(out), nat.stream_ptr(dev)),
+              "ath_back_projection")
+    return out
+
+
+def _finish(out_t, as_numpy):
+    if as_numpy:
+        return out_t.cpu().numpy()
+    return out_t
+
+
+_PRECISIONS = {None: None, "fp32": "fp32", "fp16": "fp16", "bf16": "bf16"}
+
+
+def _storage_dtype(precision):
+    import torch
+    return {"fp16": torch.float16, "bf16": torch.bfloat16}.get(
+        precision, torch.float32)
+
+
+# --------------------------------------------------------------------------
+# public API
+# --------------------------------------------------------------------------
+def apply_ramp_filter(sinogram, ramp_win=None, filter_name=None, pad=None,
+                      pad_mode="edge"):
+    """Apply the ramp filter to a sinogram (reconstruction.py:118-153).
+
+    Parameters
+    ----------
+    sinogram : array_like
+        2D array (nrow, ncol). A 3D (nrow, nslice, ncol) stack is filtered
+        row by row as well.
+    ramp_win : complex ndarray or None
+        Ramp window in the Fourier space, shape (nrow, ncol + 2*pad);
+        regenerated if None or of another shape.
+    filter_name : {None, "hann", "bartlett", "blackman", "hamming",
+                   "nuttall", "parzen", "triang"}
+        Name of a smoothing window used.
+    pad : int or None
+        Padding before the FFT; min(int(0.15*ncol), 150) if None.
+    pad_mode : str
+        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap".
+
+    Returns
+    -------
+    array_like
+        Filtered sinogram, float64, same shape as the input.
+    """
+    import torch
+    as_numpy = not _is_tensor(sinogram)
+    dev = nat.device()
+    shape = tuple(sinogram.shape)
+    nrow, ncol = shape[0], shape[-1]
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    width = ncol + 2 * pad
+    taps = _window_taps(ramp_win, nrow, width, filter_name)
+    sino_t = _to_device(sinogram, dev)
+    stack = sino_t.reshape(nrow, -1, ncol)
+    out = _filter_stack(stack, pad, pad_mode, False, taps, torch.float64)
+    return _finish(out.reshape(shape), as_numpy)
+
+
+def back_projection_gpu(sinogram, angles, center, block=(16, 16),
+                        edge_pad=False):
+    """Back-projection of one sinogram with float64 accumulation
+    (reconstruction.py:156-226).  Returns (ncol, ncol) float32."""
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinogram)
+    nrow, ncol = tuple(sinogram.shape)
+    t = _to_device(sinogram, dev, allow_f64=False).reshape(nrow, 1, ncol)
+    out = _backproject_stack(t, angles, center, nat.ACC_F64, edge_pad)
+    return _finish(out.reshape(ncol, ncol), as_numpy)
+
+
+def back_projection_gpu_chunk(sinograms, angles, center, block=(16, 16),
+                              edge_pad=False):
+    """Back-projection of a (nrow, nslice, ncol) stack with float32 storage
+    after every angle (reconstruction.py:229-304).  Returns
+    (ncol, nslice, ncol) float32."""
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinograms)
+    t = _to_device(sinograms, dev, allow_f64=False)
+    out = _backproject_stack(t, angles, center, nat.ACC_F32SEQ, edge_pad)
+    return _finish(out, as_numpy)
+
+
+def back_projection_cpu(sinogram, angles, center, edge_pad=False):
+    """The arithmetic of the reference's CPU back-projector
+    (reconstruction.py:307-359: float32 storage after every angle), computed
+    on the GPU without its prange race.  Returns (ncol, ncol) float32."""
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinogram)
+    nrow, ncol = tuple(sinogram.shape)
+    t = _to_device(sinogram, dev, allow_f64=False).reshape(nrow, 1, ncol)
+    out = _backproject_stack(t, angles, center, nat.ACC_F32SEQ, edge_pad)
+    return _finish(out.reshape(ncol, ncol), as_numpy)
+
+
+def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
+                       ramp_win=None, filter_name="hann", pad=None,
+                       pad_mode="edge", apply_log=True, gpu=True,
+                       block=(16, 16), ncore=None, *, precision=None):
+    """Apply the FBP (filtered back-projection) reconstruction method to a
+    sinogram-image or a chunk of sinogram-images (reconstruction.py:362-493).
+    Angular axis is 0; for 3D input the slicing axis is 1.
+
+    Parameters
+    ----------
+    sinogram : array_like
+        2D/3D array. Sinogram image(s); numpy or a torch tensor.
+    center : float
+        Center of rotation.
+    angles : array_like, optional
+        1D array. List of angles (in radian) corresponding to the sinogram.
+    ratio : float, optional
+        To apply a circle mask to the reconstructed image.
+    ramp_win : complex ndarray, optional
+        Ramp window in the Fourier space. Generated if None.
+    filter_name : {None, "hann", "bartlett", "blackman", "hamming",
+                  "nuttall", "parzen", "triang"}
+        Apply a smoothing filter.
+    pad : int, optional
+        To apply padding before the FFT; min(int(0.15*ncol), 150) if None.
+    pad_mode : str, optional
+        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap".
+    apply_log : bool, optional
+        Apply the logarithm function to the sinogram before reconstruction.
+    gpu : bool, optional
+        Which reference back-projector's arithmetic to reproduce (see the
+        module docstring); the computation always runs on the GPU.
+    block : tuple of two integer-values, optional
+        Accepted for compatibility; ignored.
+    ncore : int or None
+        Accepted for compatibility; ignored.
+    precision : {None, "fp32", "fp16", "bf16"}, keyword only
+        None keeps the reference arithmetic.
+
+    Returns
+    -------
+    array_like
+        Square array(s): (ncol, ncol) for 2D input, (ncol, nslice, ncol)
+        for 3D input, float32.
+    """
+    if precision not in _PRECISIONS:
+        raise ValueError("precision must be None, 'fp32', 'fp16' or 'bf16'")
+    shape = tuple(sinogram.shape)
+    input_3d = len(shape) == 3
+    if input_3d:
+        (nrow, num_sino, ncol) = shape
+    else:
+        num_sino = 1
+        (nrow, ncol) = shape
+    if center < 0 or center >= ncol:
+        raise ValueError("Center is out of the range [0, {}]".format(ncol - 1))
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    elif len(angles) != nrow:
+        raise ValueError("!!!Number of angles is not the same as the row "
+                         "number of the sinogram!!!")
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinogram)
+    if apply_log:
+        _clamp_nonpositive(sinogram)
+    mask_radius = -1.0
+    if ratio is not None:
+        if ratio == 0.0:
+            ratio = min(center, ncol - center) / (0.5 * ncol)
+        mask_radius = float(ratio * (ncol // 2))
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    taps = _window_taps(ramp_win, nrow, ncol + 2 * pad, filter_name)
+    if precision is None:
+        acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ
+    else:
+        acc = nat.ACC_F32
+    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
+    filt = _filter_stack(sino_t, pad, pad_mode, apply_log, taps,
+                         _storage_dtype(precision))
+    out = _backproject_stack(filt, angles, center, acc,
+                             mask_radius=mask_radius, apply_scale=True)
+    if not input_3d:
+        out = out.reshape(ncol, ncol)
+    return _finish(out, as_numpy)

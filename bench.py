@@ -1,0 +1,173 @@
+"""Benchmark: FBP of a 64 x 1801 x 2560 sinogram stack on one MI355X.
+
+Workload (the reference's stack case): 64 seeded random-ellipse sinograms,
+1801 angles over [0, pi), 2560 detector columns, transmission
+Poisson(1e4 exp(-k p)) / 1e4 with max(k p) = 3, apply_log=True, centre
+1280.37, hann window, edge padding, circle mask, output 64 x 2560 x 2560
+float32.  One step = one fbp_reconstruction call on the device-resident
+stack (log, pad, ramp filter, back-projection, mask, scale).
+
+value = pixel-angle updates per second (GUPS = 1e9 updates/s),
+U = 64 * 2560^2 * 1801 = 7.554e11 updates per step.
+
+roofline (back-projection kernel, the dominant one): algorithmic bytes per
+launch = U * 8 B (two float32 taps per update, the gather-byte model) over
+the kernel's mean duration from HIP events on the launch stream; peak = 8 TB/s.
+cpu_baseline: the oracle's OpenMP C port (numba-typed restatement of the
+reference's CPU path) on 4 of the 64 sinograms, on this host's cores.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NSLICE, NROW, NCOL, CENTER = 64, 1801, 2560, 1280.37
+PEAK_HBM_GBS = 8000.0
+METRIC = ("GUPS (pixel·angle updates/s) FBP 2560²×1801 ang; "
+          "% HBM roofline; grad steps/s")
+
+
+def make_stack(dev, nslice=NSLICE, seed0=1000):
+    import torch
+    from algotom_amd.util import phantom
+    angles = np.linspace(0.0, np.pi, NROW, endpoint=False)
+    sino = torch.empty((NROW, nslice, NCOL), dtype=torch.float32, device=dev)
+    for j in range(nslice):
+        prm = phantom.random_ellipses(NCOL, 20, seed=seed0 + j)
+        p = phantom.ellipse_sinogram(prm, angles, NCOL, np.float32(CENTER),
+                                     device=dev)
+        sino[:, j, :] = phantom.transmission(p, seed=seed0 + j)
+    torch.cuda.synchronize()
+    return sino, angles
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "bp_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(sino_dev, angles, gpu_out, nsample=4):
+    """Times the oracle C port on nsample slices; also checks parity of the
+    GPU result on those slices."""
+    from oracle import port
+    sample = np.ascontiguousarray(sino_dev[:, :nsample, :].cpu().numpy())
+    threads = port.default_threads()
+    port.fbp(sample[:, :1, :], CENTER, angles, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    ref = port.fbp(sample, CENTER, angles, nthreads=threads)
+    dt = time.perf_counter() - t0
+    updates = nsample * NCOL * NCOL * NROW
+    got = gpu_out[:, :nsample, :].cpu().numpy()
+    err = float(np.abs(got.astype(np.float64) - ref).max()
+                / np.abs(ref).max())
+    return {"value": updates / dt / 1e9, "unit": "GUPS", "cores": threads,
+            "kind": "port",
+            "sample": "%d of %d sinograms (%dx%d), log+filter+BP+mask+scale, "
+                      "%.2f s" % (nsample, NSLICE, NROW, NCOL, dt)}, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default=None,
+                    choices=[None, "fp32", "fp16", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs no multi-GPU "
+              "path (slices shard with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from algotom_amd import _native as nat
+    from algotom_amd.rec import reconstruction as rec
+
+    dev = nat.device()
+    sino, angles = make_stack(dev)
+    updates = NSLICE * NCOL * NCOL * NROW
+    pad = min(int(0.15 * NCOL), 150)
+    taps = rec._window_taps(None, NROW, NCOL + 2 * pad, "hann")
+    acc = nat.ACC_F32SEQ if args.precision is None else nat.ACC_F32
+    store = rec._storage_dtype(args.precision)
+    mask_radius = float(1.0 * (NCOL // 2))
+
+    def step(events=None):
+        # exactly fbp_reconstruction's device path, split so the
+        # back-projection kernel can be bracketed by events
+        rec._clamp_nonpositive(sino)
+        filt = rec._filter_stack(sino, pad, "edge", True, taps, store)
+        if events:
+            events[0].record()
+        out = rec._backproject_stack(filt, angles, CENTER, acc,
+                                     mask_radius=mask_radius,
+                                     apply_scale=True)
+        if events:
+            events[1].record()
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = step(ev[k])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    bp_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms = dt / args.steps * 1e3
+    achieved = updates * 8.0 / (bp_ms * 1e-3) / 1e9
+
+    base, err = (None, None)
+    if not args.no_cpu_baseline:
+        base, err = cpu_baseline(sino, angles, out)
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": updates / (ms * 1e-3) / 1e9,
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision is None else
+                 "%s-storage/fp32-acc" % args.precision,
+        "data": "synthetic",
+        "config": {"workload": "fbp_stack_64x1801x2560", "nslice": NSLICE,
+                   "nrow": NROW, "ncol": NCOL, "center": CENTER,
+                   "apply_log": True, "filter": "hann", "pad": pad,
+                   "precision": args.precision or "reference",
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": (traffic or {}).get("bytes_per_launch")},
+        "cpu_baseline": base,
+        "detail": {"bp_kernel_ms": bp_ms, "filter_and_rest_ms": ms - bp_ms,
+                   "bp_gups": updates / (bp_ms * 1e-3) / 1e9,
+                   "max_rel_err_vs_oracle_sample": err},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

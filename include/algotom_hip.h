@@ -1,0 +1,157 @@
+/*
+ * algotom_hip.h -- C ABI of the MI355X (gfx950) filtered back-projection path.
+ *
+ * Every entry point returns an int status (ATH_OK = 0, negative on error);
+ * the message of the last error on the calling thread is read with
+ * ath_last_error().  Nothing is thrown across the ABI.  All array arguments
+ * are DEVICE pointers owned by the caller; the library allocates nothing and
+ * copies nothing between host and device.  Work is enqueued on `stream`
+ * (a hipStream_t, NULL = the default stream) and is asynchronous.
+ *
+ * Layouts follow algotom's own: a sinogram stack is (nrow, nslice, ncol) =
+ * (angle, slice, detector column), C-contiguous; a 2D sinogram is the
+ * nslice = 1 case.  Reconstructions are (ncol, nslice, ncol) = (y, slice, x),
+ * i.e. algotom's recon[y, j, x] (reconstruction.py:477-481).
+ */
+#ifndef ALGOTOM_HIP_H
+#define ALGOTOM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ATH_OK 0
+#define ATH_ERR_INVALID_ARG (-1)
+#define ATH_ERR_HIP (-2)
+#define ATH_ERR_UNSUPPORTED (-3)
+#define ATH_ERR_NO_DEVICE (-4)
+#define ATH_ERR_WORKSPACE (-5)
+
+/* element types of sinogram / filtered-sinogram buffers */
+#define ATH_F32 0
+#define ATH_F64 1
+#define ATH_F16 2
+#define ATH_BF16 3
+
+/* numpy.pad modes applied inside the filter kernel */
+#define ATH_PAD_EDGE 0
+#define ATH_PAD_CONSTANT 1
+#define ATH_PAD_REFLECT 2
+#define ATH_PAD_SYMMETRIC 3
+#define ATH_PAD_WRAP 4
+
+/* back-projection accumulation (which reference kernel's arithmetic):
+ *  ATH_ACC_F64    float64 register sum, one rounding at the end
+ *                 (__back_projection_gpu_kernel, reconstruction.py:168,182)
+ *  ATH_ACC_F32SEQ float32 storage after every angle, float64 interpolation
+ *                 (__back_projection_gpu_chunk_kernel :256 and
+ *                  back_projection_cpu :351)
+ *  ATH_ACC_F32    float32 interpolation (one FMA) and float32 sum; detector
+ *                 position still float64 (throughput mode, not bit-exact) */
+#define ATH_ACC_F64 0
+#define ATH_ACC_F32SEQ 1
+#define ATH_ACC_F32 2
+
+/* Library version (major*10000 + minor*100 + patch). No reference
+ * counterpart. */
+int ath_version(void);
+
+/* Copies the calling thread's last error message (NUL-terminated, truncated
+ * to len) into buf; returns the status code of that error (ATH_OK if none).
+ * No reference counterpart: replaces Python exceptions raised by numba. */
+int ath_last_error(char* buf, size_t len);
+
+/* Number of visible HIP devices. Replaces numba's cuda.is_available()
+ * (reconstruction.py:420). */
+int ath_device_count(int* count);
+
+/* FFT length used by the ramp filter for ncol columns and `pad` padding:
+ * the smallest power of two L >= max(2*(ncol+2*pad) - pad, 64).  The filter
+ * evaluates the reference's length-W circular convolution exactly as a
+ * linear convolution with the doubled kernel [g, g] on this length. */
+int ath_filter_fft_length(int ncol, int pad);
+
+/* Device bytes of a filter plan holding `nwin` kernel spectra. */
+size_t ath_filter_plan_bytes(int fft_len, int nwin);
+
+/* Builds a filter plan on the device: twiddles and, for each of `nwin`
+ * rows of `taps` (nwin x width float64, device), the half spectrum of the
+ * doubled real kernel.  taps[r] = Re(ifftshift(ifft(ifftshift(R_r)))) for
+ * row r of algotom's ramp window R (make_2d_ramp_window,
+ * reconstruction.py:86-115); ath_filter_taps_from_window computes it.
+ * Replaces the per-call window product of apply_ramp_filter
+ * (reconstruction.py:148-152). */
+int ath_filter_plan_build(const double* taps, int nwin, int width,
+                          int fft_len, void* plan, size_t plan_bytes,
+                          void* stream);
+
+/* HOST helper: real convolution taps of a ramp-window row (host arrays).
+ * win_re/win_im: width values of one row of make_2d_ramp_window (in its
+ * fftshift order); taps_out: width doubles.  Long-double DFT. */
+int ath_filter_taps_from_window(const double* win_re, const double* win_im,
+                                int width, double* taps_out);
+
+/* Ramp filter of every row of a (nrow, nslice, ncol) stack:
+ * optional -log (apply_log, reconstruction.py:431-436; values must already
+ * be > 0), numpy.pad by `pad` columns (pad_mode), windowed ramp filter
+ * (circular convolution of width ncol + 2*pad), real part, crop
+ * [pad, pad + ncol).  Replaces apply_ramp_filter (reconstruction.py:118-153)
+ * and its 3D fan-out (:460-475).  in_dtype: ATH_F32 or ATH_F64;
+ * out_dtype: any ATH_* element type.  nwin is 1 (one window for all rows)
+ * or nrow (one window per angle row). */
+int ath_ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
+                    int ncol, int pad, int pad_mode, int apply_log,
+                    const void* plan, int fft_len, int nwin, void* out,
+                    int out_dtype, void* stream);
+
+/* Linear-interpolation back-projection of a (nrow, nslice, ncol) filtered
+ * stack into recon (ncol, nslice, ncol) float32.  angles: nrow float32
+ * radians (device); center: rotation centre (rounded to float32 as the
+ * reference does).  edge_pad: out-of-range positions add the edge sample
+ * (reconstruction.py:183-189).  mask_radius >= 0 multiplies by
+ * make_circle_mask (utility.py:160-182) with that radius; apply_scale
+ * multiplies by pi/(nrow-1) in float32 (reconstruction.py:493).
+ * Replaces back_projection_gpu (:193-226, acc ATH_ACC_F64),
+ * back_projection_gpu_chunk (:267-304) and back_projection_cpu (:307-359)
+ * (acc ATH_ACC_F32SEQ).  sino_dtype: ATH_F32, or ATH_F16 / ATH_BF16 with
+ * acc ATH_ACC_F32 (reduced-precision storage, float32 accumulation). */
+int ath_back_projection(const void* sino, int sino_dtype, int nrow,
+                        int nslice, int ncol, const float* angles,
+                        float center, int edge_pad, int acc,
+                        double mask_radius, int apply_scale, float* recon,
+                        void* stream);
+
+/* Same contract, one thread per output pixel reading global memory directly
+ * (the reference kernels' structure).  Kept as an independent second
+ * implementation for cross-checks; float32 sinograms only. */
+int ath_back_projection_naive(const float* sino, int nrow, int nslice,
+                              int ncol, const float* angles, float center,
+                              int edge_pad, int acc, double mask_radius,
+                              int apply_scale, float* recon, void* stream);
+
+/* Device workspace bytes needed by ath_fbp_reconstruction. */
+size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
+                               int nwin, int filt_dtype);
+
+/* Whole FBP of a (nrow, nslice, ncol) stack: filter (log, pad, windowed
+ * ramp) into the workspace, then back-projection with mask and scale into
+ * recon (ncol, nslice, ncol).  taps: nwin x (ncol + 2*pad) float64 (device).
+ * Replaces fbp_reconstruction (reconstruction.py:362-493) below its Python
+ * argument handling. */
+int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
+                           int nslice, int ncol, const float* angles,
+                           float center, double mask_radius,
+                           const double* taps, int nwin, int pad,
+                           int pad_mode, int apply_log, int acc,
+                           int filt_dtype, void* workspace,
+                           size_t workspace_bytes, float* recon,
+                           void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALGOTOM_HIP_H */

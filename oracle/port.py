@@ -1,0 +1,124 @@
+"""ctypes binding of the oracle's C port -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+The C source (oracle/fbp_port.c) restates algotom/rec/reconstruction.py
+:118-153 and :156-359 with numba's typing; see its header.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from oracle import fbp_np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libfbp_port.so")
+_lib = None
+
+PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
+             "wrap": 4}
+
+
+def build():
+    """Compile oracle/fbp_port.c with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle port not built: run make -C oracle")
+        _lib = ctypes.CDLL(_LIB_PATH)
+        p = ctypes.c_void_p
+        i = ctypes.c_int
+        _lib.orc_trig_tables.argtypes = [p, i, p, p]
+        _lib.orc_backproject.argtypes = [p, i, i, i, p, p, ctypes.c_float,
+                                         i, i, p, i]
+        _lib.orc_ramp_filter.argtypes = [p, i, i, i, i, i, i, p, p, i]
+        _lib.orc_finalize.argtypes = [p, i, i, i, ctypes.c_double, i]
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_threads():
+    return len(os.sched_getaffinity(0))
+
+
+def trig_tables(angles):
+    a = np.ascontiguousarray(angles, dtype=np.float32)
+    c = np.empty_like(a)
+    s = np.empty_like(a)
+    lib().orc_trig_tables(_ptr(a), a.size, _ptr(c), _ptr(s))
+    return c, s
+
+
+def backproject(sino, angles, center, edge_pad=False, acc=fbp_np.ACC_F32SEQ,
+                nthreads=None):
+    """sino: (nrow, ncol) or (nrow, nslice, ncol) float32 -> float32 image(s)
+    of shape (ncol, ncol) or (ncol, nslice, ncol)."""
+    s = np.ascontiguousarray(sino, dtype=np.float32)
+    two_d = s.ndim == 2
+    if two_d:
+        s = s[:, None, :]
+    nrow, nslice, ncol = s.shape
+    c, sn = trig_tables(angles)
+    out = np.empty((ncol, nslice, ncol), dtype=np.float32)
+    lib().orc_backproject(_ptr(s), nrow, nslice, ncol, _ptr(c), _ptr(sn),
+                          float(np.float32(center)), int(bool(edge_pad)),
+                          int(acc), _ptr(out), nthreads or default_threads())
+    return out[:, 0, :] if two_d else out
+
+
+def ramp_filter(rows, g, pad, pad_mode="edge", apply_log=False,
+                nthreads=None):
+    """rows: (..., ncol) float32/float64; g: real kernel of length
+    ncol + 2*pad.  Returns float64 (..., ncol)."""
+    r = np.asarray(rows)
+    in_f64 = r.dtype == np.float64
+    r = np.ascontiguousarray(r, dtype=np.float64 if in_f64 else np.float32)
+    ncol = r.shape[-1]
+    nrows = r.size // ncol
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    assert g.size == ncol + 2 * pad
+    out = np.empty(r.shape, dtype=np.float64)
+    lib().orc_ramp_filter(_ptr(r), int(in_f64), nrows, ncol, int(pad),
+                          PAD_MODES[pad_mode], int(bool(apply_log)), _ptr(g),
+                          _ptr(out), nthreads or default_threads())
+    return out
+
+
+def fbp(sinogram, center, angles=None, ratio=1.0, ramp_row=None,
+        filter_name="hann", pad=None, pad_mode="edge", apply_log=True,
+        gpu=True, nthreads=None):
+    """Same contract as fbp_np.fbp, evaluated by the C port."""
+    sino = np.array(sinogram, copy=True)
+    input_3d = sino.ndim == 3
+    if not input_3d:
+        sino = sino[:, None, :]
+    nrow, nslice, ncol = sino.shape
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    if apply_log:
+        sino[sino <= 0.0] = 1.0
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    width = ncol + 2 * pad
+    if ramp_row is None or np.shape(ramp_row)[-1] != width:
+        ramp_row = fbp_np.ramp_window_row(width, filter_name)
+    g = fbp_np.filter_kernel(ramp_row)
+    filt = ramp_filter(sino, g, pad, pad_mode, apply_log,
+                       nthreads).astype(np.float32)
+    acc = fbp_np.ACC_F64 if (gpu and nslice == 1) else fbp_np.ACC_F32SEQ
+    out = backproject(filt, angles, center, acc=acc, nthreads=nthreads)
+    use_mask = ratio is not None
+    if use_mask and ratio == 0.0:
+        ratio = min(center, ncol - center) / (0.5 * ncol)
+    lib().orc_finalize(_ptr(out), ncol, nslice, nrow,
+                       float(ratio if use_mask else 0.0), int(use_mask))
+    return out if input_3d else out[:, 0, :]

@@ -1,0 +1,161 @@
+"""NumPy models of the HIP kernels' index arithmetic (CPU, test-only).
+
+These mirror algotom_amd/csrc/ramp_filter.hip and backproject.hip step by
+step (same pass structure, same index formulas, same table indices) so that
+the kernels' addressing can be checked on the CPU against the oracle before
+anything runs on a GPU.  They are not the oracle: the oracle is oracle/.
+"""
+
+import math
+
+import numpy as np
+
+
+def twiddles(L):
+    n = np.arange(L)
+    return np.exp(-2j * np.pi * n / L)
+
+
+def lds_fft(buf, tw, direction):
+    """Model of lds_fft<DIR>: radix-4 Stockham passes then one radix-2."""
+    buf = buf.copy()
+    M = buf.size
+    L = 2 * M
+    ns = 1
+    while ns * 4 <= M:
+        nb = M >> 2
+        tstep = L // (ns * 4)
+        j = np.arange(nb)
+        v = np.stack([buf[j + r * nb] for r in range(4)])
+        k = j & (ns - 1)
+        for r in range(1, 4):
+            w = tw[r * k * tstep]
+            if direction > 0:
+                w = np.conj(w)
+            v[r] = v[r] * w
+        a0 = v[0] + v[2]
+        a1 = v[0] - v[2]
+        a2 = v[1] + v[3]
+        d = v[1] - v[3]
+        a3 = -1j * d if direction < 0 else 1j * d
+        out = [a0 + a2, a1 + a3, a0 - a2, a1 - a3]
+        base = (j - k) * 4 + k
+        for r in range(4):
+            buf[base + r * ns] = out[r]
+        ns *= 4
+    if ns < M:
+        nb = M >> 1
+        j = np.arange(nb)
+        x0 = buf[j].copy()
+        x1 = buf[j + nb].copy()
+        w = tw[2 * j]
+        if direction > 0:
+            w = np.conj(w)
+        x1 = x1 * w
+        buf[j] = x0 + x1
+        buf[j + ns] = x0 - x1
+    return buf
+
+
+def split_real(zk, zm, wk):
+    e = 0.5 * (zk + np.conj(zm))
+    o = -1j * (0.5 * (zk - np.conj(zm)))
+    wo = wk * o
+    return e + wo, np.conj(e - wo)
+
+
+def kernel_spectrum(g, L):
+    """Model of kernel_spectrum_kernel: rfft_L([g, g, 0...]) / M."""
+    M = L // 2
+    W = g.size
+    tw = twiddles(L)
+    x = np.zeros(L)
+    n = np.arange(min(2 * W, L))
+    x[n] = g[np.where(n < W, n, n - W)]
+    buf = x[0::2] + 1j * x[1::2]
+    z = lds_fft(buf, tw, -1)
+    G = np.zeros(M + 1, dtype=complex)
+    k = np.arange(M // 2 + 1)
+    km = (M - k) & (M - 1)
+    xk, xm = split_real(z[k], z[km], tw[k])
+    G[k] = xk / M
+    G[M - k] = xm / M
+    return G
+
+
+def ramp_filter_row(p_row, g, pad, L):
+    """Model of ramp_filter_kernel on one already padded row (length W)."""
+    M = L // 2
+    W = p_row.size
+    ncol = W - 2 * pad
+    tw = twiddles(L)
+    G = kernel_spectrum(g, L)
+    x = np.zeros(L)
+    x[:W] = p_row
+    z = lds_fft(x[0::2] + 1j * x[1::2], tw, -1)
+    out = z.copy()
+    for k in range(M // 2 + 1):
+        km = (M - k) & (M - 1)
+        wk = tw[k]
+        xk, xm = split_real(z[k], z[km], wk)
+        yk = xk * G[k]
+        ym = xm * G[M - k]
+        ek = 0.5 * (yk + np.conj(ym))
+        ok = 1j * (np.conj(wk) * (0.5 * (yk - np.conj(ym))))
+        out[k] = ek + ok
+        if k != 0 and k != km:
+            em = 0.5 * (ym + np.conj(yk))
+            om = -1j * (wk * (0.5 * (ym - np.conj(yk))))
+            out[km] = em + om
+    zy = lds_fft(out, tw, 1)
+    y = np.empty(L)
+    y[0::2] = zy.real
+    y[1::2] = zy.imag
+    off = W + pad
+    return y[off:off + ncol]
+
+
+def fft_length(ncol, pad):
+    need = 2 * (ncol + 2 * pad) - pad
+    L = 64
+    while L < need:
+        L *= 2
+    return L
+
+
+def bp_span(tx, ty):
+    return int(math.ceil(math.sqrt((tx - 1) ** 2 + (ty - 1) ** 2) - 1e-12)) + 3
+
+
+def bp_tile_footprints(ncol, angles, center, py=8, tx=64):
+    """Model of bp_tiled_kernel's per-(tile, angle) staging bounds.
+
+    Returns the largest staged index any pixel of any tile reads
+    (d - dlo on the range-free path, for pixels the kernel accumulates) and
+    the span the kernel allocates.
+    """
+    ty = 4 * py
+    span = bp_span(tx, ty)
+    ic = 0.5 * (ncol - 1)
+    th = -np.asarray(angles, dtype=np.float32).astype(np.float64)
+    cs = np.cos(th).astype(np.float32).astype(np.float64)
+    sn = np.sin(th).astype(np.float32).astype(np.float64)
+    worst = 0
+    for y0 in range(0, ncol, ty):
+        for x0 in range(0, ncol, tx):
+            xs = np.arange(x0, x0 + tx) - ic
+            ys = np.arange(y0, y0 + ty) - ic
+            cx = np.array([x0, x0 + tx - 1]) - ic
+            cy = np.array([y0, y0 + ty - 1]) - ic
+            for c, s in zip(cs, sn):
+                corners = (cy[:, None] * s + (cx[None, :] * c)) + center
+                fmn, fmx = corners.min(), corners.max()
+                dlo = math.floor(fmn)
+                f = (ys[:, None] * s + (xs[None, :] * c)) + center
+                assert f.min() >= fmn and f.max() <= fmx
+                inner = fmn >= 0.0 and fmx <= ncol - 1
+                if inner:
+                    d = np.trunc(f).astype(np.int64)
+                    worst = max(worst, int((d - dlo).max()))
+                    assert int((d - dlo).min()) >= 0
+    return worst, span
