@@ -43,6 +43,8 @@ SIGNATURES = (
                              _I, _P]),
     ("ath_back_projection", _I, [_P, _I, _I, _I, _I, _P, _F, _I, _I, _D, _I,
                                  _P, _P]),
+    ("ath_back_projection_tiled", _I, [_P, _I, _I, _I, _I, _P, _F, _I, _I,
+                                       _D, _I, _P, _I, _I, _P]),
     ("ath_back_projection_naive", _I, [_P, _I, _I, _I, _P, _F, _I, _I, _D,
                                        _I, _P, _P]),
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),

@@ -82,7 +82,8 @@ struct BpArgs {
     float pi32, denom;     // float32(pi), float32(nrow - 1)
 };
 
-int back_projection(const BpArgs& a, int sino_dtype, int acc,
+// sb / py <= 0 select the default tile configuration
+int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb, int py,
                     hipStream_t stream);
 int back_projection_naive(const BpArgs& a, int acc, hipStream_t stream);
 

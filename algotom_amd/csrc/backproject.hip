@@ -15,16 +15,24 @@
 // PY rows y0 + w*PY + k, so one wave instruction touches 64 consecutive x
 // and its LDS gathers hit consecutive columns (no bank conflicts).  Angles
 // are processed in blocks of A: for each angle the workgroup computes the
-// tile's detector footprint [dlo, dlo+SPAN) exactly from its four corners,
-// stages (s[d], s[d+1]-s[d]) pairs of the SB slices interleaved per column
-// into LDS, and every thread then gathers one 8*SB-byte record per pixel
-// and angle.  f, d and the weight are computed once per pixel and angle
-// and reused by all SB slices.  Angles whose footprint lies inside
-// [0, ncol-1] for the whole tile (every interior tile) skip the range test;
-// the rest take the exact per-pixel test.  Each pixel sums its angles in
-// ascending order, as the reference does, so ATH_ACC_F64 and
-// ATH_ACC_F32SEQ reproduce the reference kernels bit for bit given the same
-// filtered sinogram.
+// tile's detector footprint [dlo, dlo+SPAN) exactly from its four corners
+// and stages one record per (column, slice) into LDS -- the sample and its
+// forward difference, interleaved over the SB slices so one pixel reads
+// all of them with one or two ds_read_b128.  f, d and the weight are
+// computed once per pixel and angle and reused by the SB slices.  Angles
+// whose footprint lies inside [0, ncol-1] for the whole tile (every interior
+// tile) skip the range test; the rest take the exact per-pixel test.  Each
+// pixel sums its angles in ascending order, as the reference does, so
+// ATH_ACC_F64 and ATH_ACC_F32SEQ reproduce the reference kernels bit for bit
+// given the same filtered sinogram.
+//
+// Per-mode arithmetic (Mode<> below):
+//   ATH_ACC_F64     record float32 (s, ds); val and sum in float64
+//   ATH_ACC_F32SEQ  record float32 (s, ds); val in float64, sum stored float32
+//   ATH_ACC_F32     record float32 (s, ds); one FMA and one add in float32
+//   kAccH16/kAccB16 float16/bfloat16 storage: record (s, ds) as a 16-bit pair,
+//                   weight pair (1, w); one v_dot2c_f32_{f16,bf16} per update
+//                   accumulating in float32.
 #include "ath_internal.h"
 
 namespace ath {
@@ -32,6 +40,11 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTileX = 64;
+constexpr int kAccH16 = 16;  // internal modes: 16-bit storage + dot2
+constexpr int kAccB16 = 17;
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
 
 constexpr int isqrt_ceil(int v) {
     int r = 0;
@@ -51,47 +64,113 @@ __device__ __forceinline__ double det_pos(double xc_c, double yc, double s,
     return fma(yc, s, xc_c) + cen;
 }
 
-template <int MODE>
-struct Acc;
-template <>
-struct Acc<ATH_ACC_F64> {
-    double v = 0.0;
-    __device__ __forceinline__ void add(float2 q, double w) {
-        v += (double)q.x + (double)q.y * w;
-    }
-    __device__ __forceinline__ void add_value(float e) { v += (double)e; }
-    __device__ __forceinline__ float result() const { return (float)v; }
-};
-template <>
-struct Acc<ATH_ACC_F32SEQ> {
-    float v = 0.0f;
-    __device__ __forceinline__ void add(float2 q, double w) {
-        v = (float)((double)v + ((double)q.x + (double)q.y * w));
-    }
-    __device__ __forceinline__ void add_value(float e) {
-        v = (float)((double)v + (double)e);
-    }
-    __device__ __forceinline__ float result() const { return v; }
-};
-template <>
-struct Acc<ATH_ACC_F32> {
-    float v = 0.0f;
-    __device__ __forceinline__ void add(float2 q, float w) {
-        v += fmaf(q.y, w, q.x);
-    }
-    __device__ __forceinline__ void add_value(float e) { v += e; }
-    __device__ __forceinline__ float result() const { return v; }
-};
+__device__ __forceinline__ float load_f32(const float* p) { return *p; }
+__device__ __forceinline__ float load_f32(const __half* p) {
+    return __half2float(*p);
+}
+__device__ __forceinline__ float load_f32(const __hip_bfloat16* p) {
+    return __bfloat162float(*p);
+}
 
 template <int MODE>
-struct Weight {
-    using type = double;
-    static __device__ __forceinline__ double make(double w) { return w; }
-};
+struct Mode;
+
 template <>
-struct Weight<ATH_ACC_F32> {
-    using type = float;
-    static __device__ __forceinline__ float make(double w) { return (float)w; }
+struct Mode<ATH_ACC_F64> {
+    using Rec = float2;
+    using W = double;
+    using Acc = double;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_float2(yd, yu - yd);
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a += (double)q.x + (double)q.y * w;
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a += (double)e;
+    }
+    static __device__ __forceinline__ float result(Acc a) { return (float)a; }
+};
+
+template <>
+struct Mode<ATH_ACC_F32SEQ> {
+    using Rec = float2;
+    using W = double;
+    using Acc = float;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_float2(yd, yu - yd);
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a = (float)((double)a + ((double)q.x + (double)q.y * w));
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a = (float)((double)a + (double)e);
+    }
+    static __device__ __forceinline__ float result(Acc a) { return a; }
+};
+
+template <>
+struct Mode<ATH_ACC_F32> {
+    using Rec = float2;
+    using W = float;
+    using Acc = float;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_float2(yd, yu - yd);
+    }
+    static __device__ __forceinline__ W weight(double fr) { return (float)fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a += fmaf(q.y, w, q.x);
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a += e;
+    }
+    static __device__ __forceinline__ float result(Acc a) { return a; }
+};
+
+template <>
+struct Mode<kAccH16> {
+    using Rec = half2_t;
+    using W = half2_t;
+    using Acc = float;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        // both samples are float16 values: the difference is exact in
+        // float32 and rounds once to float16
+        return Rec{(_Float16)yd, (_Float16)(yu - yd)};
+    }
+    static __device__ __forceinline__ W weight(double fr) {
+        // (1, w): 1 is exact; w rounds toward zero (one v_cvt_pkrtz)
+        const auto p = __builtin_amdgcn_cvt_pkrtz(1.0f, (float)fr);
+        return __builtin_bit_cast(W, p);
+    }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a = __builtin_amdgcn_fdot2(q, w, a, false);
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a += e;
+    }
+    static __device__ __forceinline__ float result(Acc a) { return a; }
+};
+
+template <>
+struct Mode<kAccB16> {
+    using Rec = bf162_t;
+    using W = bf162_t;
+    using Acc = float;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return Rec{(__bf16)yd, (__bf16)(yu - yd)};
+    }
+    static __device__ __forceinline__ W weight(double fr) {
+        return W{(__bf16)1.0f, (__bf16)(float)fr};
+    }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a = __builtin_amdgcn_fdot2_f32_bf16(q, w, a, false);
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a += e;
+    }
+    static __device__ __forceinline__ float result(Acc a) { return a; }
 };
 
 // mask (utility.py:160-182) and pi/(nrow-1) (reconstruction.py:493), in the
@@ -110,12 +189,37 @@ __device__ __forceinline__ float finalize(const BpArgs& a, float v, int x,
     return v;
 }
 
+// One angle over the thread's PY x SB pixels when the whole tile is in
+// range: f >= 0, so the truncation is the floor and fract(f) = f - floor(f).
+// The record address is one shift-add on a workgroup-uniform offset.
+template <class M, int SB, int PY, int SPAN>
+__device__ __forceinline__ void inner_angle(
+    const typename M::Rec* s_rec, int al, double c, double s, int dlo,
+    double xc, const double (&yc)[PY], double cen,
+    typename M::Acc (&acc)[PY][SB]) {
+    using Rec = typename M::Rec;
+    const int off = __builtin_amdgcn_readfirstlane((al * SPAN - dlo) * SB);
+    const Rec* qbase = s_rec + off;
+    const double xcc = xc * c;
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+        const double f = det_pos(xcc, yc[k], s, cen);
+        const int d = (int)f;
+        const auto w = M::weight(__builtin_amdgcn_fract(f));
+        const Rec* q = qbase + d * SB;
+#pragma unroll
+        for (int js = 0; js < SB; ++js) M::add(acc[k][js], q[js], w);
+    }
+}
+
 template <int MODE, int SB, int PY, int A, typename Tin>
 __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
     BpArgs a, int tiles_x, int tiles_y) {
+    using M = Mode<MODE>;
+    using Rec = typename M::Rec;
     constexpr int TY = 4 * PY;
     constexpr int SPAN = bp_span(kTileX, TY);
-    __shared__ float2 s_pair[A * SPAN * SB];
+    __shared__ Rec s_rec[A * SPAN * SB];
     __shared__ double s_cos[A], s_sin[A];
     __shared__ int s_dlo[A], s_inner[A];
     __shared__ float s_edge[A * SB * 2];
@@ -132,6 +236,7 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
     const int x0 = tx * kTileX, y0 = ty * TY, j0 = grp * SB;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
     const double ic = a.icenter, cen = a.center, w1 = (double)(C - 1);
+    const int nsb = min(SB, S - j0);
 
     const int x = x0 + lane;
     const double xc = (double)x - ic;
@@ -142,11 +247,16 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
     const double cx0 = (double)x0 - ic, cx1 = (double)(x0 + kTileX - 1) - ic;
     const double cy0 = (double)y0 - ic, cy1 = (double)(y0 + TY - 1) - ic;
 
-    Acc<MODE> acc[PY][SB];
+    typename M::Acc acc[PY][SB];
+#pragma unroll
+    for (int k = 0; k < PY; ++k)
+#pragma unroll
+        for (int js = 0; js < SB; ++js) acc[k][js] = 0;
 
     for (int a0 = 0; a0 < N; a0 += A) {
         const int na = min(A, N - a0);
         __syncthreads();  // previous block's LDS fully consumed
+        int inner = 1;
         if (tid < na) {
             const float th = -a.angles[a0 + tid];
             const double c = (double)(float)cos((double)th);
@@ -160,33 +270,45 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
             s_cos[tid] = c;
             s_sin[tid] = s;
             s_dlo[tid] = (int)floor(fmn);
-            s_inner[tid] = (fmn >= 0.0) && (fmx <= w1);
+            inner = (fmn >= 0.0) && (fmx <= w1);
+            s_inner[tid] = inner;
         }
-        __syncthreads();
-        const int nstage = na * SB * SPAN;
-        for (int e = tid; e < nstage; e += kThreads) {
-            const int k = e % SPAN;
-            const int t = e / SPAN;
-            const int js = t % SB;
-            const int al = t / SB;
-            const int j = j0 + js;
+        // barrier that also tells every thread whether all na angles are
+        // in range for the whole tile
+        const int all_inner = __syncthreads_and(inner);
+        // one record per (angle, column): the SB slices' (s, ds)
+        const int nrec = na * SPAN;
+        for (int e = tid; e < nrec; e += kThreads) {
+            const int al = e / SPAN;
+            const int k = e - al * SPAN;
             const int col = s_dlo[al] + k;
-            float yd = 0.0f, yu = 0.0f;
-            if (j < S) {
-                const Tin* row = sino + ((size_t)(a0 + al) * S + j) * C;
-                if (col >= 0 && col < C) yd = to_f32(row[col]);
-                if (col + 1 >= 0 && col + 1 < C) yu = to_f32(row[col + 1]);
+            const bool in0 = col >= 0 && col < C;
+            const bool in1 = col + 1 >= 0 && col + 1 < C;
+            const Tin* rows = sino + ((size_t)(a0 + al) * S + j0) * C;
+            Rec r[SB];
+#pragma unroll
+            for (int js = 0; js < SB; ++js) {
+                float yd = 0.0f, yu = 0.0f;
+                if (js < nsb) {
+                    const Tin* row = rows + (size_t)js * C;
+                    if (in0) yd = load_f32(row + col);
+                    if (in1) yu = load_f32(row + col + 1);
+                }
+                r[js] = M::make(yd, yu);
             }
-            s_pair[(al * SPAN + k) * SB + js] = make_float2(yd, yu - yd);
+            Rec* dst = s_rec + e * SB;
+#pragma unroll
+            for (int js = 0; js < SB; ++js) dst[js] = r[js];
         }
         if (a.edge_pad) {
             for (int e = tid; e < na * SB; e += kThreads) {
-                const int js = e % SB, al = e / SB, j = j0 + js;
+                const int js = e % SB, al = e / SB;
                 float e0 = 0.0f, e1 = 0.0f;
-                if (j < S) {
-                    const Tin* row = sino + ((size_t)(a0 + al) * S + j) * C;
-                    e0 = to_f32(row[0]);
-                    e1 = to_f32(row[C - 1]);
+                if (js < nsb) {
+                    const Tin* row =
+                        sino + ((size_t)(a0 + al) * S + j0 + js) * C;
+                    e0 = load_f32(row);
+                    e1 = load_f32(row + C - 1);
                 }
                 s_edge[2 * e] = e0;
                 s_edge[2 * e + 1] = e1;
@@ -194,40 +316,41 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
         }
         __syncthreads();
 
-        for (int al = 0; al < na; ++al) {
-            const double c = s_cos[al], s = s_sin[al];
-            const int dlo = s_dlo[al];
-            const double xcc = xc * c;
-            const float2* base = s_pair + al * SPAN * SB;
-            if (s_inner[al]) {
-#pragma unroll
-                for (int k = 0; k < PY; ++k) {
-                    const double f = det_pos(xcc, yc[k], s, cen);
-                    const int d = (int)f;
-                    const auto w = Weight<MODE>::make(f - (double)d);
-                    const float2* q = base + (d - dlo) * SB;
-#pragma unroll
-                    for (int js = 0; js < SB; ++js) acc[k][js].add(q[js], w);
+        if (all_inner) {
+            // common case: the whole block is in range for the whole tile
+            for (int al = 0; al < na; ++al)
+                inner_angle<M, SB, PY, SPAN>(s_rec, al, s_cos[al], s_sin[al],
+                                             s_dlo[al], xc, yc, cen, acc);
+        } else {
+            for (int al = 0; al < na; ++al) {
+                if (s_inner[al]) {
+                    inner_angle<M, SB, PY, SPAN>(s_rec, al, s_cos[al],
+                                                 s_sin[al], s_dlo[al], xc, yc,
+                                                 cen, acc);
+                    continue;
                 }
-            } else {
+                const double c = s_cos[al], s = s_sin[al];
+                const int dlo = s_dlo[al];
+                const double xcc = xc * c;
+                const Rec* base = s_rec + al * SPAN * SB;
 #pragma unroll
                 for (int k = 0; k < PY; ++k) {
                     const double f = det_pos(xcc, yc[k], s, cen);
                     const bool in = (f >= 0.0) && (f <= w1);
                     const int d = (int)f;
                     const int idx = min(max(d - dlo, 0), SPAN - 1);
-                    const auto w = Weight<MODE>::make(f - (double)d);
-                    const float2* q = base + idx * SB;
+                    const auto w = M::weight(f - (double)d);
+                    const Rec* q = base + idx * SB;
                     if (in) {
 #pragma unroll
                         for (int js = 0; js < SB; ++js)
-                            acc[k][js].add(q[js], w);
+                            M::add(acc[k][js], q[js], w);
                     } else if (a.edge_pad) {
                         const int side = f < 0.0 ? 0 : 1;
 #pragma unroll
                         for (int js = 0; js < SB; ++js)
-                            acc[k][js].add_value(
-                                s_edge[2 * (al * SB + js) + side]);
+                            M::add_value(acc[k][js],
+                                         s_edge[2 * (al * SB + js) + side]);
                     }
                 }
             }
@@ -241,10 +364,9 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
             if (y >= C) continue;
 #pragma unroll
             for (int js = 0; js < SB; ++js) {
-                const int j = j0 + js;
-                if (j < S)
-                    a.recon[((size_t)y * S + j) * C + x] =
-                        finalize(a, acc[k][js].result(), x, y);
+                if (js < nsb)
+                    a.recon[((size_t)y * S + j0 + js) * C + x] =
+                        finalize(a, M::result(acc[k][js]), x, y);
             }
         }
     }
@@ -253,6 +375,7 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
 // One thread per (x, y, slice), sinogram read straight from global memory.
 template <int MODE>
 __global__ void bp_naive_kernel(BpArgs a) {
+    using M = Mode<MODE>;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int j = blockIdx.z;
@@ -261,7 +384,7 @@ __global__ void bp_naive_kernel(BpArgs a) {
     const float* sino = static_cast<const float*>(a.sino);
     const double ic = a.icenter, w1 = (double)(C - 1);
     const double xc = (double)x - ic, yc = (double)y - ic;
-    Acc<MODE> acc;
+    typename M::Acc acc = 0;
     for (int i = 0; i < a.nrow; ++i) {
         const float th = -a.angles[i];
         const double c = (double)(float)cos((double)th);
@@ -272,13 +395,12 @@ __global__ void bp_naive_kernel(BpArgs a) {
             const int d = (int)f;
             const float yd = row[d];
             const float yu = d + 1 < C ? row[d + 1] : 0.0f;
-            acc.add(make_float2(yd, yu - yd),
-                    Weight<MODE>::make(f - (double)d));
+            M::add(acc, M::make(yd, yu), M::weight(f - (double)d));
         } else if (a.edge_pad) {
-            acc.add_value(f < 0.0 ? row[0] : row[C - 1]);
+            M::add_value(acc, f < 0.0 ? row[0] : row[C - 1]);
         }
     }
-    a.recon[((size_t)y * S + j) * C + x] = finalize(a, acc.result(), x, y);
+    a.recon[((size_t)y * S + j) * C + x] = finalize(a, M::result(acc), x, y);
 }
 
 template <int MODE, int SB, int PY, int A, typename Tin>
@@ -296,23 +418,53 @@ int launch_tiled(const BpArgs& a, hipStream_t stream) {
     return check_hip(hipGetLastError(), "bp_tiled_kernel launch");
 }
 
+// Tile configurations compiled per mode: slice batch SB and rows per lane
+// PY (tile 64 x 4*PY).  16-bit records also allow SB = 8 (32 bytes a pixel,
+// the same as SB = 4 float32 records).
 template <int MODE, typename Tin>
-int pick_sb(const BpArgs& a, hipStream_t s) {
-    constexpr int PY = 8, A = 16;
-    if (a.nslice == 1) return launch_tiled<MODE, 1, PY, A, Tin>(a, s);
-    if (a.nslice == 2) return launch_tiled<MODE, 2, PY, A, Tin>(a, s);
-    return launch_tiled<MODE, 4, PY, A, Tin>(a, s);
+int launch_config(const BpArgs& a, int sb, int py, hipStream_t s) {
+    constexpr int A = 16;
+    constexpr bool wide = (MODE == kAccH16 || MODE == kAccB16);
+#define ATH_BP_CASE(SB_, PY_) \
+    if (sb == SB_ && py == PY_) return launch_tiled<MODE, SB_, PY_, A, Tin>(a, s);
+    ATH_BP_CASE(1, 4) ATH_BP_CASE(1, 8) ATH_BP_CASE(1, 16)
+    ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
+    ATH_BP_CASE(4, 4) ATH_BP_CASE(4, 8) ATH_BP_CASE(4, 16)
+    if constexpr (wide) {
+        ATH_BP_CASE(8, 4) ATH_BP_CASE(8, 8)
+    }
+#undef ATH_BP_CASE
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "back-projection: no kernel for slice_batch " +
+                         std::to_string(sb) + ", tile_rows " +
+                         std::to_string(4 * py));
+}
+
+// Default configuration per mode and stack depth (measured on MI355X with
+// a 64 x 1801 x 2560 stack; the autotuner can override it per shape).
+template <int MODE>
+void default_config(int nslice, int& sb, int& py) {
+    sb = nslice <= 2 ? nslice : 4;
+    py = 8;
+    if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == kAccH16)) py = 4;
+}
+
+template <int MODE, typename Tin>
+int dispatch(const BpArgs& a, int sb, int py, hipStream_t s) {
+    if (sb <= 0 || py <= 0) default_config<MODE>(a.nslice, sb, py);
+    return launch_config<MODE, Tin>(a, sb, py, s);
 }
 
 }  // namespace
 
-int back_projection(const BpArgs& a, int sino_dtype, int acc,
-                    hipStream_t s) {
+int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb,
+                    int py, hipStream_t s) {
     if (sino_dtype == ATH_F32) {
         switch (acc) {
-        case ATH_ACC_F64: return pick_sb<ATH_ACC_F64, float>(a, s);
-        case ATH_ACC_F32SEQ: return pick_sb<ATH_ACC_F32SEQ, float>(a, s);
-        case ATH_ACC_F32: return pick_sb<ATH_ACC_F32, float>(a, s);
+        case ATH_ACC_F64: return dispatch<ATH_ACC_F64, float>(a, sb, py, s);
+        case ATH_ACC_F32SEQ:
+            return dispatch<ATH_ACC_F32SEQ, float>(a, sb, py, s);
+        case ATH_ACC_F32: return dispatch<ATH_ACC_F32, float>(a, sb, py, s);
         }
         return set_error(ATH_ERR_INVALID_ARG, "back-projection: bad acc mode");
     }
@@ -320,9 +472,9 @@ int back_projection(const BpArgs& a, int sino_dtype, int acc,
         return set_error(ATH_ERR_UNSUPPORTED,
                          "back-projection: float16/bfloat16 sinograms need "
                          "acc = ATH_ACC_F32");
-    if (sino_dtype == ATH_F16) return pick_sb<ATH_ACC_F32, __half>(a, s);
+    if (sino_dtype == ATH_F16) return dispatch<kAccH16, __half>(a, sb, py, s);
     if (sino_dtype == ATH_BF16)
-        return pick_sb<ATH_ACC_F32, __hip_bfloat16>(a, s);
+        return dispatch<kAccB16, __hip_bfloat16>(a, sb, py, s);
     return set_error(ATH_ERR_INVALID_ARG, "back-projection: bad sinogram dtype");
 }
 

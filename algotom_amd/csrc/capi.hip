@@ -210,7 +210,25 @@ int ath_back_projection(const void* sino, int sino_dtype, int nrow,
     int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center,
                           edge_pad, mask_radius, apply_scale, recon);
     if (st != ATH_OK) return st;
-    return back_projection(a, sino_dtype, acc,
+    return back_projection(a, sino_dtype, acc, 0, 0,
+                           static_cast<hipStream_t>(stream));
+}
+
+int ath_back_projection_tiled(const void* sino, int sino_dtype, int nrow,
+                              int nslice, int ncol, const float* angles,
+                              float center, int edge_pad, int acc,
+                              double mask_radius, int apply_scale,
+                              float* recon, int slice_batch, int tile_rows,
+                              void* stream) {
+    clear_error();
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center,
+                          edge_pad, mask_radius, apply_scale, recon);
+    if (st != ATH_OK) return st;
+    if (slice_batch < 1 || tile_rows < 4 || tile_rows % 4)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "slice_batch must be >= 1, tile_rows a multiple of 4");
+    return back_projection(a, sino_dtype, acc, slice_batch, tile_rows / 4,
                            static_cast<hipStream_t>(stream));
 }
 
@@ -277,7 +295,7 @@ int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
                          apply_log, ws, L, nwin, filt, filt_dtype, stream);
     if (st != ATH_OK) return st;
     a.sino = filt;
-    return back_projection(a, filt_dtype, acc, s);
+    return back_projection(a, filt_dtype, acc, 0, 0, s);
 }
 
 }  // extern "C"

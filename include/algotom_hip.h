@@ -124,6 +124,19 @@ int ath_back_projection(const void* sino, int sino_dtype, int nrow,
                         double mask_radius, int apply_scale, float* recon,
                         void* stream);
 
+/* ath_back_projection with an explicit tile configuration: slice_batch
+ * slices per workgroup (1, 2, 4; 8 for float16/bfloat16) and tile_rows
+ * image rows per 64-column tile (16, 32, 64).  Used by the autotuner;
+ * returns ATH_ERR_UNSUPPORTED for a configuration that is not compiled.
+ * No reference counterpart (numba's block=(16,16) argument,
+ * reconstruction.py:193-194, is the nearest). */
+int ath_back_projection_tiled(const void* sino, int sino_dtype, int nrow,
+                              int nslice, int ncol, const float* angles,
+                              float center, int edge_pad, int acc,
+                              double mask_radius, int apply_scale,
+                              float* recon, int slice_batch, int tile_rows,
+                              void* stream);
+
 /* Same contract, one thread per output pixel reading global memory directly
  * (the reference kernels' structure).  Kept as an independent second
  * implementation for cross-checks; float32 sinograms only. */
