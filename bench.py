@@ -50,6 +50,26 @@ def make_stack(dev, nslice=NSLICE, seed0=1000):
     return sino, angles
 
 
+def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
+    """A plain device copy of nbytes (torch's copy kernel): measured HBM
+    bandwidth (read + write bytes / time), and the calibration launch for
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "bp_traffic.json")
     if not os.path.exists(path):
@@ -134,6 +154,7 @@ def main():
     ms = dt / args.steps * 1e3
     achieved = updates * 8.0 / (bp_ms * 1e-3) / 1e9
 
+    copy_gbs = copy_bandwidth(dev)
     base, err = (None, None)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(sino, angles, out)
@@ -163,6 +184,8 @@ def main():
                      "traffic": (traffic or {}).get("bytes_per_launch")},
         "cpu_baseline": base,
         "detail": {"bp_kernel_ms": bp_ms, "filter_and_rest_ms": ms - bp_ms,
+                   "hbm_copy_GBs": copy_gbs,
+                   "frac_of_measured_copy_bw": achieved / copy_gbs,
                    "bp_gups": updates / (bp_ms * 1e-3) / 1e9,
                    "max_rel_err_vs_oracle_sample": err},
     }
