@@ -23,7 +23,7 @@ ATH_ERR_WORKSPACE = -5
 ATH_F32, ATH_F64, ATH_F16, ATH_BF16 = 0, 1, 2, 3
 PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
              "wrap": 4}
-ACC_F64, ACC_F32SEQ, ACC_F32 = 0, 1, 2
+ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV = 0, 1, 2, 3
 
 # (name, restype, argtypes) of every exported entry point
 _P = ctypes.c_void_p
@@ -45,12 +45,37 @@ SIGNATURES = (
                                  _P, _P]),
     ("ath_back_projection_tiled", _I, [_P, _I, _I, _I, _I, _P, _F, _I, _I,
                                        _D, _I, _P, _I, _I, _P]),
+    ("ath_bp_job_tiles", ctypes.c_longlong, [_I, _I, _I, _I]),
+    ("ath_back_projection_batched", _I, [_P, _I, ctypes.c_longlong, _I, _I,
+                                         _I, _I, _I, _P, _P]),
+    ("ath_ramp_filter_batched", _I, [_P, _I, ctypes.c_longlong, _I, _I, _I,
+                                     _I, _I, _P]),
     ("ath_back_projection_naive", _I, [_P, _I, _I, _I, _P, _F, _I, _I, _D,
                                        _I, _P, _P]),
+    ("ath_forward_projection", _I, [_P, _I, _I, _I, _P, _F, _D, _D, _P, _P]),
+    ("ath_ramp_filter_adjoint", _I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _P,
+                                     _P]),
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),
     ("ath_fbp_reconstruction", _I, [_P, _I, _I, _I, _I, _P, _F, _D, _P, _I,
                                     _I, _I, _I, _I, _I, _P, _S, _P, _P]),
 )
+
+
+
+class FilterJob(ctypes.Structure):
+    """ath_filter_job (include/algotom_hip.h)."""
+    _fields_ = [("sino", _P), ("out", _P), ("plan", _P), ("nrow", _I),
+                ("nslice", _I), ("ncol", _I), ("pad", _I), ("nwin", _I),
+                ("reserved", _I), ("row_begin", ctypes.c_longlong)]
+
+
+class BpJob(ctypes.Structure):
+    """ath_bp_job (include/algotom_hip.h)."""
+    _fields_ = [("sino", _P), ("recon", _P), ("angles", _P), ("nrow", _I),
+                ("nslice", _I), ("ncol", _I), ("center", _F),
+                ("mask_radius", _D), ("apply_scale", _I), ("edge_pad", _I),
+                ("tile_begin", ctypes.c_longlong)]
+
 
 _lib = None
 _lock = threading.Lock()

@@ -11,7 +11,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 SRC_DIR = os.path.join(_HERE, "csrc")
 LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libalgotom_hip.so")
-SOURCES = ("capi.hip", "ramp_filter.hip", "backproject.hip")
+SOURCES = ("capi.hip", "ramp_filter.hip", "backproject.hip",
+           "forward_project.hip")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

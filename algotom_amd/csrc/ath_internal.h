@@ -87,4 +87,29 @@ int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb, int py,
                     hipStream_t stream);
 int back_projection_naive(const BpArgs& a, int acc, hipStream_t stream);
 
+// ---- forward projection = adjoint of the back-projection ------------------
+struct FpArgs {
+    const float* image;    // (ncol, nslice, ncol) = (y, slice, x)
+    float* sino;           // (nrow, nslice, ncol)
+    const float* angles;
+    int nrow, nslice, ncol;
+    double center, icenter;
+    int use_mask;
+    double mask_r2;
+    int mask_c;
+    double scale;          // multiplies every output
+};
+int forward_projection(const FpArgs& a, hipStream_t stream);
+int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
+                        int pad, int pad_mode, const void* plan, int fft_len,
+                        int nwin, float* out, hipStream_t stream);
+int back_projection_batched(const ath_bp_job* jobs, int njobs,
+                            long long total, int sino_dtype, int acc, int sb,
+                            int py, int nblocks, int* counter,
+                            hipStream_t stream);
+int ramp_filter_batched(const ath_filter_job* jobs, int njobs,
+                        long long total_rows, int in_dtype, int out_dtype,
+                        int fft_len, int pad_mode, int apply_log,
+                        hipStream_t stream);
+
 }  // namespace ath

@@ -57,6 +57,13 @@ constexpr int isqrt_ceil(int v) {
 constexpr int bp_span(int tx, int ty) {
     return isqrt_ceil((tx - 1) * (tx - 1) + (ty - 1) * (ty - 1)) + 3;
 }
+// Modes that evaluate f in float32 stage from floor(fmin) - 1 and one more
+// column, so float32 rounding just outside the exact range still lands on a
+// staged (zero-padded) record and interpolates continuously.
+template <class M>
+constexpr int mode_span(int tx, int ty) {
+    return bp_span(tx, ty) + (M::kF32Pos ? 2 : 0);
+}
 
 // detector position exactly as the reference evaluates it
 __device__ __forceinline__ double det_pos(double xc_c, double yc, double s,
@@ -77,6 +84,7 @@ struct Mode;
 
 template <>
 struct Mode<ATH_ACC_F64> {
+    static constexpr bool kF32Pos = false;
     using Rec = float2;
     using W = double;
     using Acc = double;
@@ -95,6 +103,7 @@ struct Mode<ATH_ACC_F64> {
 
 template <>
 struct Mode<ATH_ACC_F32SEQ> {
+    static constexpr bool kF32Pos = false;
     using Rec = float2;
     using W = double;
     using Acc = float;
@@ -113,6 +122,7 @@ struct Mode<ATH_ACC_F32SEQ> {
 
 template <>
 struct Mode<ATH_ACC_F32> {
+    static constexpr bool kF32Pos = false;
     using Rec = float2;
     using W = float;
     using Acc = float;
@@ -129,8 +139,30 @@ struct Mode<ATH_ACC_F32> {
     static __device__ __forceinline__ float result(Acc a) { return a; }
 };
 
+// Derivative of the reference interpolation with respect to the centre:
+// d/df [s[d] + (s[d+1] - s[d]) (f - d)] = s[d+1] - s[d] at d = floor f, and
+// df/dcenter = 1.  Summed in float64; edge-padded samples do not move.
+template <>
+struct Mode<ATH_ACC_DERIV> {
+    static constexpr bool kF32Pos = false;
+    using Rec = float2;
+    using W = double;
+    using Acc = double;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_float2(yd, yu - yd);
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W) {
+        a += (double)q.y;
+    }
+    static __device__ __forceinline__ void add_value(Acc&, float) {}
+    static __device__ __forceinline__ float result(Acc a) { return (float)a; }
+};
+
 template <>
 struct Mode<kAccH16> {
+    // detector position in float32 from a float64 per-angle offset
+    static constexpr bool kF32Pos = true;
     using Rec = half2_t;
     using W = half2_t;
     using Acc = float;
@@ -140,8 +172,11 @@ struct Mode<kAccH16> {
         return Rec{(_Float16)yd, (_Float16)(yu - yd)};
     }
     static __device__ __forceinline__ W weight(double fr) {
+        return weight32((float)fr);
+    }
+    static __device__ __forceinline__ W weight32(float fr) {
         // (1, w): 1 is exact; w rounds toward zero (one v_cvt_pkrtz)
-        const auto p = __builtin_amdgcn_cvt_pkrtz(1.0f, (float)fr);
+        const auto p = __builtin_amdgcn_cvt_pkrtz(1.0f, fr);
         return __builtin_bit_cast(W, p);
     }
     static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
@@ -155,6 +190,8 @@ struct Mode<kAccH16> {
 
 template <>
 struct Mode<kAccB16> {
+    // detector position in float32 from a float64 per-angle offset
+    static constexpr bool kF32Pos = true;
     using Rec = bf162_t;
     using W = bf162_t;
     using Acc = float;
@@ -162,7 +199,10 @@ struct Mode<kAccB16> {
         return Rec{(__bf16)yd, (__bf16)(yu - yd)};
     }
     static __device__ __forceinline__ W weight(double fr) {
-        return W{(__bf16)1.0f, (__bf16)(float)fr};
+        return weight32((float)fr);
+    }
+    static __device__ __forceinline__ W weight32(float fr) {
+        return W{(__bf16)1.0f, (__bf16)fr};
     }
     static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
         a = __builtin_amdgcn_fdot2_f32_bf16(q, w, a, false);
@@ -200,6 +240,23 @@ __device__ __forceinline__ void inner_angle(
     using Rec = typename M::Rec;
     const int off = __builtin_amdgcn_readfirstlane((al * SPAN - dlo) * SB);
     const Rec* qbase = s_rec + off;
+    if constexpr (M::kF32Pos) {
+        // f = (y - ic) s + [(x - ic) c + center]; the bracket in float64,
+        // rounded once; |error of f| < 3e-4 px at 8192 columns
+        const float xcc = (float)(xc * c + cen);
+        const float s32 = (float)s;
+#pragma unroll
+        for (int k = 0; k < PY; ++k) {
+            const float f = fmaf((float)yc[k], s32, xcc);
+            const float fl = floorf(f);
+            const int d = (int)fl;
+            const auto w = M::weight32(f - fl);
+            const Rec* q = qbase + d * SB;
+#pragma unroll
+            for (int js = 0; js < SB; ++js) M::add(acc[k][js], q[js], w);
+        }
+        return;
+    }
     const double xcc = xc * c;
 #pragma unroll
     for (int k = 0; k < PY; ++k) {
@@ -212,27 +269,36 @@ __device__ __forceinline__ void inner_angle(
     }
 }
 
+template <class Rec, int SB, int A, int SPAN>
+struct alignas(16) BpShared {
+    alignas(16) Rec rec[A * SPAN * SB];  // 16 B aligned: ds_read_b128 records
+    double cos_[A], sin_[A];
+    int dlo[A], inner[A];
+    float edge[A * SB * 2];
+};
+
+// Back-projects tile (tx, ty) of slice group grp of one stack.  Called by
+// every thread of the workgroup; starts and ends with all LDS consumed.
 template <int MODE, int SB, int PY, int A, typename Tin>
-__global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
-    BpArgs a, int tiles_x, int tiles_y) {
+__device__ __forceinline__ void bp_tile(
+    const BpArgs& a, int tx, int ty, int grp,
+    BpShared<typename Mode<MODE>::Rec, SB, A,
+             mode_span<Mode<MODE>>(kTileX, 4 * PY)>& sh) {
     using M = Mode<MODE>;
     using Rec = typename M::Rec;
     constexpr int TY = 4 * PY;
-    constexpr int SPAN = bp_span(kTileX, TY);
-    __shared__ Rec s_rec[A * SPAN * SB];
-    __shared__ double s_cos[A], s_sin[A];
-    __shared__ int s_dlo[A], s_inner[A];
-    __shared__ float s_edge[A * SB * 2];
+    constexpr int SPAN = mode_span<M>(kTileX, TY);
+    Rec* s_rec = sh.rec;
+    double* s_cos = sh.cos_;
+    double* s_sin = sh.sin_;
+    int* s_dlo = sh.dlo;
+    int* s_inner = sh.inner;
+    float* s_edge = sh.edge;
 
     const Tin* __restrict__ sino = static_cast<const Tin*>(a.sino);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
-    unsigned bid = blockIdx.x;
-    const int tx = bid % tiles_x;
-    bid /= tiles_x;
-    const int ty = bid % tiles_y;
-    const int grp = bid / tiles_y;
     const int x0 = tx * kTileX, y0 = ty * TY, j0 = grp * SB;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
     const double ic = a.icenter, cen = a.center, w1 = (double)(C - 1);
@@ -269,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
             const double fmx = fmax(fmax(f00, f01), fmax(f10, f11));
             s_cos[tid] = c;
             s_sin[tid] = s;
-            s_dlo[tid] = (int)floor(fmn);
+            s_dlo[tid] = (int)floor(fmn) - (M::kF32Pos ? 1 : 0);
             inner = (fmn >= 0.0) && (fmx <= w1);
             s_inner[tid] = inner;
         }
@@ -332,14 +398,28 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
                 const double c = s_cos[al], s = s_sin[al];
                 const int dlo = s_dlo[al];
                 const double xcc = xc * c;
+                // float32-position modes interpolate exactly as inner_angle
+                // does (so results do not depend on the tiling); the range
+                // test is always the exact float64 one
+                const float xcc32 = (float)(xc * c + cen);
+                const float s32 = (float)s;
                 const Rec* base = s_rec + al * SPAN * SB;
 #pragma unroll
                 for (int k = 0; k < PY; ++k) {
                     const double f = det_pos(xcc, yc[k], s, cen);
                     const bool in = (f >= 0.0) && (f <= w1);
-                    const int d = (int)f;
+                    int d;
+                    typename M::W w;
+                    if constexpr (M::kF32Pos) {
+                        const float f32 = fmaf((float)yc[k], s32, xcc32);
+                        const float fl = floorf(f32);
+                        d = (int)fl;
+                        w = M::weight32(f32 - fl);
+                    } else {
+                        d = (int)f;
+                        w = M::weight(f - (double)d);
+                    }
                     const int idx = min(max(d - dlo, 0), SPAN - 1);
-                    const auto w = M::weight(f - (double)d);
                     const Rec* q = base + idx * SB;
                     if (in) {
 #pragma unroll
@@ -370,6 +450,82 @@ __global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
             }
         }
     }
+}
+
+
+template <int MODE, int SB, int PY, int A, typename Tin>
+__global__ __launch_bounds__(kThreads) void bp_tiled_kernel(
+    BpArgs a, int tiles_x, int tiles_y) {
+    __shared__ BpShared<typename Mode<MODE>::Rec, SB, A,
+                        mode_span<Mode<MODE>>(kTileX, 4 * PY)> sh;
+    unsigned bid = blockIdx.x;
+    const int tx = bid % tiles_x;
+    bid /= tiles_x;
+    const int ty = bid % tiles_y;
+    const int grp = bid / tiles_y;
+    bp_tile<MODE, SB, PY, A, Tin>(a, tx, ty, grp, sh);
+}
+
+// Persistent kernel over a work list of (stack, tile) items from many stacks
+// of different widths, angle counts and centres.  Workgroups take the next
+// item from a device counter (reset by the preceding reset_counter launch)
+// until the list is exhausted; every workgroup reaches that exit.
+template <int MODE, int SB, int PY, int A, typename Tin>
+__global__ __launch_bounds__(kThreads) void bp_batched_kernel(
+    const ath_bp_job* __restrict__ jobs, int njobs, long long total,
+    int* __restrict__ counter) {
+    __shared__ BpShared<typename Mode<MODE>::Rec, SB, A,
+                        mode_span<Mode<MODE>>(kTileX, 4 * PY)> sh;
+    __shared__ long long s_item;
+    __shared__ int s_job;
+    constexpr int TY = 4 * PY;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const long long item = atomicAdd(counter, 1);
+            int lo = 0, hi = njobs - 1;  // last job with tile_begin <= item
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (jobs[mid].tile_begin <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            s_item = item;
+            s_job = lo;
+        }
+        __syncthreads();
+        const long long item = s_item;
+        const int j = s_job;
+        __syncthreads();
+        if (item >= total) break;
+        const ath_bp_job jb = jobs[j];
+        BpArgs a;
+        a.sino = jb.sino;
+        a.recon = jb.recon;
+        a.angles = jb.angles;
+        a.nrow = jb.nrow;
+        a.nslice = jb.nslice;
+        a.ncol = jb.ncol;
+        a.center = (double)jb.center;
+        a.icenter = 0.5 * (double)(jb.ncol - 1);
+        a.edge_pad = jb.edge_pad;
+        a.use_mask = jb.mask_radius >= 0.0;
+        a.mask_r2 = jb.mask_radius * jb.mask_radius;
+        a.mask_c = jb.ncol / 2;
+        a.apply_scale = jb.apply_scale;
+        a.pi32 = (float)M_PI;
+        a.denom = (float)(jb.nrow - 1);
+        const int tiles_x = (jb.ncol + kTileX - 1) / kTileX;
+        const int tiles_y = (jb.ncol + TY - 1) / TY;
+        long long local = item - jb.tile_begin;
+        const int tx = (int)(local % tiles_x);
+        local /= tiles_x;
+        const int ty = (int)(local % tiles_y);
+        const int grp = (int)(local / tiles_y);
+        bp_tile<MODE, SB, PY, A, Tin>(a, tx, ty, grp, sh);
+    }
+}
+
+__global__ void reset_counter_kernel(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
 // One thread per (x, y, slice), sinogram read straight from global memory.
@@ -418,6 +574,54 @@ int launch_tiled(const BpArgs& a, hipStream_t stream) {
     return check_hip(hipGetLastError(), "bp_tiled_kernel launch");
 }
 
+template <int MODE, int SB, int PY, int A, typename Tin>
+int launch_batched(const ath_bp_job* jobs, int njobs, long long total,
+                   int nblocks, int* counter, hipStream_t stream) {
+    auto kern = bp_batched_kernel<MODE, SB, PY, A, Tin>;
+    static int resident = 0;  // workgroups resident at once, per kernel
+    if (nblocks <= 0 && resident > 0) nblocks = resident;
+    if (nblocks <= 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        int st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                               &per_cu, reinterpret_cast<const void*>(kern),
+                               kThreads, 0),
+                           "occupancy query");
+        if (st != ATH_OK) return st;
+        st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+        if (st != ATH_OK) return st;
+        st = check_hip(hipDeviceGetAttribute(
+                           &cus, hipDeviceAttributeMultiprocessorCount, dev),
+                       "CU count");
+        if (st != ATH_OK) return st;
+        nblocks = (per_cu > 0 ? per_cu : 1) * cus;
+        resident = nblocks;
+    }
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream,
+                       counter);
+    int st = check_hip(hipGetLastError(), "reset_counter_kernel launch");
+    if (st != ATH_OK) return st;
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kThreads), 0, stream, jobs,
+                       njobs, total, counter);
+    return check_hip(hipGetLastError(), "bp_batched_kernel launch");
+}
+
+template <int MODE, typename Tin>
+int batched_config(const ath_bp_job* jobs, int njobs, long long total,
+                   int sb, int py, int nblocks, int* counter, hipStream_t s) {
+    constexpr int A = 16;
+#define ATH_BPB_CASE(SB_, PY_)                                             \
+    if (sb == SB_ && py == PY_)                                            \
+        return launch_batched<MODE, SB_, PY_, A, Tin>(jobs, njobs, total,  \
+                                                      nblocks, counter, s);
+    ATH_BPB_CASE(1, 4) ATH_BPB_CASE(1, 8) ATH_BPB_CASE(1, 16)
+    ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
+#undef ATH_BPB_CASE
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "batched back-projection: no kernel for slice_batch " +
+                         std::to_string(sb) + ", tile_rows " +
+                         std::to_string(4 * py));
+}
+
 // Tile configurations compiled per mode: slice batch SB and rows per lane
 // PY (tile 64 x 4*PY).  16-bit records also allow SB = 8 (32 bytes a pixel,
 // the same as SB = 4 float32 records).
@@ -446,7 +650,7 @@ template <int MODE>
 void default_config(int nslice, int& sb, int& py) {
     sb = nslice <= 2 ? nslice : 4;
     py = 8;
-    if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == kAccH16)) py = 4;
+    if (nslice > 2 && MODE == ATH_ACC_F64) py = 4;
 }
 
 template <int MODE, typename Tin>
@@ -459,6 +663,16 @@ int dispatch(const BpArgs& a, int sb, int py, hipStream_t s) {
 
 int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb,
                     int py, hipStream_t s) {
+    if (acc == ATH_ACC_DERIV) {
+        if (sino_dtype != ATH_F32)
+            return set_error(ATH_ERR_UNSUPPORTED,
+                             "centre derivative needs a float32 sinogram");
+        constexpr int A = 16;
+        const int n = a.nslice;
+        if (n == 1) return launch_tiled<ATH_ACC_DERIV, 1, 8, A, float>(a, s);
+        if (n == 2) return launch_tiled<ATH_ACC_DERIV, 2, 8, A, float>(a, s);
+        return launch_tiled<ATH_ACC_DERIV, 4, 8, A, float>(a, s);
+    }
     if (sino_dtype == ATH_F32) {
         switch (acc) {
         case ATH_ACC_F64: return dispatch<ATH_ACC_F64, float>(a, sb, py, s);
@@ -475,6 +689,36 @@ int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb,
     if (sino_dtype == ATH_F16) return dispatch<kAccH16, __half>(a, sb, py, s);
     if (sino_dtype == ATH_BF16)
         return dispatch<kAccB16, __hip_bfloat16>(a, sb, py, s);
+    return set_error(ATH_ERR_INVALID_ARG, "back-projection: bad sinogram dtype");
+}
+
+int back_projection_batched(const ath_bp_job* jobs, int njobs,
+                            long long total, int sino_dtype, int acc, int sb,
+                            int py, int nblocks, int* counter, hipStream_t s) {
+    if (sino_dtype == ATH_F32) {
+        switch (acc) {
+        case ATH_ACC_F64:
+            return batched_config<ATH_ACC_F64, float>(jobs, njobs, total, sb,
+                                                      py, nblocks, counter, s);
+        case ATH_ACC_F32SEQ:
+            return batched_config<ATH_ACC_F32SEQ, float>(
+                jobs, njobs, total, sb, py, nblocks, counter, s);
+        case ATH_ACC_F32:
+            return batched_config<ATH_ACC_F32, float>(jobs, njobs, total, sb,
+                                                      py, nblocks, counter, s);
+        }
+        return set_error(ATH_ERR_INVALID_ARG, "back-projection: bad acc mode");
+    }
+    if (acc != ATH_ACC_F32)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "back-projection: float16/bfloat16 sinograms need "
+                         "acc = ATH_ACC_F32");
+    if (sino_dtype == ATH_F16)
+        return batched_config<kAccH16, __half>(jobs, njobs, total, sb, py,
+                                               nblocks, counter, s);
+    if (sino_dtype == ATH_BF16)
+        return batched_config<kAccB16, __hip_bfloat16>(
+            jobs, njobs, total, sb, py, nblocks, counter, s);
     return set_error(ATH_ERR_INVALID_ARG, "back-projection: bad sinogram dtype");
 }
 

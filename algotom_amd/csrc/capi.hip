@@ -232,6 +232,57 @@ int ath_back_projection_tiled(const void* sino, int sino_dtype, int nrow,
                            static_cast<hipStream_t>(stream));
 }
 
+long long ath_bp_job_tiles(int ncol, int nslice, int slice_batch,
+                           int tile_rows) {
+    if (ncol < 1 || nslice < 1 || slice_batch < 1 || tile_rows < 4 ||
+        tile_rows % 4)
+        return 0;
+    const long long tx = (ncol + 63) / 64, ty = (ncol + tile_rows - 1) / tile_rows;
+    return tx * ty * ((nslice + slice_batch - 1) / slice_batch);
+}
+
+int ath_back_projection_batched(const ath_bp_job* jobs, int njobs,
+                                long long total_tiles, int sino_dtype,
+                                int acc, int slice_batch, int tile_rows,
+                                int nblocks, int* counter, void* stream) {
+    clear_error();
+    if (!jobs || !counter)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (njobs < 1 || total_tiles < 1 || total_tiles > 0x7fffffffLL)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "need 1 <= njobs and 1 <= total_tiles < 2^31");
+    if (slice_batch < 1 || tile_rows < 4 || tile_rows % 4)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "slice_batch must be >= 1, tile_rows a multiple of 4");
+    return back_projection_batched(jobs, njobs, total_tiles, sino_dtype, acc,
+                                   slice_batch, tile_rows / 4, nblocks,
+                                   counter, static_cast<hipStream_t>(stream));
+}
+
+int ath_ramp_filter_batched(const ath_filter_job* jobs, int njobs,
+                            long long total_rows, int in_dtype,
+                            int out_dtype, int fft_len, int pad_mode,
+                            int apply_log, void* stream) {
+    clear_error();
+    if (!jobs) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (njobs < 1 || total_rows < 1 || total_rows > 0x7fffffffLL)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "need 1 <= njobs and 1 <= total_rows < 2^31");
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+        return set_error(ATH_ERR_INVALID_ARG, "unsupported pad mode");
+    if (!valid_dtype(out_dtype) ||
+        (in_dtype != ATH_F32 && in_dtype != ATH_F64))
+        return set_error(ATH_ERR_INVALID_ARG, "bad dtype");
+    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)))
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "fft_len must be a power of two in [64, 16384]");
+    // every job must satisfy fft_len >= 2*(ncol+2*pad) - pad: checked by the
+    // caller, who built each job's plan for this fft_len
+    return ramp_filter_batched(jobs, njobs, total_rows, in_dtype, out_dtype,
+                               fft_len, pad_mode, apply_log,
+                               static_cast<hipStream_t>(stream));
+}
+
 int ath_back_projection_naive(const float* sino, int nrow, int nslice,
                               int ncol, const float* angles, float center,
                               int edge_pad, int acc, double mask_radius,
@@ -244,6 +295,59 @@ int ath_back_projection_naive(const float* sino, int nrow, int nslice,
     if (nslice > 65535)
         return set_error(ATH_ERR_UNSUPPORTED, "naive kernel: nslice > 65535");
     return back_projection_naive(a, acc, static_cast<hipStream_t>(stream));
+}
+
+int ath_forward_projection(const float* image, int nrow, int nslice,
+                           int ncol, const float* angles, float center,
+                           double mask_radius, double scale, float* sino,
+                           void* stream) {
+    clear_error();
+    if (!image || !angles || !sino)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    int st = check_geometry(nrow, nslice, ncol);
+    if (st != ATH_OK) return st;
+    if (nrow > 65535)
+        return set_error(ATH_ERR_UNSUPPORTED, "forward projection: nrow > 65535");
+    if (!std::isfinite(center) || !std::isfinite(scale))
+        return set_error(ATH_ERR_INVALID_ARG, "center and scale must be finite");
+    FpArgs a;
+    a.image = image;
+    a.sino = sino;
+    a.angles = angles;
+    a.nrow = nrow;
+    a.nslice = nslice;
+    a.ncol = ncol;
+    a.center = (double)center;
+    a.icenter = 0.5 * (double)(ncol - 1);
+    a.use_mask = mask_radius >= 0.0 ? 1 : 0;
+    a.mask_r2 = mask_radius * mask_radius;
+    a.mask_c = ncol / 2;
+    a.scale = scale;
+    return forward_projection(a, static_cast<hipStream_t>(stream));
+}
+
+int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
+                            int pad, int pad_mode, const void* plan,
+                            int fft_len, int nwin, float* out, void* stream) {
+    clear_error();
+    if (!in || !plan || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    int st = check_geometry(nrow, nslice, ncol);
+    if (st != ATH_OK) return st;
+    if (pad < 0) return set_error(ATH_ERR_INVALID_ARG, "pad must be >= 0");
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+        return set_error(ATH_ERR_INVALID_ARG, "unsupported pad mode");
+    if (nwin != 1 && nwin != nrow)
+        return set_error(ATH_ERR_INVALID_ARG, "nwin must be 1 or nrow");
+    const long long width = (long long)ncol + 2LL * pad;
+    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)) ||
+        (long long)fft_len < 2 * width)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "adjoint filter: fft_len must be a power of two "
+                         ">= 2*(ncol+2*pad)");
+    return ramp_filter_adjoint(in, nrow, nslice, ncol, pad, pad_mode, plan,
+                               fft_len, nwin, out,
+                               static_cast<hipStream_t>(stream));
 }
 
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,

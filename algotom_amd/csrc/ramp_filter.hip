@@ -152,20 +152,20 @@ __device__ __forceinline__ double load_sample<double>(const double* row,
     return apply_log ? -log(x) : x;
 }
 
-// One workgroup per row (angle i, slice j) of a (nrow, nslice, ncol) stack.
-template <int MAXB, typename Tin, typename Tout>
-__global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
-    const Tin* __restrict__ sino, Tout* __restrict__ out, int nslice,
+// Filters one row in LDS buffer `buf` (M double2): pad, rfft, multiply by
+// the kernel half spectrum G, irfft, crop into dst.  All 256 threads.
+// ADJ: the transpose of the filter -- the row is embedded with zeros
+// (pad_mode is then the FORWARD pass's mode, used only to fold), the kernel
+// spectrum is that of the reversed taps, and every one of the W outputs of
+// the circular correlation is folded back through the transpose of the pad
+// index map instead of cropped.
+template <int MAXB, typename Tin, typename Tout, bool ADJ = false>
+__device__ __forceinline__ void filter_row(
+    double2* buf, const Tin* __restrict__ src_row, Tout* __restrict__ dst,
     int ncol, int pad, int pad_mode, int apply_log,
-    const double2* __restrict__ tw, const double2* __restrict__ spec,
-    int nwin, int M) {
-    extern __shared__ double2 buf[];
+    const double2* __restrict__ tw, const double2* __restrict__ G, int M) {
     const int tid = threadIdx.x;
-    const size_t row = blockIdx.x;
     const int width = ncol + 2 * pad;
-    const Tin* src_row = sino + row * (size_t)ncol;
-    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
-
     // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
     for (int n = tid; n < M; n += kThreads) {
         double v[2];
@@ -174,7 +174,8 @@ __global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
             const int k = 2 * n + h;
             v[h] = 0.0;
             if (k < width) {
-                const int s = pad_source(k - pad, ncol, pad_mode);
+                const int s = ADJ ? pad_source(k - pad, ncol, ATH_PAD_CONSTANT)
+                                  : pad_source(k - pad, ncol, pad_mode);
                 if (s >= 0) v[h] = load_sample<Tin>(src_row, s, apply_log);
             }
         }
@@ -210,9 +211,84 @@ __global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
     lds_fft<1, MAXB>(buf, M, tw);
 
     const double* y = reinterpret_cast<const double*>(buf);
-    Tout* dst = out + row * (size_t)ncol;
-    const int off = width + pad;
-    for (int c = tid; c < ncol; c += kThreads) store_elem(dst + c, y[off + c]);
+    if (!ADJ) {
+        const int off = width + pad;
+        for (int c = tid; c < ncol; c += kThreads)
+            store_elem(dst + c, y[off + c]);
+    } else {
+        // z[n] = y[W + n], n in [0, W); out[c] = sum of z[n] over the n whose
+        // pad source is c (the interior n = pad + c, plus folded pad samples)
+        const double* z = y + width;
+        for (int c = tid; c < ncol; c += kThreads) {
+            double v = z[pad + c];
+            if (pad_mode == ATH_PAD_EDGE) {
+                if (c == 0)
+                    for (int n = 0; n < pad; ++n) v += z[n];
+                if (c == ncol - 1)
+                    for (int n = pad + ncol; n < width; ++n) v += z[n];
+            } else if (pad_mode != ATH_PAD_CONSTANT) {
+                for (int n = 0; n < pad; ++n)
+                    if (pad_source(n - pad, ncol, pad_mode) == c) v += z[n];
+                for (int n = pad + ncol; n < width; ++n)
+                    if (pad_source(n - pad, ncol, pad_mode) == c) v += z[n];
+            }
+            store_elem(dst + c, v);
+        }
+    }
+    __syncthreads();  // buf free for the caller's next row
+}
+
+// One workgroup per row (angle i, slice j) of a (nrow, nslice, ncol) stack.
+template <int MAXB, typename Tin, typename Tout>
+__global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
+    const Tin* __restrict__ sino, Tout* __restrict__ out, int nslice,
+    int ncol, int pad, int pad_mode, int apply_log,
+    const double2* __restrict__ tw, const double2* __restrict__ spec,
+    int nwin, int M) {
+    extern __shared__ double2 buf[];
+    const size_t row = blockIdx.x;
+    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+    filter_row<MAXB, Tin, Tout>(buf, sino + row * (size_t)ncol,
+                                out + row * (size_t)ncol, ncol, pad, pad_mode,
+                                apply_log, tw, G, M);
+}
+
+// Transpose of ramp_filter_kernel (float32 in and out): one workgroup per row.
+template <int MAXB>
+__global__ __launch_bounds__(kThreads) void ramp_filter_adjoint_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int nslice,
+    int ncol, int pad, int pad_mode, const double2* __restrict__ tw,
+    const double2* __restrict__ spec, int nwin, int M) {
+    extern __shared__ double2 buf[];
+    const size_t row = blockIdx.x;
+    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+    filter_row<MAXB, float, float, true>(buf, in + row * (size_t)ncol,
+                                         out + row * (size_t)ncol, ncol, pad,
+                                         pad_mode, 0, tw, G, M);
+}
+
+// One workgroup per row of many stacks sharing the FFT length L = 2M.
+template <int MAXB, typename Tin, typename Tout>
+__global__ __launch_bounds__(kThreads) void ramp_filter_batched_kernel(
+    const ath_filter_job* __restrict__ jobs, int njobs, int pad_mode,
+    int apply_log, int M) {
+    extern __shared__ double2 buf[];
+    const long long grow = blockIdx.x;
+    int lo = 0, hi = njobs - 1;  // last job with row_begin <= grow
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row_begin <= grow) lo = mid;
+        else hi = mid - 1;
+    }
+    const ath_filter_job jb = jobs[lo];
+    const size_t row = (size_t)(grow - jb.row_begin);
+    const double2* tw = static_cast<const double2*>(jb.plan);
+    const double2* G = tw + 2 * M +
+                       (jb.nwin > 1 ? (row / jb.nslice) : 0) * (size_t)(M + 1);
+    filter_row<MAXB, Tin, Tout>(
+        buf, static_cast<const Tin*>(jb.sino) + row * (size_t)jb.ncol,
+        static_cast<Tout*>(jb.out) + row * (size_t)jb.ncol, jb.ncol, jb.pad,
+        pad_mode, apply_log, tw, G, M);
 }
 
 // tw[n] = exp(-2 pi i n / L)
@@ -263,17 +339,18 @@ int maxb_for(int M) {
     return 0;
 }
 
+// Allows dynamic LDS above 64 KB for `kernel`, once per kernel (a host-side
+// attribute, so later launches, including graph-captured ones, enqueue only
+// the kernel).  `bytes` is the largest size the kernel is launched with.
 template <typename K>
-int prepare_lds(K kernel, size_t bytes) {
-    if (bytes > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(kernel),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-        }
-    }
-    return ATH_OK;
+void prepare_lds(K kernel, size_t bytes) {
+    static bool done = false;  // one flag per kernel instantiation
+    if (done || bytes <= 64 * 1024) return;
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    done = true;
 }
 
 template <int MAXB, typename Tin, typename Tout>
@@ -285,7 +362,7 @@ int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
     const double2* spec = tw + fft_len;
     const size_t lds = (size_t)M * sizeof(double2);
     auto kern = ramp_filter_kernel<MAXB, Tin, Tout>;
-    prepare_lds(kern, lds);
+    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
     const size_t rows = (size_t)nrow * nslice;
     hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kThreads), lds, stream,
                        static_cast<const Tin*>(sino), static_cast<Tout*>(out),
@@ -319,6 +396,54 @@ int dispatch_out(int out_dtype, const void* sino, void* out, int nrow,
     return set_error(ATH_ERR_INVALID_ARG, "ramp filter: bad output dtype");
 }
 
+template <int MAXB, typename Tin, typename Tout>
+int launch_filter_batched(const ath_filter_job* jobs, int njobs,
+                          long long total_rows, int fft_len, int pad_mode,
+                          int apply_log, hipStream_t stream) {
+    const int M = fft_len / 2;
+    const size_t lds = (size_t)M * sizeof(double2);
+    auto kern = ramp_filter_batched_kernel<MAXB, Tin, Tout>;
+    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
+    hipLaunchKernelGGL(kern, dim3((unsigned)total_rows), dim3(kThreads), lds,
+                       stream, jobs, njobs, pad_mode, apply_log, M);
+    return check_hip(hipGetLastError(), "ramp_filter_batched_kernel launch");
+}
+
+template <int MAXB, typename Tin>
+int batched_out(int out_dtype, const ath_filter_job* jobs, int njobs,
+                long long total_rows, int fft_len, int pad_mode,
+                int apply_log, hipStream_t s) {
+    switch (out_dtype) {
+    case ATH_F32:
+        return launch_filter_batched<MAXB, Tin, float>(
+            jobs, njobs, total_rows, fft_len, pad_mode, apply_log, s);
+    case ATH_F64:
+        return launch_filter_batched<MAXB, Tin, double>(
+            jobs, njobs, total_rows, fft_len, pad_mode, apply_log, s);
+    case ATH_F16:
+        return launch_filter_batched<MAXB, Tin, __half>(
+            jobs, njobs, total_rows, fft_len, pad_mode, apply_log, s);
+    case ATH_BF16:
+        return launch_filter_batched<MAXB, Tin, __hip_bfloat16>(
+            jobs, njobs, total_rows, fft_len, pad_mode, apply_log, s);
+    }
+    return set_error(ATH_ERR_INVALID_ARG, "ramp filter: bad output dtype");
+}
+
+template <int MAXB>
+int batched_in(int in_dtype, int out_dtype, const ath_filter_job* jobs,
+               int njobs, long long total_rows, int fft_len, int pad_mode,
+               int apply_log, hipStream_t s) {
+    if (in_dtype == ATH_F32)
+        return batched_out<MAXB, float>(out_dtype, jobs, njobs, total_rows,
+                                        fft_len, pad_mode, apply_log, s);
+    if (in_dtype == ATH_F64)
+        return batched_out<MAXB, double>(out_dtype, jobs, njobs, total_rows,
+                                         fft_len, pad_mode, apply_log, s);
+    return set_error(ATH_ERR_INVALID_ARG,
+                     "ramp filter: input dtype must be float32 or float64");
+}
+
 template <int MAXB>
 int dispatch_in(int in_dtype, int out_dtype, const void* sino, void* out,
                 int nrow, int nslice, int ncol, int pad, int pad_mode,
@@ -342,7 +467,7 @@ int launch_spectrum(const double* taps, int nwin, int width,
                     hipStream_t stream) {
     const size_t lds = (size_t)M * sizeof(double2);
     auto kern = kernel_spectrum_kernel<MAXB>;
-    prepare_lds(kern, lds);
+    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
     hipLaunchKernelGGL(kern, dim3((unsigned)nwin), dim3(kThreads), lds,
                        stream, taps, width, tw, spec, M);
     return check_hip(hipGetLastError(), "kernel_spectrum_kernel launch");
@@ -377,6 +502,60 @@ int plan_build(const double* taps, int nwin, int width, int fft_len,
     case 2: return launch_spectrum<2>(taps, nwin, width, tw, spec, M, stream);
     case 4: return launch_spectrum<4>(taps, nwin, width, tw, spec, M, stream);
     case 8: return launch_spectrum<8>(taps, nwin, width, tw, spec, M, stream);
+    }
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "ramp filter: FFT length above 16384 is not supported");
+}
+
+int ramp_filter_batched(const ath_filter_job* jobs, int njobs,
+                        long long total_rows, int in_dtype, int out_dtype,
+                        int fft_len, int pad_mode, int apply_log,
+                        hipStream_t s) {
+    switch (maxb_for(fft_len / 2)) {
+    case 1: return batched_in<1>(in_dtype, out_dtype, jobs, njobs, total_rows,
+                                 fft_len, pad_mode, apply_log, s);
+    case 2: return batched_in<2>(in_dtype, out_dtype, jobs, njobs, total_rows,
+                                 fft_len, pad_mode, apply_log, s);
+    case 4: return batched_in<4>(in_dtype, out_dtype, jobs, njobs, total_rows,
+                                 fft_len, pad_mode, apply_log, s);
+    case 8: return batched_in<8>(in_dtype, out_dtype, jobs, njobs, total_rows,
+                                 fft_len, pad_mode, apply_log, s);
+    }
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "ramp filter: FFT length above 16384 is not supported");
+}
+
+namespace {
+template <int MAXB>
+int launch_adjoint(const float* in, int nrow, int nslice, int ncol, int pad,
+                   int pad_mode, const void* plan, int fft_len, int nwin,
+                   float* out, hipStream_t stream) {
+    const int M = fft_len / 2;
+    const double2* tw = static_cast<const double2*>(plan);
+    const double2* spec = tw + fft_len;
+    const size_t lds = (size_t)M * sizeof(double2);
+    auto kern = ramp_filter_adjoint_kernel<MAXB>;
+    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
+    const size_t rows = (size_t)nrow * nslice;
+    hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kThreads), lds, stream,
+                       in, out, nslice, ncol, pad, pad_mode, tw, spec, nwin,
+                       M);
+    return check_hip(hipGetLastError(), "ramp_filter_adjoint_kernel launch");
+}
+}  // namespace
+
+int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
+                        int pad, int pad_mode, const void* plan, int fft_len,
+                        int nwin, float* out, hipStream_t s) {
+    switch (maxb_for(fft_len / 2)) {
+    case 1: return launch_adjoint<1>(in, nrow, nslice, ncol, pad, pad_mode,
+                                     plan, fft_len, nwin, out, s);
+    case 2: return launch_adjoint<2>(in, nrow, nslice, ncol, pad, pad_mode,
+                                     plan, fft_len, nwin, out, s);
+    case 4: return launch_adjoint<4>(in, nrow, nslice, ncol, pad, pad_mode,
+                                     plan, fft_len, nwin, out, s);
+    case 8: return launch_adjoint<8>(in, nrow, nslice, ncol, pad, pad_mode,
+                                     plan, fft_len, nwin, out, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
                      "ramp filter: FFT length above 16384 is not supported");

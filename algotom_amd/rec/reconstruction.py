@@ -31,6 +31,7 @@ import numpy as np
 from scipy.signal import windows as _windows
 
 from algotom_amd import _native as nat
+from algotom_amd.rec import autotune
 
 _WINDOWS = ("hann", "bartlett", "blackman", "hamming", "nuttall", "parzen",
             "triang")
@@ -235,14 +236,19 @@ def _backproject_stack(filt_t, angles, center, acc, edge_pad=False,
                          "number of the sinogram!!!")
     out = torch.empty((ncol, nslice, ncol), dtype=torch.float32, device=dev)
     lib = nat.load()
-    nat.check(lib.ath_back_projection(nat.ptr(filt_t),
-                                      nat.dtype_code(filt_t.dtype), nrow,
-                                      nslice, ncol, nat.ptr(ang),
-                                      float(np.float32(center)),
-                                      int(bool(edge_pad)), int(acc),
-                                      float(mask_radius), int(bool(apply_scale)),
-                                      nat.ptr(out), nat.stream_ptr(dev)),
-              "ath_back_projection")
+    args = (nat.ptr(filt_t), nat.dtype_code(filt_t.dtype), nrow, nslice, ncol,
+            nat.ptr(ang), float(np.float32(center)), int(bool(edge_pad)),
+            int(acc), float(mask_radius), int(bool(apply_scale)), nat.ptr(out))
+    cfg = autotune.lookup(autotune.make_key(
+        "stack", acc, str(filt_t.dtype).split(".")[-1], ncol, nslice))
+    if cfg is None:
+        nat.check(lib.ath_back_projection(*args, nat.stream_ptr(dev)),
+                  "ath_back_projection")
+    else:
+        nat.check(lib.ath_back_projection_tiled(*args, int(cfg[0]),
+                                                int(cfg[1]),
+                                                nat.stream_ptr(dev)),
+                  "ath_back_projection_tiled")
     return out
 
 

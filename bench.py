@@ -17,6 +17,14 @@ cpu_baseline: the oracle's OpenMP C port (numba-typed restatement of the
 reference's CPU path) on 4 of the 64 sinograms, on this host's cores.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32]
+
+--workload uneven: 256 sinograms with widths from {1024, 2048, 2560, 4096},
+angle counts from {901, 1801, 3601} over [0, pi) and centres within +-20 px
+of mid-width (seed 3000), exact projections, reconstructed as one batch: one
+filter launch per FFT length and one persistent back-projection launch,
+captured in a HIP graph, tile height autotuned; every precision mode is
+timed and its error against the all-FP64 oracle reported on sampled
+sinograms.
 """
 
 import argparse
@@ -98,6 +106,96 @@ def cpu_baseline(sino_dev, angles, gpu_out, nsample=4):
                       "%.2f s" % (nsample, NSLICE, NROW, NCOL, dt)}, err
 
 
+def run_uneven(args, dev):
+    import torch
+    from algotom_amd.rec import batch
+    from algotom_amd.util import phantom
+    from oracle import port
+    rng = np.random.default_rng(3000)
+    nsino = 256
+    widths = rng.choice([1024, 2048, 2560, 4096], nsino)
+    nangles = rng.choice([901, 1801, 3601], nsino)
+    centers = [float((w - 1) / 2 + rng.uniform(-20.0, 20.0)) for w in widths]
+    angles = [np.linspace(0.0, np.pi, int(n), endpoint=False) for n in nangles]
+    shapes = [(int(n), int(w)) for n, w in zip(nangles, widths)]
+    sinos = []
+    for i, ((nrow, ncol), c) in enumerate(zip(shapes, centers)):
+        prm = phantom.random_ellipses(ncol, 20, seed=3000 + i)
+        sinos.append(phantom.ellipse_sinogram(
+            prm, angles[i], ncol, np.float32(c), device=dev).float())
+    torch.cuda.synchronize()
+    updates = sum(n * w * w for n, w in shapes)
+    # FP64 oracle on the two smallest sinograms
+    sample = sorted(range(nsino), key=lambda i: shapes[i][0] * shapes[i][1])[:2]
+    refs = {i: port.fbp_f64(sinos[i].cpu().numpy(), centers[i], angles[i],
+                            apply_log=False) for i in sample}
+    # the plain drop-in: one fbp_reconstruction call per sinogram (device
+    # tensors in and out, reference arithmetic, no batching or graph)
+    from algotom_amd.rec import reconstruction as rec
+    for i in range(min(4, nsino)):
+        rec.fbp_reconstruction(sinos[i], centers[i], angles[i],
+                               apply_log=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(nsino):
+        rec.fbp_reconstruction(sinos[i], centers[i], angles[i],
+                               apply_log=False)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    modes = {}
+    for prec in (None, "fp32", "fp16", "bf16"):
+        plan = batch.BatchFBP(shapes, centers, angles, apply_log=False,
+                              precision=prec)
+        plan.load(sinos)
+        best, timings = plan.autotune(persist=False)
+        plan.capture()
+        for _ in range(args.warmup):
+            plan.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        errs = []
+        for i in sample:
+            got = plan.outputs[i][:, 0, :].cpu().numpy().astype(np.float64)
+            errs.append(float(np.abs(got - refs[i]).max()
+                              / np.abs(refs[i]).max()))
+        modes[prec or "reference"] = {
+            "ms_per_batch": ms, "gups": updates / (ms * 1e-3) / 1e9,
+            "tile_rows": best[1],
+            "autotune_ms": {str(k[1]): v for k, v in timings.items()},
+            "max_rel_err_vs_fp64": max(errs)}
+        del plan
+        torch.cuda.empty_cache()
+    key = args.precision or "reference"
+    line = {
+        "metric": METRIC, "value": modes[key]["gups"], "unit": "GUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": modes[key]["ms_per_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if args.precision is None else
+                 "%s-storage/fp32-acc" % args.precision,
+        "data": "synthetic",
+        "config": {"workload": "fbp_uneven_batch_256", "nsino": nsino,
+                   "updates": updates, "graph": True,
+                   "parallelism": "single-gpu"},
+        "detail": {"modes": modes,
+                   "plain_per_call": {"ms": plain_ms,
+                                      "gups": updates / (plain_ms * 1e-3)
+                                      / 1e9},
+                   "fp16_batched_over_plain": modes["fp16"]["gups"]
+                   / (updates / (plain_ms * 1e-3) / 1e9),
+                   "bf16_batched_over_plain": modes["bf16"]["gups"]
+                   / (updates / (plain_ms * 1e-3) / 1e9),
+                   "fp16_over_reference_batched": modes["fp16"]["gups"]
+                   / modes["reference"]["gups"],
+                   "fp64_sample": [list(shapes[i]) for i in sample]},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +204,7 @@ def main():
     ap.add_argument("--precision", default=None,
                     choices=[None, "fp32", "fp16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="stack", choices=["stack", "uneven"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU "
@@ -118,6 +217,9 @@ def main():
     from algotom_amd.rec import reconstruction as rec
 
     dev = nat.device()
+    if args.workload == "uneven":
+        run_uneven(args, dev)
+        return
     sino, angles = make_stack(dev)
     updates = NSLICE * NCOL * NCOL * NROW
     pad = min(int(0.15 * NCOL), 150)

@@ -54,6 +54,35 @@ extern "C" {
 #define ATH_ACC_F64 0
 #define ATH_ACC_F32SEQ 1
 #define ATH_ACC_F32 2
+/*  ATH_ACC_DERIV  sum over angles of s[d+1] - s[d] at d = floor f: the
+ *                 derivative of the back-projection with respect to the
+ *                 rotation centre (float64 sum; no reference counterpart) */
+#define ATH_ACC_DERIV 3
+
+/* One stack of a batched launch (ath_ramp_filter_batched): rows of
+ * `sino` (nrow, nslice, ncol) are filtered into `out`; `plan` comes from
+ * ath_filter_plan_build for width ncol + 2*pad and the batch's fft_len.
+ * row_begin = sum of nrow*nslice over the preceding jobs. */
+typedef struct {
+    const void* sino;
+    void* out;
+    const void* plan;
+    int nrow, nslice, ncol, pad, nwin, reserved;
+    long long row_begin;
+} ath_filter_job;
+
+/* One stack of a batched back-projection (ath_back_projection_batched).
+ * tile_begin = sum of ath_bp_job_tiles(...) over the preceding jobs. */
+typedef struct {
+    const void* sino;      /* filtered (nrow, nslice, ncol) */
+    float* recon;          /* (ncol, nslice, ncol) */
+    const float* angles;   /* nrow float32 radians */
+    int nrow, nslice, ncol;
+    float center;
+    double mask_radius;    /* < 0: no mask */
+    int apply_scale, edge_pad;
+    long long tile_begin;
+} ath_bp_job;
 
 /* Library version (major*10000 + minor*100 + patch). No reference
  * counterpart. */
@@ -137,6 +166,34 @@ int ath_back_projection_tiled(const void* sino, int sino_dtype, int nrow,
                               float* recon, int slice_batch, int tile_rows,
                               void* stream);
 
+/* Number of work items (tiles) one stack contributes to a batched
+ * back-projection with the given tile configuration. */
+long long ath_bp_job_tiles(int ncol, int nslice, int slice_batch,
+                           int tile_rows);
+
+/* Back-projects many stacks of different widths, angle counts and centres
+ * in one persistent launch: `nblocks` workgroups (0 = as many as fit on the
+ * device at once) take (stack, tile) items from a device work counter until
+ * all total_tiles items are done.  jobs: device array of njobs descriptors,
+ * ordered by decreasing per-tile cost (nrow) for load balance.  counter:
+ * one device int, reset by a kernel inside this call (no memset, so the
+ * call can be captured in a HIP graph).  Each stack gets exactly the
+ * arithmetic of ath_back_projection.  No reference counterpart: the
+ * reference loops over slices in Python (reconstruction.py:460-490) and over
+ * datasets in SLURM jobs. */
+int ath_back_projection_batched(const ath_bp_job* jobs, int njobs,
+                                long long total_tiles, int sino_dtype,
+                                int acc, int slice_batch, int tile_rows,
+                                int nblocks, int* counter, void* stream);
+
+/* Ramp filter of many stacks of different widths sharing one FFT length,
+ * one launch: a workgroup per row, its stack found from row_begin.
+ * total_rows = sum of nrow*nslice.  Same arithmetic as ath_ramp_filter. */
+int ath_ramp_filter_batched(const ath_filter_job* jobs, int njobs,
+                            long long total_rows, int in_dtype,
+                            int out_dtype, int fft_len, int pad_mode,
+                            int apply_log, void* stream);
+
 /* Same contract, one thread per output pixel reading global memory directly
  * (the reference kernels' structure).  Kept as an independent second
  * implementation for cross-checks; float32 sinograms only. */
@@ -144,6 +201,28 @@ int ath_back_projection_naive(const float* sino, int nrow, int nslice,
                               int ncol, const float* angles, float center,
                               int edge_pad, int acc, double mask_radius,
                               int apply_scale, float* recon, void* stream);
+
+/* Adjoint (transpose) of ath_back_projection: a forward projection with
+ * the back-projector's own linear-interpolation weights.  image:
+ * (ncol, nslice, ncol) float32; sino: (nrow, nslice, ncol) float32 output,
+ * sino = scale * B^T (mask * image) with B the back-projection of
+ * reconstruction.py:156-190 and mask as in ath_back_projection
+ * (mask_radius < 0: none).  Deterministic gather, float64 sums.  nrow must
+ * not exceed 65535.  No reference counterpart (the backward pass of
+ * differentiable FBP). */
+int ath_forward_projection(const float* image, int nrow, int nslice,
+                           int ncol, const float* angles, float center,
+                           double mask_radius, double scale, float* sino,
+                           void* stream);
+
+/* Adjoint of ath_ramp_filter for float32 rows: circular correlation with
+ * the taps (plan built by ath_filter_plan_build from the REVERSED taps
+ * g[(W - n) mod W] with fft_len >= 2 * (ncol + 2 * pad)), then the
+ * transpose of the pad index map pad_mode (edge samples collect the padded
+ * copies).  No reference counterpart. */
+int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
+                            int pad, int pad_mode, const void* plan,
+                            int fft_len, int nwin, float* out, void* stream);
 
 /* Device workspace bytes needed by ath_fbp_reconstruction. */
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,

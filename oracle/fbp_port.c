@@ -121,6 +121,45 @@ void orc_backproject(const float* sino, int nrow, int nslice, int ncol,
     }
 }
 
+/* All-float64 back-projection (no float32 rounding of data, trig or sums):
+ * the reference's formula in exact-as-possible arithmetic, for reporting
+ * errors.  sino (nrow, nslice, ncol) double; recon (ncol, nslice, ncol). */
+void orc_backproject_f64(const double* sino, int nrow, int nslice, int ncol,
+                         const double* cos_t, const double* sin_t,
+                         double center, double* recon, int nthreads) {
+    set_threads(nthreads);
+    const int w1 = ncol - 1;
+    const double icenter = 0.5 * (double)w1;
+    const long long sstride = (long long)nslice * ncol;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < ncol; ++y) {
+        const double yc = (double)y - icenter;
+        double* acc = (double*)calloc((size_t)ncol * nslice, sizeof(double));
+        for (int i = 0; i < nrow; ++i) {
+            const double c = cos_t[i], s = sin_t[i];
+            const double* srow = sino + (long long)i * sstride;
+            for (int x = 0; x < ncol; ++x) {
+                const double f = ((double)x - icenter) * c + yc * s + center;
+                if (!(f >= 0.0 && f <= (double)w1)) continue;
+                const double fl = floor(f);
+                const int d = (int)fl;
+                const double t = f - fl;
+                for (int j = 0; j < nslice; ++j) {
+                    const double* r = srow + (long long)j * ncol;
+                    const double v = t > 0.0 ? r[d] + (r[d + 1] - r[d]) * t
+                                             : r[d];
+                    acc[(long long)j * ncol + x] += v;
+                }
+            }
+        }
+        for (int j = 0; j < nslice; ++j)
+            for (int x = 0; x < ncol; ++x)
+                recon[((long long)y * nslice + j) * ncol + x] =
+                    acc[(long long)j * ncol + x];
+        free(acc);
+    }
+}
+
 /* ------------------------------------------------------------------------
  * Filter: out = crop(real(pad(row) (*)_W g)), g the real kernel.
  * Circular convolution of length W evaluated as a linear convolution with

@@ -39,6 +39,8 @@ def lib():
                                          i, i, p, i]
         _lib.orc_ramp_filter.argtypes = [p, i, i, i, i, i, i, p, p, i]
         _lib.orc_finalize.argtypes = [p, i, i, i, ctypes.c_double, i]
+        _lib.orc_backproject_f64.argtypes = [p, i, i, i, p, p,
+                                             ctypes.c_double, p, i]
     return _lib
 
 
@@ -73,6 +75,52 @@ def backproject(sino, angles, center, edge_pad=False, acc=fbp_np.ACC_F32SEQ,
                           float(np.float32(center)), int(bool(edge_pad)),
                           int(acc), _ptr(out), nthreads or default_threads())
     return out[:, 0, :] if two_d else out
+
+
+def backproject_f64(sino, angles, center, nthreads=None, round_center=True):
+    """All-float64 back-projection: sino (nrow, ncol) or (nrow, nslice, ncol)
+    float64; angles used as float32 (the reference's rounding), trig in
+    float64.  Returns float64 (ncol, ncol) or (ncol, nslice, ncol)."""
+    s = np.ascontiguousarray(sino, dtype=np.float64)
+    two_d = s.ndim == 2
+    if two_d:
+        s = s[:, None, :]
+    nrow, nslice, ncol = s.shape
+    th = -np.asarray(angles, dtype=np.float32).astype(np.float64)
+    c = np.ascontiguousarray(np.cos(th))
+    sn = np.ascontiguousarray(np.sin(th))
+    out = np.empty((ncol, nslice, ncol), dtype=np.float64)
+    cen = float(np.float32(center)) if round_center else float(center)
+    lib().orc_backproject_f64(_ptr(s), nrow, nslice, ncol, _ptr(c), _ptr(sn),
+                              cen, _ptr(out),
+                              nthreads or default_threads())
+    return out[:, 0, :] if two_d else out
+
+
+def fbp_f64(sinogram, center, angles, ratio=1.0, filter_name="hann",
+            pad=None, apply_log=True, nthreads=None, round_center=True):
+    """All-float64 FBP of float32 input (log, filter, BP, mask, scale in
+    float64; only the reference's float32 rounding of angles and centre)."""
+    x = np.asarray(sinogram)
+    input_3d = x.ndim == 3
+    if not input_3d:
+        x = x[:, None, :]
+    nrow, nslice, ncol = x.shape
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    xd = x.astype(np.float64)
+    if apply_log:
+        xd = -np.log(np.where(xd <= 0.0, 1.0, xd))
+    g = fbp_np.filter_kernel(fbp_np.ramp_window_row(ncol + 2 * pad,
+                                                    filter_name))
+    filt = ramp_filter(xd, g, pad, nthreads=nthreads)
+    out = backproject_f64(filt, angles, center, nthreads, round_center)
+    if ratio is not None:
+        if ratio == 0.0:
+            ratio = min(center, ncol - center) / (0.5 * ncol)
+        out = out * fbp_np.circle_mask(ncol, ratio)[:, None, :]
+    out = out * np.pi / (nrow - 1)
+    return out if input_3d else out[:, 0, :]
 
 
 def ramp_filter(rows, g, pad, pad_mode="edge", apply_log=False,
