@@ -55,6 +55,8 @@ SIGNATURES = (
     ("ath_forward_projection", _I, [_P, _I, _I, _I, _P, _F, _D, _D, _P, _P]),
     ("ath_ramp_filter_adjoint", _I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _P,
                                      _P]),
+    ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _I, _I, _I, _I,
+                                             _P, _P, _P]),
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),
     ("ath_fbp_reconstruction", _I, [_P, _I, _I, _I, _I, _P, _F, _D, _P, _I,
                                     _I, _I, _I, _I, _I, _P, _S, _P, _P]),

@@ -100,6 +100,9 @@ struct FpArgs {
     double scale;          // multiplies every output
 };
 int forward_projection(const FpArgs& a, hipStream_t stream);
+int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
+                      int overlap, int side, int norm, int width,
+                      float* factors, float* out, hipStream_t s);
 int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                         int pad, int pad_mode, const void* plan, int fft_len,
                         int nwin, float* out, hipStream_t stream);

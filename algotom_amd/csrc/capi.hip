@@ -350,6 +350,29 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                                static_cast<hipStream_t>(stream));
 }
 
+int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
+                                    int nslice, int ncol, int overlap,
+                                    int side, int norm, int width,
+                                    float* factors, float* out,
+                                    void* stream) {
+    clear_error();
+    if (!sino360 || !out || (norm && !factors))
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    int st = check_geometry(nrow, nslice, ncol);
+    if (st != ATH_OK) return st;
+    if (nrow < 2) return set_error(ATH_ERR_INVALID_ARG, "nrow must be >= 2");
+    if (overlap < 1 || overlap > ncol)
+        return set_error(ATH_ERR_INVALID_ARG, "overlap must be in [1, ncol]");
+    if (side != 0 && side != 1)
+        return set_error(ATH_ERR_INVALID_ARG, "side must be 0 or 1");
+    if ((long long)width < 2LL * ncol - overlap)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "width must be >= 2*ncol - overlap");
+    return stitch_360_to_180(sino360, nrow, nslice, ncol, overlap, side, norm,
+                             width, factors, out,
+                             static_cast<hipStream_t>(stream));
+}
+
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
                                int nwin, int filt_dtype) {
     if (nrow < 1 || nslice < 1 || ncol < 1 || pad < 0 || nwin < 1 ||

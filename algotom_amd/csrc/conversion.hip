@@ -1,0 +1,139 @@
+// Half-acquisition (offset rotation axis) 360 -> 180 degree sinogram
+// conversion on the GPU, the step before FBP for such scans.
+//
+// algotom's convert_sinogram_360_to_180 (algotom/prep/conversion.py:318-364)
+// takes the first N//2+1 rows and the left-right flipped last N//2+1 rows of
+// a 360-degree sinogram and stitches them with a linear ramp over `overlap`
+// columns (stitch_image, conversion.py:76-144; weights from
+// make_weight_matrix, :45-73): the second image is first scaled by the ratio
+// of the mean intensities of the two overlap regions (norm=True).  For an
+// integer overlap no sub-pixel shift is involved.  Here one workgroup per
+// slice reduces the two overlap means (float64 sums), then a streaming
+// kernel writes the stitched rows; every output sample is one or two
+// multiply-adds of input samples, so the pass is HBM-bound.
+#include "ath_internal.h"
+
+namespace ath {
+namespace {
+
+constexpr int kThreads = 256;
+
+// numpy.linspace(1.0, 0.0, ov)[t]: t * step + start, the last value exactly
+// the stop (make_weight_matrix's list_down)
+__device__ __forceinline__ double ramp_down(int t, int ov) {
+    if (ov == 1) return 1.0;
+    if (t == ov - 1) return 0.0;
+    const double step = -1.0 / (double)(ov - 1);
+    return (double)t * step + 1.0;
+}
+
+// factors[2*j] = mean of the first image's overlap, factors[2*j+1] = mean of
+// the second image's overlap (slice j), float64 sums rounded to float32.
+__global__ __launch_bounds__(kThreads) void stitch_means_kernel(
+    const float* __restrict__ sino, int nrow, int nslice, int ncol,
+    int nrow_180, int overlap, int side, float* __restrict__ factors) {
+    __shared__ double red[2][kThreads];
+    const int j = blockIdx.x;
+    const int tid = threadIdx.x;
+    // image 1 = top rows [0, nrow_180); image 2 = rows [nrow - nrow_180,
+    // nrow) flipped left-right.  side 1: mean(img1[:, -ov:]),
+    // mean(img2[:, :ov]) = mean of the bottom rows' last ov columns; side 0:
+    // mean(img1[:, :ov]), mean(img2[:, -ov:]) = bottom rows' first ov.
+    const int c1 = side == 1 ? ncol - overlap : 0;
+    const int c2 = side == 1 ? ncol - overlap : 0;
+    const long long n = (long long)nrow_180 * overlap;
+    double s1 = 0.0, s2 = 0.0;
+    for (long long e = tid; e < n; e += kThreads) {
+        const int r = (int)(e / overlap), k = (int)(e % overlap);
+        s1 += (double)sino[((size_t)r * nslice + j) * ncol + c1 + k];
+        const int rb = nrow - nrow_180 + r;
+        s2 += (double)sino[((size_t)rb * nslice + j) * ncol + c2 + k];
+    }
+    red[0][tid] = s1;
+    red[1][tid] = s2;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        factors[2 * j] = (float)(red[0][0] / (double)n);
+        factors[2 * j + 1] = (float)(red[1][0] / (double)n);
+    }
+}
+
+// out: (nrow_180, nslice, width), width >= 2*ncol - overlap (edge-extended)
+__global__ __launch_bounds__(kThreads) void stitch_kernel(
+    const float* __restrict__ sino, int nrow, int nslice, int ncol,
+    int nrow_180, int overlap, int side, int norm,
+    const float* __restrict__ factors, int width, float* __restrict__ out) {
+    const long long total = (long long)nrow_180 * nslice * width;
+    const int w0 = 2 * ncol - overlap;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        int k = (int)(e % width);
+        const long long rj = e / width;
+        const int j = (int)(rj % nslice), r = (int)(rj / nslice);
+        if (k >= w0) k = w0 - 1;  // edge extension to total_width
+        const float* top = sino + ((size_t)r * nslice + j) * ncol;
+        const float* bot =
+            sino + ((size_t)(nrow - nrow_180 + r) * nslice + j) * ncol;
+        // image 2 is scaled by factor1 / factor2 ((mat2 * f1) / f2)
+        const float f1 = norm ? factors[2 * j] : 1.0f;
+        const float f2 = norm ? factors[2 * j + 1] : 1.0f;
+        // first image (placed at 0) and second image (placed at ncol - ov)
+        const int a_off = 0, b_off = ncol - overlap;
+        // side 1: first = img1 (top), second = img2; side 0: swapped
+        float v = 0.0f;
+        for (int part = 0; part < 2; ++part) {
+            const int off = part == 0 ? a_off : b_off;
+            const int col = k - off;
+            if (col < 0 || col >= ncol) continue;
+            const bool is_img1 = (side == 1) == (part == 0);
+            float x, wgt = 1.0f;
+            if (is_img1) {
+                x = top[col];
+            } else {
+                x = bot[ncol - 1 - col];  // left-right flip
+                if (norm) x = (x * f1) / f2;
+            }
+            // linear ramp over the overlap (make_weight_matrix): the image
+            // on the left fades out over its last ov columns, the one on the
+            // right fades in over its first ov columns
+            if (part == 0 && col >= ncol - overlap) {
+                wgt = (float)ramp_down(col - (ncol - overlap), overlap);
+            } else if (part == 1 && col < overlap) {
+                wgt = (float)(1.0 - ramp_down(col, overlap));  // list_up
+            }
+            v += x * wgt;
+        }
+        out[e] = v;
+    }
+}
+
+}  // namespace
+
+int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
+                      int overlap, int side, int norm, int width,
+                      float* factors, float* out, hipStream_t s) {
+    const int nrow_180 = nrow / 2 + 1;
+    if (norm) {
+        hipLaunchKernelGGL(stitch_means_kernel, dim3(nslice), dim3(kThreads),
+                           0, s, sino, nrow, nslice, ncol, nrow_180, overlap,
+                           side, factors);
+        const int st = check_hip(hipGetLastError(), "stitch_means_kernel");
+        if (st != ATH_OK) return st;
+    }
+    const long long total = (long long)nrow_180 * nslice * width;
+    const long long blocks = std::min<long long>((total + kThreads - 1) /
+                                                     kThreads, 1 << 20);
+    hipLaunchKernelGGL(stitch_kernel, dim3((unsigned)blocks), dim3(kThreads),
+                       0, s, sino, nrow, nslice, ncol, nrow_180, overlap, side,
+                       norm, factors, width, out);
+    return check_hip(hipGetLastError(), "stitch_kernel launch");
+}
+
+}  // namespace ath

@@ -18,6 +18,19 @@ reference's CPU path) on 4 of the 64 sinograms, on this host's cores.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32]
 
+--workload refine: differentiable FBP least-squares refinement. 16
+sinograms of 901 angles over a 120-degree arc x 1024 columns (seed 4000),
+Gaussian noise sigma = 1% of max; loss = ||FBP(s; c) - phantom||^2 +
+1e-3 TV(FBP(s; c)); Adam on the sinogram s and the centre c (started 1.5 px
+off) through the hand-written adjoint; value = grad steps/s over --steps
+steps (default 200).
+
+--workload volume: 128 slices of a 3601-angle x 4096-column half-acquisition
+scan over [0, 2 pi) with the rotation axis at column 499.5 (overlap 1000),
+Poisson transmission at 1e4 photons (seed 5000); each step converts
+360 -> 180 degrees on the GPU (7192 stitched columns) and reconstructs the
+128 x 7192 x 7192 volume in chunks of 16 slices (apply_log=True).
+
 --workload uneven: 256 sinograms with widths from {1024, 2048, 2560, 4096},
 angle counts from {901, 1801, 3601} over [0, pi) and centres within +-20 px
 of mid-width (seed 3000), exact projections, reconstructed as one batch: one
@@ -196,6 +209,120 @@ def run_uneven(args, dev):
     print(json.dumps(line))
 
 
+def run_volume(args, dev):
+    import torch
+    from algotom_amd.rec import volume
+    from algotom_amd.util import phantom
+    nsl, nrow, ncol, cor = 128, 3601, 4096, 499.5
+    step_a = 2.0 * np.pi / nrow
+    angles360 = np.arange(nrow) * step_a
+    sino = torch.empty((nrow, nsl, ncol), dtype=torch.float32, device=dev)
+    for j in range(nsl):
+        prm = phantom.random_ellipses(7192, 20, seed=5000 + j)
+        p = phantom.ellipse_sinogram(prm, angles360, ncol, cor, device=dev)
+        sino[:, j, :] = phantom.transmission(p, seed=5000 + j)
+    torch.cuda.synchronize()
+    n180 = nrow // 2 + 1
+    angles = np.arange(n180) * step_a
+    width = 2 * ncol - 1000
+    out = torch.empty((width, nsl, width), dtype=torch.float32, device=dev)
+    updates = nsl * width * width * n180
+
+    def run():
+        volume.reconstruct_volume_360(sino, cor, chunk=16, angles=angles,
+                                      precision=args.precision, out=out)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    line = {
+        "metric": METRIC, "value": updates / (ms * 1e-3) / 1e9,
+        "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if args.precision is None else
+                 "%s-storage/fp32-acc" % args.precision,
+        "data": "synthetic",
+        "config": {"workload": "fbp_volume_128x3601x4096_half_acquisition",
+                   "stitched_width": width, "nrow_180": n180,
+                   "chunk": 16, "updates": updates,
+                   "output_GB": width * width * nsl * 4 / 1e9,
+                   "parallelism": "single-gpu"},
+        "detail": {"finite": bool(torch.isfinite(out).all().item()),
+                   "max_abs": float(out.abs().max().item())},
+    }
+    print(json.dumps(line))
+
+
+def run_refine(args, dev):
+    import torch
+    from algotom_amd.rec import autograd
+    from algotom_amd.util import phantom
+    nsl, nrow, ncol = 16, 901, 1024
+    angles = np.linspace(0.0, 2.0 * np.pi / 3.0, nrow)
+    c_true = (ncol - 1) / 2.0 + 0.37
+    sinos, targets = [], []
+    rng = np.random.default_rng(4000)
+    for j in range(nsl):
+        prm = phantom.random_ellipses(ncol, 20, seed=4000 + j)
+        targets.append(torch.from_numpy(phantom.ellipse_image(prm, ncol)))
+        p = phantom.ellipse_sinogram(prm, angles, ncol, np.float32(c_true),
+                                     device=dev)
+        noise = torch.from_numpy(rng.standard_normal(p.shape)).to(dev)
+        sinos.append((p + 0.01 * float(p.max()) * noise).float())
+    sino = torch.stack(sinos, dim=1).contiguous()
+    target = torch.stack(targets, dim=1).to(dev)
+    geom = autograd.FBPGeometry(nrow, ncol, angles, ratio=1.0,
+                                checkpoint="filtered")
+    s_param = sino.clone().requires_grad_(True)
+    c_param = torch.tensor(c_true + 1.5, dtype=torch.float64,
+                           requires_grad=True)
+    opt = torch.optim.Adam([{"params": [s_param], "lr": 1e-3},
+                            {"params": [c_param], "lr": 0.05}])
+    npix = float(target.numel())
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        y = autograd.fbp_differentiable(s_param, c_param, geom)
+        loss = (torch.sum((y - target) ** 2)
+                + 1e-3 * autograd.total_variation(y)) / npix
+        loss.backward()
+        opt.step()
+        return loss
+
+    first = float(step().item())
+    for _ in range(max(args.warmup - 1, 0)):
+        step()
+    torch.cuda.synchronize()
+    nsteps = args.steps if args.steps_given else 200
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        loss = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    line = {
+        "metric": METRIC, "value": nsteps / dt, "unit": "grad steps/s",
+        "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+        "ms_per_step": dt / nsteps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "fbp_refine_16x901x1024_arc120",
+                   "nslice": nsl, "nrow": nrow, "ncol": ncol,
+                   "checkpoint": "filtered", "parallelism": "single-gpu"},
+        "detail": {"loss_first": first, "loss_last": float(loss.item()),
+                   "center_start": c_true + 1.5,
+                   "center_end": float(c_param.item()),
+                   "center_true": c_true,
+                   "updates_per_forward": nsl * nrow * ncol * ncol},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,8 +331,10 @@ def main():
     ap.add_argument("--precision", default=None,
                     choices=[None, "fp32", "fp16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="stack", choices=["stack", "uneven"])
+    ap.add_argument("--workload", default="stack",
+                    choices=["stack", "uneven", "refine", "volume"])
     args = ap.parse_args()
+    args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU "
               "path (slices shard with no exchange; see DESIGN.md)",
@@ -219,6 +348,12 @@ def main():
     dev = nat.device()
     if args.workload == "uneven":
         run_uneven(args, dev)
+        return
+    if args.workload == "refine":
+        run_refine(args, dev)
+        return
+    if args.workload == "volume":
+        run_volume(args, dev)
         return
     sino, angles = make_stack(dev)
     updates = NSLICE * NCOL * NCOL * NROW

@@ -224,6 +224,21 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                             int pad, int pad_mode, const void* plan,
                             int fft_len, int nwin, float* out, void* stream);
 
+/* 360-degree -> 180-degree conversion of a half-acquisition (offset axis)
+ * sinogram stack: rows [0, nrow/2] and the left-right flipped last
+ * nrow/2 + 1 rows stitched with a linear ramp over an INTEGER `overlap`
+ * (side 0: overlap on the left of the first half, 1: on the right).
+ * norm != 0 scales the second half by the ratio of the overlap means.
+ * sino360: (nrow, nslice, ncol); out: (nrow/2 + 1, nslice, width) with
+ * width >= 2*ncol - overlap (extra columns repeat the edge); factors:
+ * device scratch of 2*nslice floats.  Replaces
+ * convert_sinogram_360_to_180 / stitch_image (algotom/prep/conversion.py
+ * :318-364, :76-144) for integer overlaps. */
+int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
+                                    int nslice, int ncol, int overlap,
+                                    int side, int norm, int width,
+                                    float* factors, float* out, void* stream);
+
 /* Device workspace bytes needed by ath_fbp_reconstruction. */
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
                                int nwin, int filt_dtype);
