@@ -18,100 +18,212 @@
 namespace ath {
 namespace {
 
-constexpr int kFpThreads = 64;
-constexpr int kFpMaxSlices = 16;
 
 __device__ __forceinline__ double fp_det_pos(double xc_c, double yc, double s,
                                              double cen) {
     return fma(yc, s, xc_c) + cen;
 }
 
-template <int NS>
-__global__ __launch_bounds__(kFpThreads) void forward_project_kernel(
-    FpArgs a, int j0) {
-    const int d = blockIdx.x * kFpThreads + threadIdx.x;
-    const int i = blockIdx.y;
-    const int C = a.ncol, S = a.nslice;
-    if (d >= C) return;
-    const int ns = min(NS, S - j0);
-    const float th = -a.angles[i];
-    const double c = (double)(float)cos((double)th);
-    const double s = (double)(float)sin((double)th);
+// LDS-staged variant.  A workgroup = 4 waves = 4 consecutive angles x the
+// same 64 detector bins x FSB slices.  The image is walked in chunks of
+// kRows lines along the major axis; for each chunk the union of the four
+// angles' candidate windows over the 64 (+2) bins is staged into LDS once
+// (line-major, FSB slices interleaved) and every lane then gathers its
+// candidates from LDS.  Each staged pixel serves up to 4 angles x 2 bins.
+// Chunks whose union window is wider than kMaxMinor columns (only for
+// angles close to 45 degrees with very small images) read global memory.
+constexpr int kMaxMinor = 192;
+// below this |slope| a line runs (almost) along the ray: take it whole
+constexpr double kMinSlope = 1.0 / 1024.0;
+
+template <int FSB, int kRows>
+__global__ __launch_bounds__(256) void forward_project_lds_kernel(
+    FpArgs a, int by_rows_block) {
+    __shared__ float tile[kRows * kMaxMinor * FSB];
+    __shared__ double s_cs[4][2];
+    __shared__ int s_win[2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int C = a.ncol, S = a.nslice, N = a.nrow;
+    const int d0 = blockIdx.x * 64;
+    const int i = blockIdx.y * 4 + wv;
+    const int j0 = blockIdx.z * FSB;
+    const int ns = min(FSB, S - j0);
+    const int d = d0 + lane;
     const double ic = a.icenter, cen = a.center, w1 = (double)(C - 1);
-    double acc[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) acc[j] = 0.0;
-    const bool by_rows = fabs(c) >= fabs(s);
-    const double lo_t = (double)d - 1.0, hi_t = (double)d + 1.0;
-    for (int m = 0; m < C; ++m) {
-        // candidate range of the other coordinate on line m
-        const double mc = (double)m - ic;
-        const double base = by_rows ? mc * s + cen : mc * c + cen;
-        const double slope = by_rows ? c : s;
-        double u0 = (lo_t - base) / slope, u1 = (hi_t - base) / slope;
-        if (u0 > u1) {
-            const double t = u0;
-            u0 = u1;
-            u1 = t;
-        }
-        int k0 = (int)floor(u0 + ic) - 1, k1 = (int)ceil(u1 + ic) + 1;
-        k0 = max(k0, 0);
-        k1 = min(k1, C - 1);
-        for (int k = k0; k <= k1; ++k) {
-            const int x = by_rows ? k : m;
-            const int y = by_rows ? m : k;
-            const double f =
-                fp_det_pos(((double)x - ic) * c, (double)y - ic, s, cen);
-            if (!(f >= 0.0 && f <= w1)) continue;
-            const double fl = floor(f);
-            const double t = f - fl;
-            double w;
-            if (fl == (double)d) w = 1.0 - t;
-            else if (fl == (double)(d - 1)) w = t;
-            else continue;
-            if (a.use_mask) {
-                const long long dx = x - a.mask_c, dy = y - a.mask_c;
-                if (!((double)(dx * dx + dy * dy) <= a.mask_r2)) continue;
-            }
-            const float* px = a.image + ((size_t)y * S + j0) * C + x;
-#pragma unroll
-            for (int j = 0; j < NS; ++j)
-                if (j < ns) acc[j] += w * (double)px[(size_t)j * C];
-        }
+    if (tid < 4) {
+        const int ia = min(blockIdx.y * 4 + tid, N - 1);
+        const float th = -a.angles[ia];
+        s_cs[tid][0] = (double)(float)cos((double)th);
+        s_cs[tid][1] = (double)(float)sin((double)th);
     }
+    __syncthreads();
+    const bool valid = (i < N) && (d < C);
+    const double c = s_cs[wv][0], s = s_cs[wv][1];
+    // walking direction of the whole block: rows when the first angle has
+    // |cos| >= |sin| (exact windows keep any direction correct)
+    const bool by_rows = by_rows_block < 0
+                             ? fabs(s_cs[0][0]) >= fabs(s_cs[0][1])
+                             : by_rows_block != 0;
+    const double slope = by_rows ? c : s;
+    // windows only need to be a superset (every candidate is tested with
+    // the exact f), so a reciprocal replaces the per-line divisions
+    const bool steep = fabs(slope) >= kMinSlope;
+    const double inv_slope = steep ? 1.0 / slope : 0.0;
+    double acc[FSB];
 #pragma unroll
-    for (int j = 0; j < NS; ++j)
-        if (j < ns)
-            a.sino[((size_t)i * S + j0 + j) * C + d] = (float)(acc[j] * a.scale);
+    for (int j = 0; j < FSB; ++j) acc[j] = 0.0;
+
+    for (int m0 = 0; m0 < C; m0 += kRows) {
+        const int m1 = min(m0 + kRows, C) - 1;
+        // union window of the minor coordinate over 4 angles, bins
+        // [d0 - 1, d0 + 64] and lines m0, m1 (windows are linear in m)
+        if (tid == 0) {
+            double lo = 1e300, hi = -1e300;
+            for (int q = 0; q < 4; ++q) {
+                const double cq = s_cs[q][0], sq = s_cs[q][1];
+                const double sl = by_rows ? cq : sq;
+                if (fabs(sl) < kMinSlope) {  // line nearly along the ray
+                    lo = -1e300;
+                    hi = 1e300;
+                    continue;
+                }
+                for (int e = 0; e < 2; ++e) {
+                    const double mc = (double)(e ? m1 : m0) - ic;
+                    const double base = (by_rows ? mc * sq : mc * cq) + cen;
+                    const double u0 = ((double)d0 - 2.0 - base) / sl;
+                    const double u1 = ((double)d0 + 65.0 - base) / sl;
+                    lo = fmin(lo, fmin(u0, u1));
+                    hi = fmax(hi, fmax(u0, u1));
+                }
+            }
+            s_win[0] = lo <= -1e299 ? 0 : max((int)floor(lo + ic) - 1, 0);
+            s_win[1] = hi >= 1e299 ? C - 1 : min((int)ceil(hi + ic) + 1, C - 1);
+        }
+        __syncthreads();
+        const int ka = s_win[0], kb = s_win[1];
+        const int wid = kb - ka + 1;
+        const bool staged = wid <= kMaxMinor;
+        if (wid > 0 && staged) {
+            const int nrows = m1 - m0 + 1;
+            const int total = nrows * wid * FSB;
+            for (int e = tid; e < total; e += 256) {
+                // coalesced along x for rows, along the slice-row for columns
+                int js, k, r;
+                if (by_rows) {
+                    k = e % wid;
+                    const int t = e / wid;
+                    js = t % FSB;
+                    r = t / FSB;
+                } else {
+                    r = e % nrows;
+                    const int t = e / nrows;
+                    js = t % FSB;
+                    k = t / FSB;
+                }
+                const int x = by_rows ? ka + k : m0 + r;
+                const int y = by_rows ? m0 + r : ka + k;
+                float v = 0.0f;
+                if (js < ns) {
+                    v = a.image[((size_t)y * S + j0 + js) * C + x];
+                    if (a.use_mask) {
+                        const long long dx = x - a.mask_c, dy = y - a.mask_c;
+                        if (!((double)(dx * dx + dy * dy) <= a.mask_r2))
+                            v = 0.0f;
+                    }
+                }
+                tile[(r * kMaxMinor + k) * FSB + js] = v;
+            }
+        }
+        __syncthreads();
+        if (valid && wid > 0) {
+            const double lo_t = (double)d - 1.0, hi_t = (double)d + 1.0;
+            for (int m = m0; m <= m1; ++m) {
+                const double mc = (double)m - ic;
+                const double base = by_rows ? mc * s + cen : mc * c + cen;
+                int k0 = 0, k1 = C - 1;
+                if (steep) {
+                    double u0 = (lo_t - base) * inv_slope;
+                    double u1 = (hi_t - base) * inv_slope;
+                    if (u0 > u1) {
+                        const double t = u0;
+                        u0 = u1;
+                        u1 = t;
+                    }
+                    k0 = max((int)floor(u0 + ic) - 1, 0);
+                    k1 = min((int)ceil(u1 + ic) + 1, C - 1);
+                }
+                if (staged) {
+                    k0 = max(k0, ka);
+                    k1 = min(k1, kb);
+                }
+                for (int k = k0; k <= k1; ++k) {
+                    const int x = by_rows ? k : m;
+                    const int y = by_rows ? m : k;
+                    const double f = fp_det_pos(((double)x - ic) * c,
+                                                (double)y - ic, s, cen);
+                    if (!(f >= 0.0 && f <= w1)) continue;
+                    const double fl = floor(f);
+                    const double t = f - fl;
+                    double w;
+                    if (fl == (double)d) w = 1.0 - t;
+                    else if (fl == (double)(d - 1)) w = t;
+                    else continue;
+                    if (staged) {
+                        const float* q =
+                            tile + ((m - m0) * kMaxMinor + (k - ka)) * FSB;
+#pragma unroll
+                        for (int j = 0; j < FSB; ++j)
+                            acc[j] += w * (double)q[j];
+                    } else {
+                        if (a.use_mask) {
+                            const long long dx = x - a.mask_c,
+                                            dy = y - a.mask_c;
+                            if (!((double)(dx * dx + dy * dy) <= a.mask_r2))
+                                continue;
+                        }
+                        const float* px =
+                            a.image + ((size_t)y * S + j0) * C + x;
+#pragma unroll
+                        for (int j = 0; j < FSB; ++j)
+                            if (j < ns) acc[j] += w * (double)px[(size_t)j * C];
+                    }
+                }
+            }
+        }
+        __syncthreads();  // tile free for the next chunk
+    }
+    if (valid) {
+#pragma unroll
+        for (int j = 0; j < FSB; ++j)
+            if (j < ns)
+                a.sino[((size_t)i * S + j0 + j) * C + d] =
+                    (float)(acc[j] * a.scale);
+    }
 }
 
 }  // namespace
 
+// Angle blocks of 4 share one walking direction: rows when the block's
+// first angle has |cos| >= |sin|.  The candidate windows are exact for any
+// direction, so a block straddling 45 degrees stays correct (its windows
+// are at most 2/0.70 + 3 wide).  Launched per direction-homogeneous run of
+// blocks: one launch for each maximal run.
 int forward_projection(const FpArgs& a, hipStream_t stream) {
-    const dim3 grid((a.ncol + kFpThreads - 1) / kFpThreads, a.nrow);
-    int j0 = 0;
-    while (j0 < a.nslice) {
-        const int rest = a.nslice - j0;
-        int step;
-        if (rest >= kFpMaxSlices) {
-            hipLaunchKernelGGL(forward_project_kernel<kFpMaxSlices>, grid,
-                               dim3(kFpThreads), 0, stream, a, j0);
-            step = kFpMaxSlices;
-        } else if (rest >= 4) {
-            hipLaunchKernelGGL(forward_project_kernel<4>, grid,
-                               dim3(kFpThreads), 0, stream, a, j0);
-            step = 4;
-        } else {
-            hipLaunchKernelGGL(forward_project_kernel<1>, grid,
-                               dim3(kFpThreads), 0, stream, a, j0);
-            step = 1;
-        }
-        const int st = check_hip(hipGetLastError(),
-                                 "forward_project_kernel launch");
-        if (st != ATH_OK) return st;
-        j0 += step;
-    }
-    return ATH_OK;
+    // host copy of the walking direction per block needs the angles: they
+    // live on the device, so the kernel decides per block instead
+    const int fsb = a.nslice >= 8 ? 8 : (a.nslice >= 4 ? 4 : 1);
+    const dim3 grid((a.ncol + 63) / 64, (a.nrow + 3) / 4,
+                    (a.nslice + fsb - 1) / fsb);
+    if (fsb == 8)
+        hipLaunchKernelGGL((forward_project_lds_kernel<8, 8>), grid,
+                           dim3(256), 0, stream, a, -1);
+    else if (fsb == 4)
+        hipLaunchKernelGGL((forward_project_lds_kernel<4, 16>), grid,
+                           dim3(256), 0, stream, a, -1);
+    else
+        hipLaunchKernelGGL((forward_project_lds_kernel<1, 16>), grid,
+                           dim3(256), 0, stream, a, -1);
+    return check_hip(hipGetLastError(), "forward_project_lds_kernel launch");
 }
 
 }  // namespace ath
