@@ -373,6 +373,50 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                              static_cast<hipStream_t>(stream));
 }
 
+int ath_gaussian_filter_rows(const float* in, float* out, int nrows,
+                             int ncol, const double* weights, int radius,
+                             void* stream) {
+    clear_error();
+    if (!in || !out || !weights)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nrows < 1 || ncol < 1 || radius < 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    return gaussian_rows(in, out, nrows, ncol, weights, radius,
+                         static_cast<hipStream_t>(stream));
+}
+
+int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
+                    int out_nrow, int out_ncol, void* stream) {
+    clear_error();
+    if (!in || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nrow < 1 || ncol < 1 || out_nrow < 1 || out_ncol < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    return zoom_linear(in, nrow, ncol, out, out_nrow, out_ncol,
+                       static_cast<hipStream_t>(stream));
+}
+
+int ath_slice_metrics(const float* images, int nimg, int ncol,
+                      long long stride, float nmin, float nmax, int metric,
+                      double* out, void* stream) {
+    clear_error();
+    if (!images || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nimg < 1 || ncol < 1 || stride < (long long)ncol * ncol)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (metric != 0 && metric != 1)
+        return set_error(ATH_ERR_INVALID_ARG, "metric must be 0 or 1");
+    if ((long long)ncol * ncol > (1LL << 30))
+        return set_error(ATH_ERR_UNSUPPORTED, "image above 2^30 pixels");
+    if (metric == 1 && ncol < 2)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "gradient needs at least 2 pixels per axis");
+    if (metric == 0 && !(nmax > nmin))
+        return set_error(ATH_ERR_INVALID_ARG, "Empty image!!!");
+    return slice_metrics(images, nimg, ncol, stride, nmin, nmax, metric, out,
+                         static_cast<hipStream_t>(stream));
+}
+
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
                                int nwin, int filt_dtype) {
     if (nrow < 1 || nslice < 1 || ncol < 1 || pad < 0 || nwin < 1 ||

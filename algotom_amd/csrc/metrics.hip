@@ -1,0 +1,261 @@
+// Kernels of the centre-of-rotation search (algotom/rec/reconstruction.py
+// :1211-1458): Gaussian smoothing of the sinogram rows, the order-1 zoom of
+// the coarse search and the per-slice metrics scored for every candidate
+// centre.  Every sum follows the order numpy / scipy use, so the metrics
+// equal the reference's (bit-for-bit for the sharpness; the entropy up to
+// the last-bit differences of the two libraries' log2).
+//
+// gaussian_rows_kernel   scipy.ndimage.gaussian_filter1d(x, sigma, axis=-1)
+//                        (mode "reflect" = numpy's "symmetric", truncate 4):
+//                        float64 weights exp(-x^2 / (2 sigma^2)) / sum and
+//                        float64 sums, float32 output.
+// zoom_linear_kernel     scipy.ndimage.zoom(x, z, order=1, mode="nearest")
+//                        (:1434), bilinear in float64.
+// entropy_kernel         __calculate_histogram_entropy (:1211-1220) on
+//                        clip((rec - nmin) / (nmax - nmin), 0, 1): uint8
+//                        levels v = trunc(255 r) (float32), 256-bin counts
+//                        (ndi.histogram(., 0, 255, 256) puts level v in bin
+//                        v), h = 1 + counts, 7-tap boxcar "valid",
+//                        metric = -sum h log2 h (numpy's pairwise sum).
+//                        Integer LDS counts: exact.
+// sharpness_kernel       __calculate_edge_sharpness (:1223-1228): np.mean of
+//                        |np.gradient| (central differences inside, one-sided
+//                        at the borders), float32 sums in numpy's order.
+#include "ath_internal.h"
+
+namespace ath {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxRadius = 64;
+constexpr int kChunk = 8192;  // numpy's default ufunc buffer size
+
+__global__ __launch_bounds__(kThreads) void gaussian_rows_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int ncol,
+    const double* __restrict__ weights, int radius) {
+    __shared__ double w[2 * kMaxRadius + 1];
+    const size_t row = blockIdx.x;
+    for (int k = threadIdx.x; k <= 2 * radius; k += kThreads)
+        w[k] = weights[k];
+    __syncthreads();
+    const float* src = in + row * (size_t)ncol;
+    for (int c = threadIdx.x; c < ncol; c += kThreads) {
+        // scipy's symmetric-kernel order: x w0 + sum_j (x[+j] + x[-j]) w_j
+        double acc = (double)src[c] * w[radius];
+        for (int k = 1; k <= radius; ++k) {
+            const int cp = pad_source(c + k, ncol, ATH_PAD_SYMMETRIC);
+            const int cm = pad_source(c - k, ncol, ATH_PAD_SYMMETRIC);
+            acc += ((double)src[cp] + (double)src[cm]) * w[radius + k];
+        }
+        out[row * (size_t)ncol + c] = (float)acc;
+    }
+}
+
+// numpy's pairwise_sum (umath loops_utils) for one block of n <= 128
+// values: 8 running partial sums, combined as a tree, then the remainder.
+template <typename T>
+__device__ T np_pairwise_leaf(const T* a, int n) {
+    if (n < 8) {
+        T r = T(0);
+        for (int i = 0; i < n; ++i) r += a[i];
+        return r;
+    }
+    T r0 = a[0], r1 = a[1], r2 = a[2], r3 = a[3];
+    T r4 = a[4], r5 = a[5], r6 = a[6], r7 = a[7];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+        r0 += a[i];     r1 += a[i + 1]; r2 += a[i + 2]; r3 += a[i + 3];
+        r4 += a[i + 4]; r5 += a[i + 5]; r6 += a[i + 6]; r7 += a[i + 7];
+    }
+    T res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; ++i) res += a[i];
+    return res;
+}
+
+// numpy's pairwise_sum for n <= kChunk values on one thread: blocks of at
+// most 128 values, larger ranges split at n/2 rounded down to a multiple of
+// 8 and summed left + right.  The recursion runs on an explicit stack (depth
+// <= 9 for n <= 8192).
+template <typename T>
+__device__ T np_pairwise_serial(const T* a, int n) {
+    int off[16], len[16], stage[16];
+    T val[16];
+    int sp = 1, vp = 0;
+    off[0] = 0;
+    len[0] = n;
+    stage[0] = 0;
+    while (sp > 0) {
+        const int t = sp - 1;
+        const int m = len[t];
+        if (m <= 128) {
+            val[vp++] = np_pairwise_leaf(a + off[t], m);
+            --sp;
+            continue;
+        }
+        int m2 = m / 2;
+        m2 -= m2 % 8;
+        if (stage[t] == 0) {
+            stage[t] = 1;
+            off[sp] = off[t];
+            len[sp] = m2;
+            stage[sp++] = 0;
+        } else if (stage[t] == 1) {
+            stage[t] = 2;
+            off[sp] = off[t] + m2;
+            len[sp] = m - m2;
+            stage[sp++] = 0;
+        } else {
+            const T r = val[--vp];
+            const T l = val[--vp];
+            val[vp++] = l + r;
+            --sp;
+        }
+    }
+    return val[0];
+}
+
+__global__ __launch_bounds__(kThreads) void entropy_kernel(
+    const float* __restrict__ images, long long stride, int ncol, float nmin,
+    float nmax, double* __restrict__ out) {
+    __shared__ unsigned int hist[256];
+    __shared__ double h7[256];
+    const float* img = images + (size_t)blockIdx.x * (size_t)stride;
+    const int tid = threadIdx.x;
+    hist[tid] = 0u;
+    __syncthreads();
+    const long long n = (long long)ncol * ncol;
+    const float range = nmax - nmin;
+    for (long long e = tid; e < n; e += kThreads) {
+        float r = (img[e] - nmin) / range;
+        r = fminf(fmaxf(r, 0.0f), 1.0f);
+        const int v = (int)(r * 255.0f);  // np.uint8 truncation
+        atomicAdd(&hist[v], 1u);
+    }
+    __syncthreads();
+    // h = 1 + counts; boxcar(7) "valid" -> 250 sliding sums of integers
+    // (exact in float64), then h log2 h
+    if (tid < 250) {
+        double s = 0.0;
+        for (int k = 0; k < 7; ++k) s += 1.0 + (double)hist[tid + k];
+        h7[tid] = s * log2(s);
+    }
+    __syncthreads();
+    if (tid == 0) out[blockIdx.x] = -np_pairwise_serial(h7, 250);
+}
+
+// |np.gradient| of pixel e of a (ncol, ncol) image: axis 0 then axis 1,
+// central differences halved inside, one-sided at the borders, float32.
+__device__ __forceinline__ float grad_magnitude(const float* img, int ncol,
+                                                int e) {
+    const int y = e / ncol, x = e - y * ncol;
+    float gy, gx;
+    if (y == 0) gy = img[e + ncol] - img[e];
+    else if (y == ncol - 1) gy = img[e] - img[e - ncol];
+    else gy = (img[e + ncol] - img[e - ncol]) / 2.0f;
+    if (x == 0) gx = img[e + 1] - img[e];
+    else if (x == ncol - 1) gx = img[e] - img[e - 1];
+    else gx = (img[e + 1] - img[e - 1]) / 2.0f;
+    return sqrtf(gy * gy + gx * gx);
+}
+
+// np.mean of a float32 image: numpy reduces through its 8192-element
+// buffer, so the float32 sum is a sequential float32 accumulation (from 0)
+// of pairwise sums of consecutive 8192-value chunks of the flattened image;
+// mean = float32(float64(sum) / n).  One workgroup per image walks its
+// chunks in order: the chunk's magnitudes go to LDS, a full chunk is 64
+// blocks of 128 (64 threads) combined as the perfect binary tree the
+// recursion forms, a shorter tail chunk is summed by one thread.
+__global__ __launch_bounds__(kThreads) void sharpness_kernel(
+    const float* __restrict__ images, long long stride, int ncol,
+    double* __restrict__ out) {
+    __shared__ float mag[kChunk];
+    __shared__ float part[64];
+    const float* img = images + (size_t)blockIdx.x * (size_t)stride;
+    const int tid = threadIdx.x;
+    const int n = ncol * ncol;
+    float total = 0.0f;  // meaningful on thread 0 only
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+        const int m = min(kChunk, n - c0);
+        for (int k = tid; k < m; k += kThreads)
+            mag[k] = grad_magnitude(img, ncol, c0 + k);
+        __syncthreads();
+        float chunk;
+        if (m == kChunk) {
+            if (tid < 64) part[tid] = np_pairwise_leaf(mag + 128 * tid, 128);
+            __syncthreads();
+            for (int w = 1; w < 64; w <<= 1) {
+                if (tid < 64 && (tid % (2 * w)) == 0)
+                    part[tid] = part[tid] + part[tid + w];
+                __syncthreads();
+            }
+            chunk = part[0];
+        } else {
+            chunk = tid == 0 ? np_pairwise_serial(mag, m) : 0.0f;
+        }
+        if (tid == 0) total = total + chunk;
+        __syncthreads();  // mag / part are rewritten by the next chunk
+    }
+    if (tid == 0) out[blockIdx.x] = (double)(float)((double)total / (double)n);
+}
+
+// scipy.ndimage.zoom(x, zoom, order=1, mode="nearest") of a 2D float32
+// image (grid_mode=False): output pixel (i, j) samples the input at
+// (i * (h-1)/(oh-1), j * (w-1)/(ow-1)) by bilinear interpolation in float64,
+// neighbours past the last row/column clamped to it.
+__global__ __launch_bounds__(kThreads) void zoom_linear_kernel(
+    const float* __restrict__ in, int h, int w, float* __restrict__ out,
+    int oh, int ow, double zy, double zx) {
+    const long long total = (long long)oh * ow;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        const int i = (int)(e / ow), j = (int)(e % ow);
+        const double yy = (double)i * zy, xx = (double)j * zx;
+        int y0 = (int)floor(yy), x0 = (int)floor(xx);
+        y0 = min(max(y0, 0), h - 1);
+        x0 = min(max(x0, 0), w - 1);
+        const double ty = yy - (double)y0, tx = xx - (double)x0;
+        const int y1 = min(y0 + 1, h - 1), x1 = min(x0 + 1, w - 1);
+        const double f00 = in[(size_t)y0 * w + x0], f01 = in[(size_t)y0 * w + x1];
+        const double f10 = in[(size_t)y1 * w + x0], f11 = in[(size_t)y1 * w + x1];
+        const double top = (1.0 - tx) * f00 + tx * f01;
+        const double bot = (1.0 - tx) * f10 + tx * f11;
+        out[e] = (float)((1.0 - ty) * top + ty * bot);
+    }
+}
+
+}  // namespace
+
+int gaussian_rows(const float* in, float* out, int nrows, int ncol,
+                  const double* weights, int radius, hipStream_t s) {
+    if (radius > kMaxRadius)
+        return set_error(ATH_ERR_UNSUPPORTED, "gaussian radius above 64");
+    hipLaunchKernelGGL(gaussian_rows_kernel, dim3(nrows), dim3(kThreads), 0,
+                       s, in, out, ncol, weights, radius);
+    return check_hip(hipGetLastError(), "gaussian_rows_kernel launch");
+}
+
+int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
+                hipStream_t s) {
+    const double zy = oh > 1 ? (double)(h - 1) / (double)(oh - 1) : 1.0;
+    const double zx = ow > 1 ? (double)(w - 1) / (double)(ow - 1) : 1.0;
+    const long long total = (long long)oh * ow;
+    const long long blocks =
+        std::min<long long>((total + kThreads - 1) / kThreads, 1 << 16);
+    hipLaunchKernelGGL(zoom_linear_kernel, dim3((unsigned)blocks),
+                       dim3(kThreads), 0, s, in, h, w, out, oh, ow, zy, zx);
+    return check_hip(hipGetLastError(), "zoom_linear_kernel launch");
+}
+
+int slice_metrics(const float* images, int nimg, int ncol, long long stride,
+                  float nmin, float nmax, int metric, double* out,
+                  hipStream_t s) {
+    if (metric == 0)
+        hipLaunchKernelGGL(entropy_kernel, dim3(nimg), dim3(kThreads), 0, s,
+                           images, stride, ncol, nmin, nmax, out);
+    else
+        hipLaunchKernelGGL(sharpness_kernel, dim3(nimg), dim3(kThreads), 0, s,
+                           images, stride, ncol, out);
+    return check_hip(hipGetLastError(), "slice metric launch");
+}
+
+}  // namespace ath

@@ -1,0 +1,213 @@
+"""Centre-of-rotation search by slice metrics -- drop-in for algotom's
+``_find_center_based_slice_metric`` and ``find_center_based_slice_metric``
+(algotom/rec/reconstruction.py:1254-1458), method "fbp".
+
+The reference reconstructs the slice once per candidate centre, re-filtering
+the sinogram every time (:1349-1355).  Here the sinogram is smoothed
+(ath_gaussian_filter_rows), logged and ramp-filtered ONCE, every candidate
+centre is back-projected in one persistent batched launch (the filtered
+sinogram shared by all jobs), and the entropy / sharpness metric of every
+slice is computed on the GPU (ath_slice_metrics).  Per centre the slice is
+exactly the fbp_reconstruction result, so the metrics and the chosen centre
+match a reference run on the same sinogram.
+"""
+
+import numpy as np
+
+from algotom_amd import _native as nat
+from algotom_amd.rec import reconstruction as rec
+
+_MAX_BATCH_BYTES = 2 << 30
+
+
+def _gaussian_weights(sigma, truncate=4.0):
+    """scipy.ndimage's normalised Gaussian (gaussian_filter1d, order 0)."""
+    radius = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-radius, radius + 1)
+    phi = np.exp(-0.5 / (float(sigma) * float(sigma)) * x ** 2)
+    return phi / phi.sum(), radius
+
+
+def _smooth_rows(sino_t, sigma):
+    import torch
+    w, radius = _gaussian_weights(sigma)
+    w_d = torch.from_numpy(np.ascontiguousarray(w)).to(sino_t.device)
+    out = torch.empty_like(sino_t)
+    nrow, ncol = sino_t.shape
+    nat.check(nat.load().ath_gaussian_filter_rows(
+        nat.ptr(sino_t), nat.ptr(out), nrow, ncol, nat.ptr(w_d), radius,
+        nat.stream_ptr(sino_t.device)), "ath_gaussian_filter_rows")
+    return out
+
+
+def _backproject_centers(filt, centers, angles, ratio, acc):
+    """(ncenter, ncol, ncol) slices of one filtered sinogram (nrow, 1, ncol)
+    back-projected at every centre in one persistent batched launch, each
+    with fbp_reconstruction's mask and scale."""
+    import torch
+    dev = filt.device
+    lib = nat.load()
+    nrow, _, ncol = filt.shape
+    ang = rec._angles_device(angles, dev)
+    if ang.numel() != nrow:
+        raise ValueError("!!!Number of angles is not the same as the row "
+                         "number of the sinogram!!!")
+    out = torch.empty((len(centers), ncol, ncol), dtype=torch.float32,
+                      device=dev)
+    rows = 16 if acc == nat.ACC_F64 else 32
+    jobs = (nat.BpJob * len(centers))()
+    tile = 0
+    for k, c in enumerate(centers):
+        if c < 0 or c >= ncol:
+            raise ValueError("Center is out of the range [0, {}]"
+                             .format(ncol - 1))
+        radius = -1.0
+        if ratio is not None:
+            r = ratio
+            if r == 0.0:
+                r = min(c, ncol - c) / (0.5 * ncol)
+            radius = float(r * (ncol // 2))
+        jobs[k] = nat.BpJob(filt.data_ptr(), out[k].data_ptr(), ang.data_ptr(),
+                            nrow, 1, ncol, float(np.float32(c)), radius, 1, 0,
+                            tile)
+        tile += lib.ath_bp_job_tiles(ncol, 1, 1, rows)
+    raw = np.frombuffer(bytes(jobs), dtype=np.uint8).copy()
+    jobs_d = torch.from_numpy(raw).to(dev)
+    counter = torch.empty(1, dtype=torch.int32, device=dev)
+    # jobs_d and counter are freed back to torch's stream-ordered cache, so
+    # no later allocation on this stream can overwrite them before the
+    # launch has read them
+    nat.check(lib.ath_back_projection_batched(
+        nat.ptr(jobs_d), len(centers), tile, nat.ATH_F32, acc, 1, rows, 0,
+        nat.ptr(counter), nat.stream_ptr(dev)), "ath_back_projection_batched")
+    return out
+
+
+def _zoom_linear(sino_t, zoom):
+    """ndi.zoom(sinogram, zoom, order=1, mode="nearest") on the GPU."""
+    import torch
+    nrow, ncol = sino_t.shape
+    # scipy's output shape: round(in * zoom) per axis
+    oh, ow = int(round(nrow * zoom)), int(round(ncol * zoom))
+    out = torch.empty((oh, ow), dtype=torch.float32, device=sino_t.device)
+    nat.check(nat.load().ath_zoom_linear(
+        nat.ptr(sino_t), nrow, ncol, nat.ptr(out), oh, ow,
+        nat.stream_ptr(sino_t.device)), "ath_zoom_linear")
+    return out
+
+
+def _find_center_based_slice_metric(sinogram, start, stop, step=1,
+                                    metric="entropy", method="fbp", gpu=True,
+                                    angles=None, ratio=1.0,
+                                    filter_name="hann", apply_log=True,
+                                    ncore=None, prefer="threads", sigma=2,
+                                    return_metric=True, invert_metric=False,
+                                    metric_function=None, **kwargs):
+    """Find the centre of rotation from metrics of slices reconstructed at
+    centres arange(start, stop, step) (reconstruction.py:1254-1362).
+
+    Returns (list_center, list_metric) if return_metric else the centre of
+    the smallest metric.  ``gpu`` selects the reference kernel's arithmetic
+    (float64 sums if True); ``ncore`` and ``prefer`` are ignored.
+    """
+    import torch
+    if method != "fbp":
+        raise NotImplementedError("only method='fbp' runs on the GPU path")
+    list_center = np.arange(start, stop, step)
+    if len(list_center) == 0:
+        raise ValueError("Invalid searching parameters: (start, stop, step)={}"
+                         "!!!".format((start, stop, step)))
+    dev = nat.device()
+    sino_t = rec._to_device(sinogram, dev, allow_f64=False)
+    if sino_t.dim() != 2:
+        raise ValueError("the centre search takes a 2D sinogram")
+    if sigma > 0:
+        sino_t = _smooth_rows(sino_t, sigma)
+    elif apply_log and sino_t is sinogram:
+        sino_t = sino_t.clone()  # the caller's tensor stays unchanged
+    if apply_log:
+        rec._clamp_nonpositive(sino_t)
+    nrow, ncol = sino_t.shape
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    pad = min(int(0.15 * ncol), 150)
+    taps = rec._window_taps(None, nrow, ncol + 2 * pad, filter_name)
+    filt = rec._filter_stack(sino_t.reshape(nrow, 1, ncol), pad, "edge",
+                             bool(apply_log), taps, torch.float32)
+    acc = nat.ACC_F64 if gpu else nat.ACC_F32SEQ
+    nmin, nmax = 0.0, 1.0
+    if metric_function is None and metric == "entropy":
+        ref = _backproject_centers(filt, [float(np.mean(list_center))],
+                                   angles, ratio, acc)
+        nmin = float(torch.min(ref).item())
+        nmax = float(torch.max(ref).item())
+        if nmin == nmax:
+            raise ValueError("Empty image!!!")
+    per_batch = max(1, _MAX_BATCH_BYTES // (4 * ncol * ncol))
+    values = []
+    lib = nat.load()
+    for b0 in range(0, len(list_center), per_batch):
+        cs = list_center[b0:b0 + per_batch]
+        imgs = _backproject_centers(filt, cs, angles, ratio, acc)
+        if metric_function is not None:
+            for k in range(len(cs)):
+                values.append(float(metric_function(imgs[k].cpu().numpy(),
+                                                    **kwargs)))
+            continue
+        out = torch.empty(len(cs), dtype=torch.float64, device=dev)
+        nat.check(lib.ath_slice_metrics(
+            nat.ptr(imgs), len(cs), ncol, ncol * ncol,
+            float(np.float32(nmin)), float(np.float32(nmax)),
+            0 if metric == "entropy" else 1, nat.ptr(out),
+            nat.stream_ptr(dev)), "ath_slice_metrics")
+        values.extend(out.cpu().numpy().tolist())
+    list_metric = np.asarray(values, dtype=np.float64)
+    if invert_metric:
+        list_metric = np.max(list_metric) - list_metric
+    if return_metric:
+        return list_center, list_metric
+    return list_center[np.argmin(list_metric)]
+
+
+def find_center_based_slice_metric(sinogram, start, stop, step=0.5,
+                                   metric="entropy", radius=2, zoom=1.0,
+                                   method="fbp", gpu=True, angles=None,
+                                   ratio=1.0, filter_name="hann",
+                                   apply_log=True, ncore=None, sigma=2,
+                                   invert_metric=False, metric_function=None,
+                                   **kwargs):
+    """Coarse (integer step, optionally on a zoomed sinogram) then fine
+    (sub-pixel step within +-radius) centre search
+    (reconstruction.py:1365-1458).  Returns the centre."""
+    zoom = float(np.clip(zoom, 0.01, 1.0))
+    angles_zoom = angles
+    sino_zoom = sinogram
+    if zoom < 1.0:
+        import scipy.ndimage as ndi
+        t = rec._to_device(sinogram, nat.device(), allow_f64=False)
+        sino_zoom = _zoom_linear(t, zoom)
+        start, stop = start * zoom, stop * zoom
+        if angles is not None:
+            # host-side metadata, as the reference does (order-3 spline)
+            angles_zoom = ndi.zoom(np.tile(angles, (1, 1)), (1.0, zoom))[0]
+    f_alias = _find_center_based_slice_metric
+    if stop > start:
+        coarse = f_alias(sino_zoom, start, stop + 1, 1.0, metric=metric,
+                         method=method, gpu=gpu, angles=angles_zoom,
+                         ratio=ratio, filter_name=filter_name,
+                         apply_log=apply_log, ncore=ncore, sigma=sigma,
+                         return_metric=False, invert_metric=invert_metric,
+                         metric_function=metric_function, **kwargs)
+        coarse = coarse / zoom
+    else:
+        coarse = start / zoom
+    if radius != 0.0:
+        radius = max(radius, step + 1.0 / zoom)
+        lo, hi = coarse - radius, coarse + radius + step
+        return f_alias(sinogram, lo, hi, step, metric=metric, method=method,
+                       gpu=gpu, angles=angles, ratio=ratio,
+                       filter_name=filter_name, apply_log=apply_log,
+                       ncore=ncore, sigma=sigma, return_metric=False,
+                       invert_metric=invert_metric,
+                       metric_function=metric_function, **kwargs)
+    return coarse

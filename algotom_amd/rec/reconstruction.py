@@ -431,3 +431,19 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
     if not input_3d:
         out = out.reshape(ncol, ncol)
     return _finish(out, as_numpy)
+
+
+# Centre search by slice metrics (reconstruction.py:1254-1458), implemented
+# in algotom_amd.rec.center and reachable here, where the reference has it.
+def _find_center_based_slice_metric(*args, **kwargs):
+    """algotom_amd.rec.center._find_center_based_slice_metric
+    (reconstruction.py:1254-1362)."""
+    from algotom_amd.rec import center
+    return center._find_center_based_slice_metric(*args, **kwargs)
+
+
+def find_center_based_slice_metric(*args, **kwargs):
+    """algotom_amd.rec.center.find_center_based_slice_metric
+    (reconstruction.py:1365-1458)."""
+    from algotom_amd.rec import center
+    return center.find_center_based_slice_metric(*args, **kwargs)

@@ -239,6 +239,33 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                                     int side, int norm, int width,
                                     float* factors, float* out, void* stream);
 
+/* Gaussian smoothing along rows (nrows x ncol float32), as
+ * scipy.ndimage.gaussian_filter1d(x, sigma, axis=-1) with mode "reflect":
+ * weights (2*radius+1 float64, device) are the normalised Gaussian the
+ * caller computed; sums in float64.  radius <= 64.  Used by the centre
+ * search (reconstruction.py:1315-1316). */
+int ath_gaussian_filter_rows(const float* in, float* out, int nrows,
+                             int ncol, const double* weights, int radius,
+                             void* stream);
+
+/* Order-1 zoom of a 2D float32 image (nrow x ncol -> out_nrow x out_ncol),
+ * as scipy.ndimage.zoom(x, z, order=1, mode="nearest") with its
+ * corner-aligned grid: float64 bilinear weights, float32 output.  Replaces
+ * the coarse-search zoom of find_center_based_slice_metric
+ * (reconstruction.py:1434). */
+int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
+                    int out_nrow, int out_ncol, void* stream);
+
+/* Centre-search metrics of nimg square images (ncol x ncol float32, image
+ * k at images + k*stride): metric 0 = histogram entropy of
+ * clip((img - nmin)/(nmax - nmin), 0, 1) (reconstruction.py:1211-1220,
+ * 1243-1246), metric 1 = mean gradient magnitude (:1223-1228; ncol >= 2).
+ * Sums follow numpy's order (pairwise within 8192-value chunks).  out: nimg
+ * float64 (device). */
+int ath_slice_metrics(const float* images, int nimg, int ncol,
+                      long long stride, float nmin, float nmax, int metric,
+                      double* out, void* stream);
+
 /* Device workspace bytes needed by ath_fbp_reconstruction. */
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
                                int nwin, int filt_dtype);

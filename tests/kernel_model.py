@@ -159,3 +159,63 @@ def bp_tile_footprints(ncol, angles, center, py=8, tx=64):
                     worst = max(worst, int((d - dlo).max()))
                     assert int((d - dlo).min()) >= 0
     return worst, span
+
+
+# ---- summation order of metrics.hip (numpy's float32 mean) ---------------
+def np_pairwise_leaf(a, f=np.float32):
+    """np_pairwise_leaf: <= 128 values, 8 partial sums of type f."""
+    n = len(a)
+    if n < 8:
+        r = f(0)
+        for x in a:
+            r = f(r + x)
+        return r
+    r = [f(a[j]) for j in range(8)]
+    i = 8
+    while i < n - (n % 8):
+        for j in range(8):
+            r[j] = f(r[j] + a[i + j])
+        i += 8
+    res = f(f(f(r[0] + r[1]) + f(r[2] + r[3]))
+            + f(f(r[4] + r[5]) + f(r[6] + r[7])))
+    while i < n:
+        res = f(res + a[i])
+        i += 1
+    return res
+
+
+def np_pairwise(a, f=np.float32):
+    """np_pairwise_serial: the recursive split at n/2 rounded to 8."""
+    n = len(a)
+    if n <= 128:
+        return np_pairwise_leaf(a, f)
+    n2 = n // 2
+    n2 -= n2 % 8
+    return f(np_pairwise(a[:n2], f) + np_pairwise(a[n2:], f))
+
+
+def np_mean_f32(x, chunk=8192):
+    """sharpness_kernel's reduction: sequential float32 sum of the pairwise
+    sums of consecutive `chunk`-value pieces; full pieces as 64 leaves of
+    128 combined by a perfect tree (vectorised here), the tail recursively;
+    mean = float32(float64(sum) / n)."""
+    flat = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    n = flat.size
+    nfull = n // chunk
+    sums = []
+    if nfull:
+        blk = flat[:nfull * chunk].reshape(nfull, 64, 16, 8)
+        r = blk[:, :, 0, :].copy()
+        for i in range(1, 16):
+            r = (r + blk[:, :, i, :]).astype(np.float32)
+        leaf = ((r[..., 0] + r[..., 1]) + (r[..., 2] + r[..., 3])) + \
+            ((r[..., 4] + r[..., 5]) + (r[..., 6] + r[..., 7]))
+        while leaf.shape[1] > 1:
+            leaf = (leaf[:, 0::2] + leaf[:, 1::2]).astype(np.float32)
+        sums.extend(leaf[:, 0].tolist())
+    if n % chunk:
+        sums.append(np_pairwise(flat[nfull * chunk:]))
+    total = np.float32(0)
+    for s in sums:
+        total = np.float32(total + np.float32(s))
+    return np.float32(np.float64(total) / np.float64(n))

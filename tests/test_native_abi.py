@@ -110,3 +110,18 @@ def test_dropin_fails_loudly_without_device():
     with pytest.raises(ValueError):
         rec.fbp_reconstruction(np.ones((10, 20), np.float32), 5.0,
                                angles=np.zeros(3))
+
+
+def test_every_module_imports_first_in_a_fresh_process():
+    import pkgutil
+    import subprocess
+    import sys
+    import algotom_amd
+    names = ["algotom_amd." + m.name for m in pkgutil.walk_packages(
+        algotom_amd.__path__, prefix="") if not m.name.startswith("_")]
+    names += ["algotom_amd.rec." + n for n in ("center", "batch", "autograd",
+                                              "volume", "reconstruction")]
+    for name in sorted(set(names)):
+        res = subprocess.run([sys.executable, "-c", "import " + name],
+                             capture_output=True, text=True, cwd=ROOT)
+        assert res.returncode == 0, name + ": " + res.stderr[-500:]

@@ -1,0 +1,92 @@
+"""Centre search oracle (oracle/center_np.py) pinned by closed forms, brute
+force and a known-answer search; the kernels' summation order pinned
+against numpy itself."""
+
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+from scipy import signal
+
+from oracle import center_np
+from tests import kernel_model as km
+
+
+def test_entropy_of_constant_image_closed_form():
+    # every pixel at level v: h = 1 + counts has one bin at 1 + n, the
+    # 7-wide sliding sums are 7 + n for the windows covering bin v
+    n = 40 * 40
+    for v in (0, 3, 128, 200, 255):
+        img = np.full((40, 40), np.float32(v) / np.float32(255))
+        lev = int(np.uint8(img[0, 0] * 255))
+        cover = sum(1 for k in range(250) if k <= lev <= k + 6)
+        big, small = 7.0 + n, 7.0
+        expect = -(cover * big * np.log2(big)
+                   + (250 - cover) * small * np.log2(small))
+        got = center_np.histogram_entropy(img)
+        assert abs(got - expect) <= 1e-12 * abs(expect)
+
+
+def test_histogram_levels_and_boxcar_by_brute_force():
+    rng = np.random.default_rng(3)
+    img = rng.random((64, 64), dtype=np.float32)
+    levels = np.uint8(img * 255)
+    # uint8 conversion truncates the float32 product
+    brute = [int(float(np.float32(x) * np.float32(255))) for x in img.ravel()]
+    assert np.array_equal(levels.ravel(), np.array(brute, dtype=np.uint8))
+    counts = [0] * 256
+    for b in brute:
+        counts[b] += 1
+    assert np.array_equal(ndi.histogram(levels, 0, 255, 256), counts)
+    h = [1.0 + c for c in counts]
+    sliding = [sum(h[k:k + 7]) for k in range(250)]
+    got = signal.convolve(np.array(h), signal.windows.boxcar(7),
+                          mode="valid")
+    assert np.array_equal(got, sliding)  # integer sums: exact
+    expect = -sum(s * np.log2(s) for s in sliding)
+    assert abs(center_np.histogram_entropy(img) - expect) <= 1e-12 * abs(expect)
+
+
+def test_sharpness_of_linear_ramp_closed_form():
+    y, x = np.mgrid[0:50, 0:50]
+    img = (3 * y + 4 * x).astype(np.float32)  # gradient (3, 4) everywhere
+    assert center_np.edge_sharpness(img) == np.float32(5.0)
+    img2 = (7 * x).astype(np.float32)
+    assert center_np.edge_sharpness(img2) == np.float32(7.0)
+
+
+@pytest.mark.parametrize("size", [3, 37, 128, 200, 513, 1000, 2560])
+def test_kernel_summation_order_is_numpys_mean(size):
+    rng = np.random.default_rng(size)
+    x = (rng.standard_normal((size, size)).astype(np.float32)
+         * np.float32(3.7)) ** 3
+    assert km.np_mean_f32(x) == np.mean(x)
+    assert km.np_mean_f32(np.abs(x)) == np.mean(np.abs(x))
+
+
+def test_pairwise_order_of_numpy_float64_sum():
+    rng = np.random.default_rng(1)
+    for n in (5, 100, 250, 1000):
+        v = rng.standard_normal(n) * 1e3
+        assert km.np_pairwise(v, np.float64) == np.sum(v)
+
+
+def _phantom_sinogram(nrow=181, ncol=96, center=51.0, seed=2):
+    import torch
+    from algotom_amd.util import phantom
+    params = phantom.random_ellipses(64, count=10, seed=seed,
+                                     axis_range=(3.0, 20.0))
+    angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    proj = phantom.ellipse_sinogram(params, torch.from_numpy(angles), ncol,
+                                    center)
+    return phantom.transmission(proj, photons=1e6, seed=seed).numpy(), angles
+
+
+@pytest.mark.parametrize("metric", ["entropy", "sharpness"])
+def test_search_finds_known_centre(metric):
+    sino, angles = _phantom_sinogram(center=51.0)
+    cs, vs = center_np.center_metrics(sino, 46, 57, 1.0, metric=metric,
+                                      angles=angles)
+    assert cs[np.argmin(vs)] == 51.0
+    c = center_np.find_center(sino, 46, 56, 0.5, metric=metric,
+                              angles=angles)
+    assert abs(c - 51.0) <= 0.5
