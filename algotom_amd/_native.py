@@ -60,7 +60,8 @@ SIGNATURES = (
     ("ath_gaussian_filter_rows", _I, [_P, _P, _I, _I, _P, _I, _P]),
     ("ath_zoom_linear", _I, [_P, _I, _I, _P, _I, _I, _P]),
     ("ath_slice_metrics", _I, [_P, _I, _I, ctypes.c_longlong, _F, _F, _I, _P,
-                               _P]),
+                               _P, _S, _P]),
+    ("ath_slice_metrics_workspace_bytes", _S, [_I, _I, _I]),
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),
     ("ath_fbp_reconstruction", _I, [_P, _I, _I, _I, _I, _P, _F, _D, _P, _I,
                                     _I, _I, _I, _I, _I, _P, _S, _P, _P]),

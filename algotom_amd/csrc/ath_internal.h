@@ -104,9 +104,10 @@ int gaussian_rows(const float* in, float* out, int nrows, int ncol,
                   const double* weights, int radius, hipStream_t s);
 int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s);
+size_t slice_metrics_workspace(int nimg, int ncol, int metric);
 int slice_metrics(const float* images, int nimg, int ncol, long long stride,
                   float nmin, float nmax, int metric, double* out,
-                  hipStream_t s);
+                  void* workspace, hipStream_t s);
 int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
                       int overlap, int side, int norm, int width,
                       float* factors, float* out, hipStream_t s);

@@ -396,14 +396,24 @@ int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
                        static_cast<hipStream_t>(stream));
 }
 
+size_t ath_slice_metrics_workspace_bytes(int nimg, int ncol, int metric) {
+    if (nimg < 1 || ncol < 1 || (metric != 0 && metric != 1) ||
+        (long long)ncol * ncol > (1LL << 30))
+        return 0;
+    return slice_metrics_workspace(nimg, ncol, metric);
+}
+
 int ath_slice_metrics(const float* images, int nimg, int ncol,
                       long long stride, float nmin, float nmax, int metric,
-                      double* out, void* stream) {
+                      double* out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
     clear_error();
-    if (!images || !out)
+    if (!images || !out || !workspace)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
     if (nimg < 1 || ncol < 1 || stride < (long long)ncol * ncol)
         return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (nimg > 65535)
+        return set_error(ATH_ERR_UNSUPPORTED, "more than 65535 images a call");
     if (metric != 0 && metric != 1)
         return set_error(ATH_ERR_INVALID_ARG, "metric must be 0 or 1");
     if ((long long)ncol * ncol > (1LL << 30))
@@ -413,8 +423,10 @@ int ath_slice_metrics(const float* images, int nimg, int ncol,
                          "gradient needs at least 2 pixels per axis");
     if (metric == 0 && !(nmax > nmin))
         return set_error(ATH_ERR_INVALID_ARG, "Empty image!!!");
+    if (workspace_bytes < slice_metrics_workspace(nimg, ncol, metric))
+        return set_error(ATH_ERR_WORKSPACE, "slice metrics: workspace too small");
     return slice_metrics(images, nimg, ncol, stride, nmin, nmax, metric, out,
-                         static_cast<hipStream_t>(stream));
+                         workspace, static_cast<hipStream_t>(stream));
 }
 
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,

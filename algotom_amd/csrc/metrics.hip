@@ -11,14 +11,16 @@
 //                        float64 sums, float32 output.
 // zoom_linear_kernel     scipy.ndimage.zoom(x, z, order=1, mode="nearest")
 //                        (:1434), bilinear in float64.
-// entropy_kernel         __calculate_histogram_entropy (:1211-1220) on
+// level_histogram_kernel + entropy_finish_kernel
+//                        __calculate_histogram_entropy (:1211-1220) on
 //                        clip((rec - nmin) / (nmax - nmin), 0, 1): uint8
 //                        levels v = trunc(255 r) (float32), 256-bin counts
 //                        (ndi.histogram(., 0, 255, 256) puts level v in bin
 //                        v), h = 1 + counts, 7-tap boxcar "valid",
 //                        metric = -sum h log2 h (numpy's pairwise sum).
 //                        Integer LDS counts: exact.
-// sharpness_kernel       __calculate_edge_sharpness (:1223-1228): np.mean of
+// sharpness_chunk_kernel + sharpness_finish_kernel
+//                        __calculate_edge_sharpness (:1223-1228): np.mean of
 //                        |np.gradient| (central differences inside, one-sided
 //                        at the borders), float32 sums in numpy's order.
 #include "ath_internal.h"
@@ -114,26 +116,47 @@ __device__ T np_pairwise_serial(const T* a, int n) {
     return val[0];
 }
 
-__global__ __launch_bounds__(kThreads) void entropy_kernel(
+// Histogram of the uint8 levels: block (b, k) counts its contiguous share
+// of image k into LDS and writes the 256 counts to
+// partial[(k * nblk + b) * 256 + bin] (integers: the split changes nothing).
+__global__ __launch_bounds__(kThreads) void level_histogram_kernel(
     const float* __restrict__ images, long long stride, int ncol, float nmin,
-    float nmax, double* __restrict__ out) {
+    float nmax, unsigned int* __restrict__ partial) {
     __shared__ unsigned int hist[256];
-    __shared__ double h7[256];
-    const float* img = images + (size_t)blockIdx.x * (size_t)stride;
     const int tid = threadIdx.x;
+    const int nblk = gridDim.x;
+    const float* img = images + (size_t)blockIdx.y * (size_t)stride;
     hist[tid] = 0u;
     __syncthreads();
     const long long n = (long long)ncol * ncol;
+    const long long per = (n + nblk - 1) / nblk;
+    const long long e0 = (long long)blockIdx.x * per;
+    const long long e1 = min(n, e0 + per);
     const float range = nmax - nmin;
-    for (long long e = tid; e < n; e += kThreads) {
+    for (long long e = e0 + tid; e < e1; e += kThreads) {
         float r = (img[e] - nmin) / range;
         r = fminf(fmaxf(r, 0.0f), 1.0f);
         const int v = (int)(r * 255.0f);  // np.uint8 truncation
         atomicAdd(&hist[v], 1u);
     }
     __syncthreads();
-    // h = 1 + counts; boxcar(7) "valid" -> 250 sliding sums of integers
-    // (exact in float64), then h log2 h
+    partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 256 + tid] = hist[tid];
+}
+
+// One workgroup per image: add the block histograms, h = 1 + counts,
+// boxcar(7) "valid" -> 250 sliding sums of integers (exact in float64),
+// metric = -sum h log2 h with numpy's pairwise order.
+__global__ __launch_bounds__(kThreads) void entropy_finish_kernel(
+    const unsigned int* __restrict__ partial, int nblk,
+    double* __restrict__ out) {
+    __shared__ unsigned int hist[256];
+    __shared__ double h7[256];
+    const int tid = threadIdx.x;
+    const unsigned int* p = partial + (size_t)blockIdx.x * nblk * 256;
+    unsigned int c = 0u;
+    for (int b = 0; b < nblk; ++b) c += p[(size_t)b * 256 + tid];
+    hist[tid] = c;
+    __syncthreads();
     if (tid < 250) {
         double s = 0.0;
         for (int k = 0; k < 7; ++k) s += 1.0 + (double)hist[tid + k];
@@ -161,41 +184,58 @@ __device__ __forceinline__ float grad_magnitude(const float* img, int ncol,
 // np.mean of a float32 image: numpy reduces through its 8192-element
 // buffer, so the float32 sum is a sequential float32 accumulation (from 0)
 // of pairwise sums of consecutive 8192-value chunks of the flattened image;
-// mean = float32(float64(sum) / n).  One workgroup per image walks its
-// chunks in order: the chunk's magnitudes go to LDS, a full chunk is 64
-// blocks of 128 (64 threads) combined as the perfect binary tree the
-// recursion forms, a shorter tail chunk is summed by one thread.
-__global__ __launch_bounds__(kThreads) void sharpness_kernel(
+// mean = float32(float64(sum) / n).
+// Block (c, k) forms chunk c of image k: magnitudes to LDS; a full chunk is
+// 64 blocks of 128 (64 threads) combined as the perfect binary tree the
+// recursion forms, a shorter tail chunk is summed by one thread.  The chunk
+// sum goes to partial[k * nchunk + c].
+__global__ __launch_bounds__(kThreads) void sharpness_chunk_kernel(
     const float* __restrict__ images, long long stride, int ncol,
-    double* __restrict__ out) {
+    float* __restrict__ partial) {
     __shared__ float mag[kChunk];
     __shared__ float part[64];
-    const float* img = images + (size_t)blockIdx.x * (size_t)stride;
+    const float* img = images + (size_t)blockIdx.y * (size_t)stride;
     const int tid = threadIdx.x;
     const int n = ncol * ncol;
-    float total = 0.0f;  // meaningful on thread 0 only
-    for (int c0 = 0; c0 < n; c0 += kChunk) {
-        const int m = min(kChunk, n - c0);
-        for (int k = tid; k < m; k += kThreads)
-            mag[k] = grad_magnitude(img, ncol, c0 + k);
+    const int c0 = blockIdx.x * kChunk;
+    const int m = min(kChunk, n - c0);
+    for (int k = tid; k < m; k += kThreads)
+        mag[k] = grad_magnitude(img, ncol, c0 + k);
+    __syncthreads();
+    float chunk;
+    if (m == kChunk) {
+        if (tid < 64) part[tid] = np_pairwise_leaf(mag + 128 * tid, 128);
         __syncthreads();
-        float chunk;
-        if (m == kChunk) {
-            if (tid < 64) part[tid] = np_pairwise_leaf(mag + 128 * tid, 128);
+        for (int w = 1; w < 64; w <<= 1) {
+            if (tid < 64 && (tid % (2 * w)) == 0)
+                part[tid] = part[tid] + part[tid + w];
             __syncthreads();
-            for (int w = 1; w < 64; w <<= 1) {
-                if (tid < 64 && (tid % (2 * w)) == 0)
-                    part[tid] = part[tid] + part[tid + w];
-                __syncthreads();
-            }
-            chunk = part[0];
-        } else {
-            chunk = tid == 0 ? np_pairwise_serial(mag, m) : 0.0f;
         }
-        if (tid == 0) total = total + chunk;
-        __syncthreads();  // mag / part are rewritten by the next chunk
+        chunk = part[0];
+    } else {
+        chunk = tid == 0 ? np_pairwise_serial(mag, m) : 0.0f;
     }
-    if (tid == 0) out[blockIdx.x] = (double)(float)((double)total / (double)n);
+    if (tid == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = chunk;
+}
+
+// One workgroup per image: the chunk sums added in order, staged in LDS.
+__global__ __launch_bounds__(kThreads) void sharpness_finish_kernel(
+    const float* __restrict__ partial, int nchunk, int ncol,
+    double* __restrict__ out) {
+    __shared__ float buf[4096];
+    const int tid = threadIdx.x;
+    const float* p = partial + (size_t)blockIdx.x * nchunk;
+    float total = 0.0f;  // meaningful on thread 0 only
+    for (int b0 = 0; b0 < nchunk; b0 += 4096) {
+        const int m = min(4096, nchunk - b0);
+        for (int k = tid; k < m; k += kThreads) buf[k] = p[b0 + k];
+        __syncthreads();
+        if (tid == 0)
+            for (int k = 0; k < m; ++k) total = total + buf[k];
+        __syncthreads();
+    }
+    const double n = (double)ncol * (double)ncol;
+    if (tid == 0) out[blockIdx.x] = (double)(float)((double)total / n);
 }
 
 // scipy.ndimage.zoom(x, zoom, order=1, mode="nearest") of a 2D float32
@@ -246,16 +286,44 @@ int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
     return check_hip(hipGetLastError(), "zoom_linear_kernel launch");
 }
 
+static int entropy_blocks(int ncol) {
+    const long long n = (long long)ncol * ncol;
+    return (int)std::min<long long>(
+        64, std::max<long long>(1, (n + 65535) / 65536));
+}
+
+size_t slice_metrics_workspace(int nimg, int ncol, int metric) {
+    if (metric == 0)
+        return (size_t)nimg * entropy_blocks(ncol) * 256 * sizeof(unsigned);
+    const long long n = (long long)ncol * ncol;
+    return (size_t)nimg * (size_t)((n + kChunk - 1) / kChunk) * sizeof(float);
+}
+
 int slice_metrics(const float* images, int nimg, int ncol, long long stride,
                   float nmin, float nmax, int metric, double* out,
-                  hipStream_t s) {
-    if (metric == 0)
-        hipLaunchKernelGGL(entropy_kernel, dim3(nimg), dim3(kThreads), 0, s,
-                           images, stride, ncol, nmin, nmax, out);
-    else
-        hipLaunchKernelGGL(sharpness_kernel, dim3(nimg), dim3(kThreads), 0, s,
-                           images, stride, ncol, out);
-    return check_hip(hipGetLastError(), "slice metric launch");
+                  void* workspace, hipStream_t s) {
+    if (metric == 0) {
+        const int nblk = entropy_blocks(ncol);
+        auto* partial = static_cast<unsigned int*>(workspace);
+        hipLaunchKernelGGL(level_histogram_kernel, dim3(nblk, nimg),
+                           dim3(kThreads), 0, s, images, stride, ncol, nmin,
+                           nmax, partial);
+        const int st = check_hip(hipGetLastError(), "level_histogram_kernel");
+        if (st != ATH_OK) return st;
+        hipLaunchKernelGGL(entropy_finish_kernel, dim3(nimg), dim3(kThreads),
+                           0, s, partial, nblk, out);
+        return check_hip(hipGetLastError(), "entropy_finish_kernel");
+    }
+    const int n = ncol * ncol;
+    const int nchunk = (n + kChunk - 1) / kChunk;
+    auto* partial = static_cast<float*>(workspace);
+    hipLaunchKernelGGL(sharpness_chunk_kernel, dim3(nchunk, nimg),
+                       dim3(kThreads), 0, s, images, stride, ncol, partial);
+    const int st = check_hip(hipGetLastError(), "sharpness_chunk_kernel");
+    if (st != ATH_OK) return st;
+    hipLaunchKernelGGL(sharpness_finish_kernel, dim3(nimg), dim3(kThreads), 0,
+                       s, partial, nchunk, ncol, out);
+    return check_hip(hipGetLastError(), "sharpness_finish_kernel");
 }
 
 }  // namespace ath

@@ -154,12 +154,15 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
                 values.append(float(metric_function(imgs[k].cpu().numpy(),
                                                     **kwargs)))
             continue
+        code = 0 if metric == "entropy" else 1
         out = torch.empty(len(cs), dtype=torch.float64, device=dev)
+        nws = lib.ath_slice_metrics_workspace_bytes(len(cs), ncol, code)
+        ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
         nat.check(lib.ath_slice_metrics(
             nat.ptr(imgs), len(cs), ncol, ncol * ncol,
-            float(np.float32(nmin)), float(np.float32(nmax)),
-            0 if metric == "entropy" else 1, nat.ptr(out),
-            nat.stream_ptr(dev)), "ath_slice_metrics")
+            float(np.float32(nmin)), float(np.float32(nmax)), code,
+            nat.ptr(out), nat.ptr(ws), nws, nat.stream_ptr(dev)),
+            "ath_slice_metrics")
         values.extend(out.cpu().numpy().tolist())
     list_metric = np.asarray(values, dtype=np.float64)
     if invert_metric:

@@ -38,6 +38,14 @@ filter launch per FFT length and one persistent back-projection launch,
 captured in a HIP graph, tile height autotuned; every precision mode is
 timed and its error against the all-FP64 oracle reported on sampled
 sinograms.
+
+--workload center: centre-of-rotation search on one 1801-angle x
+2560-column transmission sinogram (seed 1000, true centre 1280.37):
+find_center_based_slice_metric(start=1260, stop=1300, step=0.5, radius=2,
+metric="entropy") -- a coarse integer pass over 41 centres plus the reference
+slice, then a sub-pixel pass; every pass smooths, logs and filters once and
+back-projects all its centres in one batched launch.  value = candidate
+slices reconstructed and scored per second.
 """
 
 import argparse
@@ -323,6 +331,62 @@ def run_refine(args, dev):
     print(json.dumps(line))
 
 
+def run_center(args, dev):
+    import torch
+    from algotom_amd.rec import center
+    from algotom_amd.util import phantom
+    nrow, ncol, c_true = NROW, NCOL, CENTER
+    angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    prm = phantom.random_ellipses(ncol, 20, seed=1000)
+    p = phantom.ellipse_sinogram(prm, angles, ncol, c_true, device=dev)
+    sino = phantom.transmission(p, seed=1000).contiguous()
+    torch.cuda.synchronize()
+    kw = dict(step=0.5, radius=2, metric="entropy", angles=angles)
+    ncoarse = len(np.arange(1260, 1301, 1.0))
+    nfine = len(np.arange(0.0, 2 * 2.0 + 0.5, 0.5))
+    nslices = ncoarse + 1 + nfine + 1   # + the nmin/nmax reference slices
+    for _ in range(args.warmup):
+        found = center.find_center_based_slice_metric(sino, 1260, 1300, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        found = center.find_center_based_slice_metric(sino, 1260, 1300, **kw)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    base = None
+    if not args.no_cpu_baseline:
+        # one candidate of the reference's loop (reconstruction.py:1349-1355:
+        # FBP at one centre, then its entropy) with the oracle's C port
+        from oracle import center_np, port
+        s_host = sino.cpu().numpy()
+        pre = center_np.preprocess(s_host, 2, True)
+        threads = port.default_threads()
+        t0 = time.perf_counter()
+        rec0 = port.fbp(pre, c_true, angles=angles, apply_log=False)
+        center_np.slice_metric(rec0, "entropy", np.min(rec0), np.max(rec0))
+        dt = time.perf_counter() - t0
+        base = {"value": 1.0 / dt, "unit": "slices/s", "cores": threads,
+                "kind": "port",
+                "sample": "one 1801x2560 candidate slice: FBP (C port, "
+                          "float64 sums) + histogram entropy"}
+    line = {
+        "metric": METRIC, "value": nslices / (ms * 1e-3),
+        "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (f64 sums)",
+        "data": "synthetic",
+        "config": {"workload": "center_search_1801x2560_entropy",
+                   "start": 1260, "stop": 1300, "step": 0.5, "radius": 2,
+                   "slices_per_search": nslices,
+                   "updates_per_search": nslices * ncol * ncol * nrow,
+                   "parallelism": "single-gpu"},
+        "cpu_baseline": base,
+        "detail": {"found": float(found), "true_center": c_true,
+                   "gups": nslices * ncol * ncol * nrow / (ms * 1e-3) / 1e9},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,7 +396,8 @@ def main():
                     choices=[None, "fp32", "fp16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="stack",
-                    choices=["stack", "uneven", "refine", "volume"])
+                    choices=["stack", "uneven", "refine", "volume",
+                             "center"])
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -354,6 +419,9 @@ def main():
         return
     if args.workload == "volume":
         run_volume(args, dev)
+        return
+    if args.workload == "center":
+        run_center(args, dev)
         return
     sino, angles = make_stack(dev)
     updates = NSLICE * NCOL * NCOL * NROW

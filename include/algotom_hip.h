@@ -261,10 +261,16 @@ int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
  * clip((img - nmin)/(nmax - nmin), 0, 1) (reconstruction.py:1211-1220,
  * 1243-1246), metric 1 = mean gradient magnitude (:1223-1228; ncol >= 2).
  * Sums follow numpy's order (pairwise within 8192-value chunks).  out: nimg
- * float64 (device). */
+ * float64 (device); workspace: caller device memory of at least
+ * ath_slice_metrics_workspace_bytes(nimg, ncol, metric) bytes. */
 int ath_slice_metrics(const float* images, int nimg, int ncol,
                       long long stride, float nmin, float nmax, int metric,
-                      double* out, void* stream);
+                      double* out, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
+/* Device workspace bytes ath_slice_metrics needs (per-block partial
+ * histograms or per-chunk partial sums); 0 for invalid arguments. */
+size_t ath_slice_metrics_workspace_bytes(int nimg, int ncol, int metric);
 
 /* Device workspace bytes needed by ath_fbp_reconstruction. */
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,
