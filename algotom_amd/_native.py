@@ -57,6 +57,15 @@ SIGNATURES = (
                                      _P]),
     ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _I, _I, _I, _I,
                                              _P, _P, _P]),
+    ("ath_minus_log", _I, [_P, _P, ctypes.c_longlong, _P]),
+    ("ath_circular_filter_plan_bytes", _S, [_I]),
+    ("ath_circular_filter_plan_work_bytes", _S, [_I, _I]),
+    ("ath_circular_filter_plan_build", _I, [_P, _I, _I, _P, _S, _P, _S, _P]),
+    ("ath_circular_ramp_filter_workspace_bytes", _S, [_I, _I, _I, _I]),
+    ("ath_circular_ramp_filter", _I, [_P, _I, _I, ctypes.c_longlong,
+                                      ctypes.c_longlong, _I, _I, _P, _I, _P,
+                                      _S, _P, _I, ctypes.c_longlong,
+                                      ctypes.c_longlong, _D, _I, _P]),
     ("ath_gaussian_filter_rows", _I, [_P, _P, _I, _I, _P, _I, _P]),
     ("ath_zoom_linear", _I, [_P, _I, _I, _P, _I, _I, _P]),
     ("ath_slice_metrics", _I, [_P, _I, _I, ctypes.c_longlong, _F, _F, _I, _P,

@@ -12,7 +12,7 @@ SRC_DIR = os.path.join(_HERE, "csrc")
 LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libalgotom_hip.so")
 SOURCES = ("capi.hip", "ramp_filter.hip", "backproject.hip",
-           "forward_project.hip", "conversion.hip",
+           "forward_project.hip", "conversion.hip", "filter2d.hip",
            "metrics.hip")
 
 HIPCC_FLAGS = [

@@ -102,6 +102,18 @@ struct FpArgs {
 int forward_projection(const FpArgs& a, hipStream_t stream);
 int gaussian_rows(const float* in, float* out, int nrows, int ncol,
                   const double* weights, int radius, hipStream_t s);
+size_t circular_plan_bytes(int L);
+size_t circular_plan_work_bytes(int width, int L);
+size_t circular_filter_work_bytes(int ncol, int pad, int L, int nimg);
+int circular_plan_build(const double* g2, int width, int L, void* plan,
+                        void* work, hipStream_t s);
+int circular_ramp_filter(const float* images, int nimg, int ncol,
+                         long long row_stride, long long img_stride, int pad,
+                         int pad_mode, const void* plan, int L, void* work,
+                         void* out, int out_dtype, long long out_row_stride,
+                         long long out_img_stride, double mask_radius,
+                         int scale_nrow, hipStream_t s);
+int minus_log(const float* in, float* out, long long n, hipStream_t s);
 int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s);
 size_t slice_metrics_workspace(int nimg, int ncol, int metric);

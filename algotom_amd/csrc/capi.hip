@@ -373,6 +373,90 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                              static_cast<hipStream_t>(stream));
 }
 
+int ath_minus_log(const float* in, float* out, long long n, void* stream) {
+    clear_error();
+    if (!in || !out) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (n < 0) return set_error(ATH_ERR_INVALID_ARG, "negative length");
+    if (n == 0) return ATH_OK;
+    return minus_log(in, out, n, static_cast<hipStream_t>(stream));
+}
+
+static bool circular_len_ok(int L) {
+    return L >= 64 && L <= 8192 && (L & (L - 1)) == 0;
+}
+
+size_t ath_circular_filter_plan_bytes(int fft_len) {
+    return circular_len_ok(fft_len) ? circular_plan_bytes(fft_len) : 0;
+}
+
+size_t ath_circular_filter_plan_work_bytes(int width, int fft_len) {
+    if (width < 1 || !circular_len_ok(fft_len)) return 0;
+    return circular_plan_work_bytes(width, fft_len);
+}
+
+int ath_circular_filter_plan_build(const double* kernel2d, int width,
+                                   int fft_len, void* plan, size_t plan_bytes,
+                                   void* work, size_t work_bytes,
+                                   void* stream) {
+    clear_error();
+    if (!kernel2d || !plan || !work)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (width < 1 || !circular_len_ok(fft_len) || fft_len < width)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "circular filter: fft_len must be a power of two in "
+                         "[64, 8192] and at least the width");
+    if (plan_bytes < circular_plan_bytes(fft_len) ||
+        work_bytes < circular_plan_work_bytes(width, fft_len))
+        return set_error(ATH_ERR_WORKSPACE,
+                         "circular filter plan: buffer too small");
+    return circular_plan_build(kernel2d, width, fft_len, plan, work,
+                               static_cast<hipStream_t>(stream));
+}
+
+size_t ath_circular_ramp_filter_workspace_bytes(int ncol, int pad,
+                                                int fft_len, int nimg) {
+    if (ncol < 1 || pad < 0 || nimg < 1 || !circular_len_ok(fft_len))
+        return 0;
+    return circular_filter_work_bytes(ncol, pad, fft_len, nimg);
+}
+
+int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
+                             long long row_stride, long long img_stride,
+                             int pad, int pad_mode, const void* plan,
+                             int fft_len, void* workspace,
+                             size_t workspace_bytes, void* out, int out_dtype,
+                             long long out_row_stride,
+                             long long out_img_stride, double mask_radius,
+                             int scale_nrow, void* stream) {
+    clear_error();
+    if (!images || !plan || !workspace || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nimg < 1 || ncol < 1 || pad < 0 || nimg > 65535)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (row_stride < ncol || out_row_stride < ncol)
+        return set_error(ATH_ERR_INVALID_ARG, "row stride below ncol");
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+        return set_error(ATH_ERR_INVALID_ARG, "bad pad mode");
+    if (out_dtype != ATH_F32 && out_dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "circular filter output must be float32 or float64");
+    if (!circular_len_ok(fft_len) || fft_len < fft_length(ncol, pad))
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "circular filter: fft_len must be a power of two, "
+                         "at least ath_filter_fft_length(ncol, pad) and at "
+                         "most 8192");
+    if (workspace_bytes < circular_filter_work_bytes(ncol, pad, fft_len, nimg))
+        return set_error(ATH_ERR_WORKSPACE,
+                         "circular filter: workspace too small");
+    if (scale_nrow == 1)
+        return set_error(ATH_ERR_INVALID_ARG, "scale needs nrow >= 2");
+    return circular_ramp_filter(images, nimg, ncol, row_stride, img_stride,
+                                pad, pad_mode, plan, fft_len, workspace, out,
+                                out_dtype, out_row_stride, out_img_stride,
+                                mask_radius, scale_nrow,
+                                static_cast<hipStream_t>(stream));
+}
+
 int ath_gaussian_filter_rows(const float* in, float* out, int nrows,
                              int ncol, const double* weights, int radius,
                              void* stream) {

@@ -114,7 +114,24 @@ __global__ __launch_bounds__(kThreads) void stitch_kernel(
     }
 }
 
+// out = -log(in) elementwise, float32 <- float64 log rounded (the filter
+// kernel's convention for the same step)
+__global__ __launch_bounds__(kThreads) void minus_log_kernel(
+    const float* __restrict__ in, float* __restrict__ out, long long n) {
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < n;
+         e += (long long)gridDim.x * kThreads)
+        out[e] = (float)(-log((double)in[e]));
+}
+
 }  // namespace
+
+int minus_log(const float* in, float* out, long long n, hipStream_t s) {
+    const long long blocks =
+        std::min<long long>((n + kThreads - 1) / kThreads, 1 << 20);
+    hipLaunchKernelGGL(minus_log_kernel, dim3((unsigned)blocks),
+                       dim3(kThreads), 0, s, in, out, n);
+    return check_hip(hipGetLastError(), "minus_log_kernel launch");
+}
 
 int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
                       int overlap, int side, int norm, int width,

@@ -1,0 +1,345 @@
+// 2D circular ramp filter of BPF (back-projection filtering) on gfx950.
+//
+// algotom's apply_circular_ramp_filter (reconstruction.py:524-560) pads the
+// back-projected image by `pad` on every side (numpy.pad, one mode for both
+// axes) to W x W, multiplies its shifted 2D spectrum by the circular ramp
+// window R2 and keeps the real part of the inverse.  That is a W x W circular
+// convolution with the real kernel g2 = Re(ifft2(ifftshift(R2))).  As in the
+// 1D filter, it is evaluated exactly as a LINEAR convolution with the kernel
+// tiled 2 x 2 (K[i][j] = g2[i mod W][j mod W], i, j < 2W) on a power-of-two
+// L x L grid, L >= 2W - pad, whose outputs (W+pad+y, W+pad+x), y, x < ncol,
+// never wrap.  All arithmetic is float64, in three passes over one image:
+//
+//   rows_fwd_kernel   one workgroup per padded row u < W: real FFT of length
+//                     L (L/2 packed complex samples in LDS + split), the
+//                     L/2+1 bins stored row-major T[u][k];
+//   cols_kernel       one workgroup per bin column k <= L/2: the W values
+//                     T[.][k] zero-extended to L, complex FFT of length L,
+//                     times the kernel spectrum column Khat[k][.], inverse
+//                     FFT, the ncol needed rows stored U[y][k];
+//   rows_inv_kernel   one workgroup per output row y: Hermitian merge, inverse
+//                     real FFT, crop, optional circle mask and pi/(N-1) scale
+//                     in the reference's order, float32 or float64 store.
+//
+// The kernel spectrum Khat (L/2+1 columns of L complex values, 1/(M L)
+// folded in) is built once per (W, window) by the same two forward passes
+// over the 2W rows of K.  The column transform holds L complex doubles in
+// LDS, so L <= 8192 (ncol up to about 3800 with the default pad).
+// Kernel samples at index >= L are dropped rather than wrapped: a kept
+// output (W+pad+y, W+pad+x) only meets K at offsets pad+1 .. 2W-pad-1 < L
+// in each axis, so no needed term is lost and none aliases.
+#include "fft_lds.h"
+
+namespace ath {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSrcImage = 0;   // rows of the padded image
+constexpr int kSrcKernel = 1;  // rows of the tiled kernel K
+
+struct Plan2d {
+    const double2* tw;    // L entries, exp(-2 pi i n / L)
+    const double2* tw2;   // 2L entries, exp(-2 pi i n / 2L)
+    const double2* khat;  // (M + 1) x L, column k contiguous
+};
+
+__host__ __device__ inline Plan2d plan_view(const void* plan, int L) {
+    const double2* base = static_cast<const double2*>(plan);
+    return Plan2d{base, base + L, base + 3 * (size_t)L};
+}
+
+// Forward real FFT of one row: image rows with numpy.pad on both axes, or
+// rows of the tiled kernel.  T row r of image k: T + (k * nrows + r) * (M+1).
+template <int MAXB, int SRC>
+__global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
+    const float* __restrict__ images, long long row_stride,
+    long long img_stride, const double* __restrict__ g2, int ncol, int pad,
+    int pad_mode, int width, const double2* __restrict__ tw,
+    double2* __restrict__ T, int M) {
+    extern __shared__ double2 buf[];
+    const int tid = threadIdx.x;
+    const int u = blockIdx.x;
+    const int nrows = gridDim.x;
+    const int img = blockIdx.y;
+    const float* src_row = nullptr;
+    const double* g_row = nullptr;
+    if (SRC == kSrcImage) {
+        const int su = pad_source(u - pad, ncol, pad_mode);
+        if (su >= 0)
+            src_row = images + img * img_stride + (long long)su * row_stride;
+    } else {
+        g_row = g2 + (size_t)(u % width) * width;
+    }
+    for (int n = tid; n < M; n += kThreads) {
+        double v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int col = 2 * n + h;
+            v[h] = 0.0;
+            if (SRC == kSrcImage) {
+                if (src_row && col < width) {
+                    const int sv = pad_source(col - pad, ncol, pad_mode);
+                    if (sv >= 0) v[h] = (double)src_row[sv];
+                }
+            } else if (col < 2 * width) {
+                v[h] = g_row[col < width ? col : col - width];
+            }
+        }
+        buf[n] = make_double2(v[0], v[1]);
+    }
+    __syncthreads();
+    lds_fft<-1, MAXB>(buf, M, tw);
+    double2* out = T + ((size_t)img * nrows + u) * (M + 1);
+    for (int k = tid; k <= M / 2; k += kThreads) {
+        const int km = (M - k) & (M - 1);
+        double2 xk, xm;
+        split_real(buf[k], buf[km], tw[k], xk, xm);
+        out[k] = xk;
+        out[M - k] = xm;
+    }
+}
+
+// Column pass.  SPECTRUM: store the forward transform of column k, scaled,
+// as Khat[k][.].  Otherwise multiply by Khat[k][.], transform back and keep
+// rows r0 .. r0+nout-1 into U[y][k].
+template <int MAXB, bool SPECTRUM>
+__global__ __launch_bounds__(kThreads) void cols_kernel(
+    const double2* __restrict__ T, int nrows_in, const double2* __restrict__ tw2,
+    double2* __restrict__ khat_out, const double2* __restrict__ khat,
+    double2* __restrict__ U, int r0, int nout, int M, double scale) {
+    extern __shared__ double2 buf[];
+    const int tid = threadIdx.x;
+    const int k = blockIdx.x;
+    const int img = blockIdx.y;
+    const int L = 2 * M;
+    const double2* col = T + (size_t)img * nrows_in * (M + 1) + k;
+    for (int i = tid; i < L; i += kThreads)
+        buf[i] = i < nrows_in ? col[(size_t)i * (M + 1)] : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, MAXB>(buf, L, tw2);
+    if (SPECTRUM) {
+        double2* dst = khat_out + (size_t)k * L;
+        for (int i = tid; i < L; i += kThreads) dst[i] = cscale(buf[i], scale);
+        return;
+    }
+    const double2* kc = khat + (size_t)k * L;
+    for (int i = tid; i < L; i += kThreads) buf[i] = cmul(buf[i], kc[i]);
+    __syncthreads();
+    lds_fft<1, MAXB>(buf, L, tw2);
+    double2* dst = U + (size_t)img * nout * (M + 1) + k;
+    for (int y = tid; y < nout; y += kThreads)
+        dst[(size_t)y * (M + 1)] = buf[r0 + y];
+}
+
+struct Finish2d {
+    int ncol, c0;             // output columns, first kept sample
+    long long out_row_stride, out_img_stride;
+    int use_mask, mask_c;
+    double mask_r2;
+    int scale;                // multiply by pi / (nrow - 1)
+    double pi64, denom64;     // float64 output
+    float pi32, denom32;      // float32 output
+};
+
+// Row inverse: Hermitian merge of U[y][0..M], inverse FFT, crop, finish.
+template <int MAXB, typename Tout>
+__global__ __launch_bounds__(kThreads) void rows_inv_kernel(
+    const double2* __restrict__ U, const double2* __restrict__ tw,
+    Tout* __restrict__ out, Finish2d f, int M) {
+    extern __shared__ double2 buf[];
+    const int tid = threadIdx.x;
+    const int y = blockIdx.x;
+    const int img = blockIdx.y;
+    const double2* Y = U + ((size_t)img * gridDim.x + y) * (M + 1);
+    for (int k = tid; k <= M / 2; k += kThreads) {
+        const int km = (M - k) & (M - 1);
+        const double2 wk = tw[k];
+        const double2 yk = Y[k];
+        const double2 ym = Y[M - k];
+        // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
+        const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
+        const double2 ok = rot<1>(cmul(conjg(wk),
+                                       cscale(csub(yk, conjg(ym)), 0.5)));
+        buf[k] = cadd(ek, ok);
+        if (k != 0 && k != km) {
+            // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
+            const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
+            const double2 om = rot<-1>(cmul(wk,
+                                            cscale(csub(ym, conjg(yk)), 0.5)));
+            buf[km] = cadd(em, om);
+        }
+    }
+    __syncthreads();
+    lds_fft<1, MAXB>(buf, M, tw);
+    const double* v = reinterpret_cast<const double*>(buf) + f.c0;
+    Tout* row = out + img * f.out_img_stride + (long long)y * f.out_row_stride;
+    for (int x = tid; x < f.ncol; x += kThreads) {
+        bool keep = true;
+        if (f.use_mask) {
+            const long long dx = x - f.mask_c, dy = y - f.mask_c;
+            keep = (double)(dx * dx + dy * dy) <= f.mask_r2;
+        }
+        if constexpr (sizeof(Tout) == 8) {
+            // float64 result of the reference's 2D branch: recon * mask,
+            // then * pi, then / (nrow - 1), all float64
+            double r = v[x] * (keep ? 1.0 : 0.0);
+            if (f.scale) r = r * f.pi64 / f.denom64;
+            row[x] = r;
+        } else {
+            // 3D branch: float32 slices, then * mask, * pi, / (nrow - 1)
+            float r = (float)v[x] * (keep ? 1.0f : 0.0f);
+            if (f.scale) {
+                r = r * f.pi32;
+                r = r / f.denom32;
+            }
+            row[x] = r;
+        }
+    }
+}
+
+template <int MAXB_R, int MAXB_C>
+int build_plan(const double* g2, int width, int L, void* plan, void* work,
+               hipStream_t s) {
+    const int M = L / 2;
+    double2* tw = static_cast<double2*>(plan);
+    double2* tw2 = tw + L;
+    double2* khat = tw + 3 * (size_t)L;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((L + 255) / 256), dim3(256), 0, s,
+                       tw, L);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((2 * L + 255) / 256), dim3(256),
+                       0, s, tw2, 2 * L);
+    int st = check_hip(hipGetLastError(), "twiddle_kernel launch");
+    if (st != ATH_OK) return st;
+    auto* T = static_cast<double2*>(work);
+    // rows of K at or beyond L never reach a kept output (see the header)
+    const int krows = std::min(2 * width, L);
+    auto rk = rows_fwd_kernel<MAXB_R, kSrcKernel>;
+    const size_t lds_r = (size_t)M * sizeof(double2);
+    prepare_lds(rk, 1024 * (size_t)MAXB_R * sizeof(double2));
+    hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(kThreads), lds_r, s,
+                       nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE, width,
+                       tw, T, M);
+    st = check_hip(hipGetLastError(), "rows_fwd_kernel (kernel) launch");
+    if (st != ATH_OK) return st;
+    auto ck = cols_kernel<MAXB_C, true>;
+    const size_t lds_c = (size_t)L * sizeof(double2);
+    prepare_lds(ck, 1024 * (size_t)MAXB_C * sizeof(double2));
+    hipLaunchKernelGGL(ck, dim3(M + 1, 1), dim3(kThreads), lds_c, s, T,
+                       krows, tw2, khat, nullptr, nullptr, 0, 0, M,
+                       1.0 / ((double)M * (double)L));
+    return check_hip(hipGetLastError(), "cols_kernel (spectrum) launch");
+}
+
+template <int MAXB_R, int MAXB_C, typename Tout>
+int run_filter(const float* images, int nimg, int ncol, long long row_stride,
+               long long img_stride, int pad, int pad_mode, const void* plan,
+               int L, void* work, Tout* out, const Finish2d& f,
+               hipStream_t s) {
+    const int M = L / 2;
+    const int width = ncol + 2 * pad;
+    const Plan2d p = plan_view(plan, L);
+    auto* T = static_cast<double2*>(work);
+    auto* U = T + (size_t)nimg * width * (M + 1);
+    auto rk = rows_fwd_kernel<MAXB_R, kSrcImage>;
+    const size_t lds_r = (size_t)M * sizeof(double2);
+    prepare_lds(rk, 1024 * (size_t)MAXB_R * sizeof(double2));
+    hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(kThreads), lds_r, s, images,
+                       row_stride, img_stride, nullptr, ncol, pad, pad_mode,
+                       width, p.tw, T, M);
+    int st = check_hip(hipGetLastError(), "rows_fwd_kernel launch");
+    if (st != ATH_OK) return st;
+    auto ck = cols_kernel<MAXB_C, false>;
+    const size_t lds_c = (size_t)L * sizeof(double2);
+    prepare_lds(ck, 1024 * (size_t)MAXB_C * sizeof(double2));
+    hipLaunchKernelGGL(ck, dim3(M + 1, nimg), dim3(kThreads), lds_c, s, T,
+                       width, p.tw2, nullptr, p.khat, U, width + pad, ncol, M,
+                       1.0);
+    st = check_hip(hipGetLastError(), "cols_kernel launch");
+    if (st != ATH_OK) return st;
+    auto ik = rows_inv_kernel<MAXB_R, Tout>;
+    prepare_lds(ik, 1024 * (size_t)MAXB_R * sizeof(double2));
+    hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(kThreads), lds_r, s, U,
+                       p.tw, out, f, M);
+    return check_hip(hipGetLastError(), "rows_inv_kernel launch");
+}
+
+}  // namespace
+
+size_t circular_plan_bytes(int L) {
+    const size_t M = (size_t)L / 2;
+    return (3 * (size_t)L + (M + 1) * (size_t)L) * sizeof(double2);
+}
+
+size_t circular_plan_work_bytes(int width, int L) {
+    return 2 * (size_t)width * ((size_t)L / 2 + 1) * sizeof(double2);
+}
+
+size_t circular_filter_work_bytes(int ncol, int pad, int L, int nimg) {
+    const size_t width = (size_t)ncol + 2 * (size_t)pad;
+    return (size_t)nimg * (width + (size_t)ncol) * ((size_t)L / 2 + 1) *
+           sizeof(double2);
+}
+
+int circular_plan_build(const double* g2, int width, int L, void* plan,
+                        void* work, hipStream_t s) {
+    switch (L) {
+    case 64: case 128: case 256: case 512: case 1024:
+        return build_plan<1, 1>(g2, width, L, plan, work, s);
+    case 2048: return build_plan<1, 2>(g2, width, L, plan, work, s);
+    case 4096: return build_plan<2, 4>(g2, width, L, plan, work, s);
+    case 8192: return build_plan<4, 8>(g2, width, L, plan, work, s);
+    }
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "circular ramp filter: FFT length above 8192");
+}
+
+template <typename Tout>
+static int dispatch_filter(const float* images, int nimg, int ncol,
+                           long long row_stride, long long img_stride,
+                           int pad, int pad_mode, const void* plan, int L,
+                           void* work, Tout* out, const Finish2d& f,
+                           hipStream_t s) {
+#define ATH_F2D(R_, C_)                                                      \
+    return run_filter<R_, C_, Tout>(images, nimg, ncol, row_stride,          \
+                                    img_stride, pad, pad_mode, plan, L, work, \
+                                    out, f, s);
+    switch (L) {
+    case 64: case 128: case 256: case 512: case 1024: ATH_F2D(1, 1)
+    case 2048: ATH_F2D(1, 2)
+    case 4096: ATH_F2D(2, 4)
+    case 8192: ATH_F2D(4, 8)
+    }
+#undef ATH_F2D
+    return set_error(ATH_ERR_UNSUPPORTED,
+                     "circular ramp filter: FFT length above 8192");
+}
+
+int circular_ramp_filter(const float* images, int nimg, int ncol,
+                         long long row_stride, long long img_stride, int pad,
+                         int pad_mode, const void* plan, int L, void* work,
+                         void* out, int out_dtype, long long out_row_stride,
+                         long long out_img_stride, double mask_radius,
+                         int scale_nrow, hipStream_t s) {
+    Finish2d f;
+    f.ncol = ncol;
+    f.c0 = ncol + 3 * pad;  // W + pad
+    f.out_row_stride = out_row_stride;
+    f.out_img_stride = out_img_stride;
+    f.use_mask = mask_radius >= 0.0;
+    f.mask_c = ncol / 2;
+    f.mask_r2 = mask_radius * mask_radius;
+    f.scale = scale_nrow >= 2;
+    f.pi64 = 3.141592653589793;
+    f.denom64 = (double)(scale_nrow - 1);
+    f.pi32 = (float)3.141592653589793;
+    f.denom32 = (float)(scale_nrow - 1);
+    if (out_dtype == ATH_F64)
+        return dispatch_filter<double>(images, nimg, ncol, row_stride,
+                                       img_stride, pad, pad_mode, plan, L,
+                                       work, static_cast<double*>(out), f, s);
+    return dispatch_filter<float>(images, nimg, ncol, row_stride, img_stride,
+                                  pad, pad_mode, plan, L, work,
+                                  static_cast<float*>(out), f, s);
+}
+
+}  // namespace ath

@@ -13,127 +13,12 @@
 // radix-4 Stockham FFT, multiplies by the kernel's half spectrum (the real
 // FFT split and the inverse merge fused in one pass) and runs the inverse.
 // The pass is HBM-bound: one read and one write of every sample.
-#include "ath_internal.h"
+#include "fft_lds.h"
 
 namespace ath {
 namespace {
 
 constexpr int kThreads = 256;
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
-    return make_double2(a.x + b.x, a.y + b.y);
-}
-__device__ __forceinline__ double2 csub(double2 a, double2 b) {
-    return make_double2(a.x - b.x, a.y - b.y);
-}
-__device__ __forceinline__ double2 conjg(double2 a) {
-    return make_double2(a.x, -a.y);
-}
-__device__ __forceinline__ double2 cscale(double2 a, double s) {
-    return make_double2(a.x * s, a.y * s);
-}
-// multiply by -i (forward) or +i (inverse)
-template <int DIR>
-__device__ __forceinline__ double2 rot(double2 a) {
-    return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
-}
-
-// In-LDS complex FFT of length M = 2^m, natural order in and out (Stockham
-// autosort: each pass reads all operands into registers, barriers, then
-// writes them to their sorted place in the same buffer).  DIR = -1 is the
-// forward transform, +1 the unscaled inverse.  tw[n] = exp(-2 pi i n / L),
-// L = 2M.  Each of the 256 threads handles up to MAXB butterflies a pass.
-template <int DIR, int MAXB>
-__device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw) {
-    const int tid = threadIdx.x;
-    const int L = 2 * M;
-    int ns = 1;
-    while (ns * 4 <= M) {
-        const int nb = M >> 2;
-        const int tstep = L / (ns * 4);
-        double2 v[MAXB][4];
-#pragma unroll
-        for (int b = 0; b < MAXB; ++b) {
-            const int j = tid + b * kThreads;
-            if (j < nb) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * nb];
-                const int k = j & (ns - 1);
-                if (k != 0) {
-#pragma unroll
-                    for (int r = 1; r < 4; ++r) {
-                        double2 w = tw[r * k * tstep];
-                        if (DIR > 0) w = conjg(w);
-                        v[b][r] = cmul(v[b][r], w);
-                    }
-                }
-                const double2 a0 = cadd(v[b][0], v[b][2]);
-                const double2 a1 = csub(v[b][0], v[b][2]);
-                const double2 a2 = cadd(v[b][1], v[b][3]);
-                const double2 a3 = rot<DIR>(csub(v[b][1], v[b][3]));
-                v[b][0] = cadd(a0, a2);
-                v[b][1] = cadd(a1, a3);
-                v[b][2] = csub(a0, a2);
-                v[b][3] = csub(a1, a3);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < MAXB; ++b) {
-            const int j = tid + b * kThreads;
-            if (j < nb) {
-                const int k = j & (ns - 1);
-                const int base = (j - k) * 4 + k;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) buf[base + r * ns] = v[b][r];
-            }
-        }
-        __syncthreads();
-        ns *= 4;
-    }
-    if (ns < M) {  // one radix-2 pass, ns = M / 2
-        const int nb = M >> 1;
-        double2 v[2 * MAXB][2];
-#pragma unroll
-        for (int b = 0; b < 2 * MAXB; ++b) {
-            const int j = tid + b * kThreads;
-            if (j < nb) {
-                v[b][0] = buf[j];
-                double2 x1 = buf[j + nb];
-                if (j != 0) {
-                    double2 w = tw[2 * j];
-                    if (DIR > 0) w = conjg(w);
-                    x1 = cmul(x1, w);
-                }
-                v[b][1] = csub(v[b][0], x1);
-                v[b][0] = cadd(v[b][0], x1);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < 2 * MAXB; ++b) {
-            const int j = tid + b * kThreads;
-            if (j < nb) {
-                buf[j] = v[b][0];
-                buf[j + ns] = v[b][1];
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// Z = FFT_M(x[2n] + i x[2n+1]) -> X[k] = rfft_L(x)[k] for k and M-k.
-__device__ __forceinline__ void split_real(double2 zk, double2 zm, double2 wk,
-                                           double2& xk, double2& xm) {
-    const double2 e = cscale(cadd(zk, conjg(zm)), 0.5);
-    const double2 o = rot<-1>(cscale(csub(zk, conjg(zm)), 0.5));
-    const double2 wo = cmul(wk, o);
-    xk = cadd(e, wo);
-    xm = conjg(csub(e, wo));
-}
 
 template <typename Tin>
 __device__ __forceinline__ double load_sample(const Tin* row, int src,
@@ -291,16 +176,6 @@ __global__ __launch_bounds__(kThreads) void ramp_filter_batched_kernel(
         pad_mode, apply_log, tw, G, M);
 }
 
-// tw[n] = exp(-2 pi i n / L)
-__global__ void twiddle_kernel(double2* tw, int L) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < L) {
-        double s, c;
-        sincospi(2.0 * (double)n / (double)L, &s, &c);
-        tw[n] = make_double2(c, -s);
-    }
-}
-
 // spec[r][k] = rfft_L([g_r, g_r, 0...])[k] / M, k = 0..M, one block per row.
 template <int MAXB>
 __global__ __launch_bounds__(kThreads) void kernel_spectrum_kernel(
@@ -329,28 +204,6 @@ __global__ __launch_bounds__(kThreads) void kernel_spectrum_kernel(
         G[k] = cscale(xk, inv);
         G[M - k] = cscale(xm, inv);
     }
-}
-
-int maxb_for(int M) {
-    if (M <= 1024) return 1;
-    if (M <= 2048) return 2;
-    if (M <= 4096) return 4;
-    if (M <= 8192) return 8;
-    return 0;
-}
-
-// Allows dynamic LDS above 64 KB for `kernel`, once per kernel (a host-side
-// attribute, so later launches, including graph-captured ones, enqueue only
-// the kernel).  `bytes` is the largest size the kernel is launched with.
-template <typename K>
-void prepare_lds(K kernel, size_t bytes) {
-    static bool done = false;  // one flag per kernel instantiation
-    if (done || bytes <= 64 * 1024) return;
-    hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(kernel),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) (void)hipGetLastError();
-    done = true;
 }
 
 template <int MAXB, typename Tin, typename Tout>

@@ -433,6 +433,244 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
     return _finish(out, as_numpy)
 
 
+# --------------------------------------------------------------------------
+# BPF: back-projection filtering (reconstruction.py:494-684)
+# --------------------------------------------------------------------------
+def make_circular_ramp_window(width, filter_name=None):
+    """Make a circular ramp window (2d) with the option of adding a smoothing
+    window (reconstruction.py:494-521).
+
+    Parameters
+    ----------
+    width : int
+        Width of the window.
+    filter_name : {None, "hann", "bartlett", "blackman", "hamming",
+                   "nuttall", "parzen", "triang"}
+        Name of a smoothing window used.
+
+    Returns
+    -------
+    array_like
+        Square float64 array, size (width, width): |windowed ramp spectrum|
+        spun around the centre.
+    """
+    from algotom_amd.util import utility
+    ramp_1d = np.abs(np.fft.fftshift(np.fft.fft(_ramp_kernel(width))))
+    ramp_1d = ramp_1d * make_smoothing_window(filter_name, width)
+    return utility.transform_1d_window_to_2d(ramp_1d, order=1,
+                                             mode="nearest")
+
+
+_circ_cache = {}
+
+
+def _circular_plan(ramp_win, dev):
+    """Host kernel of a W x W circular window, with the device plans built
+    from it (one per FFT length): the filter is the circular convolution
+    with g2 = Re(ifft2(ifftshift(R2))) (reconstruction.py:555-559),
+    evaluated by ath_circular_ramp_filter.  One window is kept resident
+    (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192)."""
+    import hashlib
+    win = np.ascontiguousarray(ramp_win, dtype=np.float64)
+    width = win.shape[0]
+    key = (dev.index, width, hashlib.sha1(win.tobytes()).hexdigest())
+    hit = _circ_cache.get(key)
+    if hit is not None:
+        return hit
+    g2 = np.ascontiguousarray(np.real(np.fft.ifft2(np.fft.ifftshift(win))))
+    entry = {"width": width, "g2": g2, "plans": {}}
+    _circ_cache.clear()
+    _circ_cache[key] = entry
+    return entry
+
+
+def _circular_plan_for(entry, fft_len, dev):
+    import torch
+    plan = entry["plans"].get(fft_len)
+    if plan is not None:
+        return plan
+    lib = nat.load()
+    width = entry["width"]
+    nbytes = lib.ath_circular_filter_plan_bytes(fft_len)
+    nwork = lib.ath_circular_filter_plan_work_bytes(width, fft_len)
+    if nbytes == 0 or nwork == 0:
+        raise NotImplementedError(
+            "the circular ramp filter supports FFT lengths up to 8192 "
+            "(width %d needs %d)" % (width, fft_len))
+    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    work = torch.empty(nwork, dtype=torch.uint8, device=dev)
+    g2_d = torch.from_numpy(entry["g2"]).to(dev)
+    nat.check(lib.ath_circular_filter_plan_build(
+        nat.ptr(g2_d), width, fft_len, nat.ptr(plan), nbytes, nat.ptr(work),
+        nwork, nat.stream_ptr(dev)), "ath_circular_filter_plan_build")
+    entry["plans"][fft_len] = plan
+    return plan
+
+
+_CIRC_WORK_BYTES = 4 << 30
+
+
+def _circular_filter(images, ncol, nimg, row_stride, img_stride, pad,
+                     pad_mode, ramp_win, out, out_row_stride, out_img_stride,
+                     mask_radius=-1.0, scale_nrow=0):
+    """Run ath_circular_ramp_filter over nimg images of a device tensor,
+    in groups that keep the workspace below 4 GiB."""
+    import torch
+    if pad_mode not in nat.PAD_MODES:
+        raise NotImplementedError(
+            "pad_mode %r is not supported by the GPU filter (supported: %s)"
+            % (pad_mode, ", ".join(nat.PAD_MODES)))
+    dev = images.device
+    lib = nat.load()
+    fft_len = nat.check(lib.ath_filter_fft_length(ncol, pad),
+                        "ath_filter_fft_length")
+    entry = _circular_plan(ramp_win, dev)
+    plan = _circular_plan_for(entry, fft_len, dev)
+    per_img = lib.ath_circular_ramp_filter_workspace_bytes(ncol, pad,
+                                                           fft_len, 1)
+    group = int(max(1, min(nimg, _CIRC_WORK_BYTES // max(per_img, 1))))
+    work = torch.empty(per_img * group, dtype=torch.uint8, device=dev)
+    esz_in = images.element_size()
+    esz_out = out.element_size()
+    for j0 in range(0, nimg, group):
+        n = min(group, nimg - j0)
+        nat.check(lib.ath_circular_ramp_filter(
+            images.data_ptr() + j0 * img_stride * esz_in, n, ncol,
+            row_stride, img_stride, int(pad), nat.PAD_MODES[pad_mode],
+            nat.ptr(plan), fft_len, nat.ptr(work), per_img * n,
+            out.data_ptr() + j0 * out_img_stride * esz_out,
+            nat.dtype_code(out.dtype), out_row_stride, out_img_stride,
+            float(mask_radius), int(scale_nrow), nat.stream_ptr(dev)),
+            "ath_circular_ramp_filter")
+    return out
+
+
+def apply_circular_ramp_filter(rec_img, ramp_win=None, filter_name=None,
+                               pad=None, pad_mode="edge"):
+    """Apply the circular ramp filter to a back-projected image
+    (reconstruction.py:524-560).
+
+    Parameters
+    ----------
+    rec_img : array_like
+        Square array. back-projected image (numpy or a GPU tensor).
+    ramp_win : array_like
+        2d circular ramp window, generated if None given or if its shape is
+        not the padded image's.
+    filter_name : {None, "hann", "bartlett", "blackman", "hamming",
+                   "nuttall", "parzen", "triang"}
+        Name of a smoothing window used.
+    pad : int or None
+        Padding before the FFT; min(int(0.15 * width), 150) if None.
+    pad_mode : str
+        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap".
+
+    Returns
+    -------
+    array_like
+        Square float64 array.
+    """
+    import torch
+    ncol = rec_img.shape[0]
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    width = ncol + 2 * pad
+    if ramp_win is None or np.shape(ramp_win) != (width, width):
+        ramp_win = make_circular_ramp_window(width, filter_name)
+    dev = nat.device()
+    as_numpy = not _is_tensor(rec_img)
+    img = _to_device(rec_img, dev, allow_f64=False)
+    out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
+    _circular_filter(img, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
+                     ramp_win, out, ncol, ncol * ncol)
+    return _finish(out, as_numpy)
+
+
+def _minus_log(sino_t):
+    import torch
+    out = torch.empty_like(sino_t)
+    nat.check(nat.load().ath_minus_log(nat.ptr(sino_t), nat.ptr(out),
+                                       sino_t.numel(),
+                                       nat.stream_ptr(sino_t.device)),
+              "ath_minus_log")
+    return out
+
+
+def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
+                       ramp_win=None, filter_name="hann", pad=None,
+                       pad_mode="edge", apply_log=True, gpu=True,
+                       block=(16, 16), ncore=None):
+    """Apply the BPF (back-projection filtering) reconstruction method to a
+    sinogram-image or a chunk of sinogram-images (reconstruction.py:563-684).
+    Angular axis is 0; for 3D input the slicing axis is 1.
+
+    The sinogram (logged if apply_log) is back-projected with edge padding,
+    then every slice goes through the 2D circular ramp filter.  ``gpu``
+    selects the reference back-projector's arithmetic as in
+    fbp_reconstruction; ``block`` and ``ncore`` are ignored.
+
+    Returns
+    -------
+    array_like
+        (ncol, ncol) float64 for one sinogram (the reference's 2D branch,
+        also for a (nrow, 1, ncol) stack, returned as (ncol, 1, ncol)), or
+        (ncol, nslice, ncol) float32 for a stack.
+    """
+    import torch
+    shape = tuple(sinogram.shape)
+    input_3d = len(shape) == 3
+    if input_3d:
+        (nrow, num_sino, ncol) = shape
+    else:
+        num_sino = 1
+        (nrow, ncol) = shape
+    if center < 0 or center >= ncol:
+        raise ValueError("Center is out of the range [0, {}]".format(ncol - 1))
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    elif len(angles) != nrow:
+        raise ValueError("!!!Number of angles is not the same as the row "
+                         "number of the sinogram!!!")
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinogram)
+    if apply_log:
+        _clamp_nonpositive(sinogram)
+    mask_radius = -1.0
+    if ratio is not None:
+        if ratio == 0.0:
+            ratio = min(center, ncol - center) / (0.5 * ncol)
+        mask_radius = float(ratio * (ncol // 2))
+    if pad is None:
+        pad = min(int(0.15 * ncol), 150)
+    width = ncol + 2 * pad
+    if ramp_win is None:
+        ramp_win = make_circular_ramp_window(width, filter_name)
+    elif np.shape(ramp_win) != (width, width):
+        # apply_circular_ramp_filter regenerates the window; the 2D branch
+        # calls it without filter_name (:662), the 3D branch with it (:675)
+        ramp_win = make_circular_ramp_window(
+            width, filter_name if num_sino > 1 else None)
+    sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
+        nrow, num_sino, ncol)
+    if apply_log:
+        sino_t = _minus_log(sino_t)
+    acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ
+    bp = _backproject_stack(sino_t, angles, center, acc, edge_pad=True)
+    if num_sino == 1:
+        out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
+        _circular_filter(bp, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
+                         ramp_win, out, ncol, ncol * ncol, mask_radius, nrow)
+        if input_3d:
+            out = out.reshape(ncol, 1, ncol)
+    else:
+        out = torch.empty((ncol, num_sino, ncol), dtype=torch.float32,
+                          device=dev)
+        _circular_filter(bp, ncol, num_sino, num_sino * ncol, ncol, pad,
+                         pad_mode, ramp_win, out, num_sino * ncol, ncol,
+                         mask_radius, nrow)
+    return _finish(out, as_numpy)
+
+
 # Centre search by slice metrics (reconstruction.py:1254-1458), implemented
 # in algotom_amd.rec.center and reachable here, where the reference has it.
 def _find_center_based_slice_metric(*args, **kwargs):

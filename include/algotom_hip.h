@@ -239,6 +239,57 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                                     int side, int norm, int width,
                                     float* factors, float* out, void* stream);
 
+/* out = -log(in) for n float32 values (in == out allowed), the log taken
+ * in float64 and rounded.  Replaces the apply-log step of
+ * bpf_reconstruction (reconstruction.py:634-639), which back-projects the
+ * logged sinogram before any filtering.  Values <= 0 must have been
+ * replaced by the caller (the reference sets them to 1 with a warning). */
+int ath_minus_log(const float* in, float* out, long long n, void* stream);
+
+/* ---- BPF: 2D circular ramp filter (reconstruction.py:494-560) ---------- */
+
+/* Bytes of a circular-filter plan for FFT length fft_len (twiddles + the
+ * 2D spectrum of the tiled kernel, float64); 0 if fft_len is not a power
+ * of two in [64, 8192].  fft_len comes from ath_filter_fft_length(ncol,
+ * pad), the same rule as the 1D filter. */
+size_t ath_circular_filter_plan_bytes(int fft_len);
+
+/* Device scratch bytes ath_circular_filter_plan_build needs. */
+size_t ath_circular_filter_plan_work_bytes(int width, int fft_len);
+
+/* Builds the plan of the W x W real kernel g2 (device, row-major float64),
+ * W = ncol + 2*pad: g2 = Re(ifft2(ifftshift(R2))) of the circular ramp
+ * window R2 (make_circular_ramp_window, reconstruction.py:494-521), so that
+ * the filter equals apply_circular_ramp_filter's spectral product
+ * (:555-559). */
+int ath_circular_filter_plan_build(const double* kernel2d, int width,
+                                   int fft_len, void* plan, size_t plan_bytes,
+                                   void* work, size_t work_bytes,
+                                   void* stream);
+
+/* Device workspace bytes of ath_circular_ramp_filter for nimg images. */
+size_t ath_circular_ramp_filter_workspace_bytes(int ncol, int pad,
+                                                int fft_len, int nimg);
+
+/* apply_circular_ramp_filter (reconstruction.py:524-560) of nimg square
+ * float32 images (ncol x ncol; element (j, y, x) at
+ * images[j*img_stride + y*row_stride + x]): numpy.pad by `pad` in
+ * `pad_mode` on both axes, 2D circular convolution with the plan's kernel,
+ * crop; float64 arithmetic.  Then, as bpf_reconstruction does
+ * (:663-684): the circle mask of radius mask_radius (< 0: none) and, if
+ * scale_nrow >= 2, the factor pi/(scale_nrow - 1) -- in float64 for
+ * out_dtype ATH_F64 (the 2D branch), after rounding to float32 for ATH_F32
+ * (the 3D branch).  Output element (j, y, x) at
+ * out[j*out_img_stride + y*out_row_stride + x]. */
+int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
+                             long long row_stride, long long img_stride,
+                             int pad, int pad_mode, const void* plan,
+                             int fft_len, void* workspace,
+                             size_t workspace_bytes, void* out, int out_dtype,
+                             long long out_row_stride,
+                             long long out_img_stride, double mask_radius,
+                             int scale_nrow, void* stream);
+
 /* Gaussian smoothing along rows (nrows x ncol float32), as
  * scipy.ndimage.gaussian_filter1d(x, sigma, axis=-1) with mode "reflect":
  * weights (2*radius+1 float64, device) are the normalised Gaussian the
