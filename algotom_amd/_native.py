@@ -20,7 +20,8 @@ ATH_ERR_UNSUPPORTED = -3
 ATH_ERR_NO_DEVICE = -4
 ATH_ERR_WORKSPACE = -5
 
-ATH_F32, ATH_F64, ATH_F16, ATH_BF16 = 0, 1, 2, 3
+ATH_F32, ATH_F64, ATH_F16, ATH_BF16, ATH_U16 = 0, 1, 2, 3, 4
+VERT_GPU, VERT_CPU = 0, 1
 PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
              "wrap": 4}
 ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV = 0, 1, 2, 3
@@ -58,6 +59,12 @@ SIGNATURES = (
     ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _I, _I, _I, _I,
                                              _P, _P, _P]),
     ("ath_minus_log", _I, [_P, _P, ctypes.c_longlong, _P]),
+    ("ath_flat_field_log", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong,
+                                ctypes.c_longlong, _P, _P, _I, _P, _P, _P]),
+    ("ath_vertical_back_projection", _I, [_P, _I, _I, _I, _P, _P, _P, _I, _D,
+                                          _I, _I, _I, _I, _P, _P]),
+    ("ath_vertical_finish", _I, [_P, _I, ctypes.c_longlong, _I, _I, _I, _I,
+                                 _P]),
     ("ath_circular_filter_plan_bytes", _S, [_I]),
     ("ath_circular_filter_plan_work_bytes", _S, [_I, _I]),
     ("ath_circular_filter_plan_build", _I, [_P, _I, _I, _P, _S, _P, _S, _P]),

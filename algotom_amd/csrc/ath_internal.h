@@ -113,6 +113,19 @@ int circular_ramp_filter(const float* images, int nimg, int ncol,
                          void* out, int out_dtype, long long out_row_stride,
                          long long out_img_stride, double mask_radius,
                          int scale_nrow, hipStream_t s);
+constexpr int kVertGpu = ATH_VERT_GPU;
+constexpr int kVertCpu = ATH_VERT_CPU;
+int vertical_back_projection(const float* proj, int nproj, int height,
+                             int width, const double* angles,
+                             const double* xs, const double* ys, int nslice,
+                             double center, int edge_pad, int mode, int chunk,
+                             int accumulate, float* out, hipStream_t s);
+int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
+                   int width, long long proj_stride, long long row_stride,
+                   const float* dark, const float* flat_dark, int apply_log,
+                   float* out, int* flag, hipStream_t s);
+int vertical_finish(void* data, int dtype, long long total, int width, int lo,
+                    int hi, int nrow, hipStream_t s);
 int minus_log(const float* in, float* out, long long n, hipStream_t s);
 int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s);

@@ -373,6 +373,65 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                              static_cast<hipStream_t>(stream));
 }
 
+int ath_flat_field_log(const void* proj, int in_dtype, int nproj, int height,
+                       int width, long long proj_stride, long long row_stride,
+                       const float* dark, const float* flat_dark,
+                       int apply_log, float* out, int* flag, void* stream) {
+    clear_error();
+    if (!proj || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if ((dark == nullptr) != (flat_dark == nullptr))
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "dark and flat_dark must both be given or both NULL");
+    if (apply_log && !flag)
+        return set_error(ATH_ERR_INVALID_ARG, "apply_log needs a flag");
+    if (in_dtype != ATH_F32 && in_dtype != ATH_U16)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "projections must be float32 or uint16");
+    if (nproj < 1 || height < 1 || width < 1 || row_stride < width ||
+        proj_stride < (long long)(height - 1) * row_stride + width)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    return flat_field_log(proj, in_dtype, nproj, height, width, proj_stride,
+                          row_stride, dark, flat_dark, apply_log, out, flag,
+                          static_cast<hipStream_t>(stream));
+}
+
+int ath_vertical_back_projection(const float* proj, int nproj, int height,
+                                 int width, const double* angles,
+                                 const double* xs, const double* ys,
+                                 int nslice, double center, int edge_pad,
+                                 int mode, int chunk, int accumulate,
+                                 float* out, void* stream) {
+    clear_error();
+    if (!proj || !angles || !xs || !ys || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nproj < 1 || height < 1 || width < 1 || nslice < 1 || chunk < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (nslice > 65535 || (height + 31) / 32 > 65535)
+        return set_error(ATH_ERR_UNSUPPORTED, "too many slices or rows");
+    if (mode != ATH_VERT_GPU && mode != ATH_VERT_CPU)
+        return set_error(ATH_ERR_INVALID_ARG, "bad vertical mode");
+    if (!std::isfinite(center))
+        return set_error(ATH_ERR_INVALID_ARG, "center must be finite");
+    return vertical_back_projection(proj, nproj, height, width, angles, xs, ys,
+                                    nslice, center, edge_pad, mode, chunk,
+                                    accumulate, out,
+                                    static_cast<hipStream_t>(stream));
+}
+
+int ath_vertical_finish(void* data, int dtype, long long total, int width,
+                        int mask_lo, int mask_hi, int nrow, void* stream) {
+    clear_error();
+    if (!data) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (dtype != ATH_F32 && dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG, "float32 or float64 data only");
+    if (total < 0 || width < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (total == 0) return ATH_OK;
+    return vertical_finish(data, dtype, total, width, mask_lo, mask_hi, nrow,
+                           static_cast<hipStream_t>(stream));
+}
+
 int ath_minus_log(const float* in, float* out, long long n, void* stream) {
     clear_error();
     if (!in || !out) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");

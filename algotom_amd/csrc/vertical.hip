@@ -1,0 +1,272 @@
+// Vertical-slice reconstruction on gfx950 (algotom/rec/vertrec.py).
+//
+// A vertical slice is the set of points (x_j, y_j), j < width, on a line
+// through the sample, reconstructed for every detector row i: recon[i][j] =
+// sum over projections n of the projection row proj[n][i] interpolated at
+// f = x_j cos(theta_n) + y_j sin(theta_n) + center.  The position is shared
+// by all rows, so a workgroup takes 64 points (one per lane) x 32 rows and
+// every lane computes f once per projection for its 8 rows; the two taps of
+// a row are gathered straight from the projection row (consecutive lanes
+// read consecutive columns, so the loads coalesce).
+//
+// The reference sums projections in chunks (vertrec.py:584-620): every
+// chunk of `chunk` projections is back-projected into a fresh float32 image
+// that is then added to the running float32 slice.  Two typings are
+// reproduced (numba's, from the kernels' argument types):
+//   kVertGpu  __vertical_back_projection_gpu(_chunk)_kernel (:176-213,
+//             :256-293): float32 angles, points, centre and trig, float32
+//             position r = x c + y s, f = r + center; float64 chunk sum.
+//   kVertCpu  vertical_back_projection_cpu(_chunk) (:53-110, :113-173):
+//             float64 angles, trig and position (points as given); the
+//             chunk image is float32, stored after every projection.
+// In both, val = yd + (yu - yd) (f - d) with yu - yd in float32 and the
+// rest in float64; outside [0, width-1] the edge value is added only with
+// edge_pad.
+#include "ath_internal.h"
+
+namespace ath {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPts = 64;       // points per workgroup, one per lane
+constexpr int kRows = 8;       // rows per lane; 4 waves -> 32 rows
+constexpr int kAngBlock = 64;  // projections whose trig is staged at once
+
+struct VertArgs {
+    const float* proj;    // (nproj, height, width)
+    const double* angles; // nproj radians
+    const double* xs;     // (nslice, width)
+    const double* ys;
+    float* out;           // (nslice, height, width)
+    int nproj, height, width, nslice;
+    double center;
+    int edge_pad, chunk, accumulate;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a) {
+    __shared__ double s_c[kAngBlock], s_s[kAngBlock];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int j = blockIdx.x * kPts + lane;
+    const int i0 = blockIdx.y * (4 * kRows) + wave;
+    const int sl = blockIdx.z;
+    const bool valid_j = j < a.width;
+    const int width1 = a.width - 1;
+    double x = 0.0, y = 0.0;
+    if (valid_j) {
+        x = a.xs[(size_t)sl * a.width + j];
+        y = a.ys[(size_t)sl * a.width + j];
+    }
+    const float x32 = (float)x, y32 = (float)y, cen32 = (float)a.center;
+    double chunk64[kRows];
+    float chunk32[kRows], total[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        chunk64[k] = 0.0;
+        chunk32[k] = 0.0f;
+        total[k] = 0.0f;
+    }
+    const size_t plane = (size_t)a.height * a.width;
+    for (int n0 = 0; n0 < a.nproj; n0 += kAngBlock) {
+        __syncthreads();
+        if (tid < kAngBlock && n0 + tid < a.nproj) {
+            const double th = a.angles[n0 + tid];
+            if (MODE == kVertGpu) {
+                const double t = (double)(float)th;
+                s_c[tid] = (double)(float)cos(t);
+                s_s[tid] = (double)(float)sin(t);
+            } else {
+                s_c[tid] = cos(th);
+                s_s[tid] = sin(th);
+            }
+        }
+        __syncthreads();
+        const int nb = min(kAngBlock, a.nproj - n0);
+        for (int q = 0; q < nb; ++q) {
+            const int n = n0 + q;
+            double fpos, frac;
+            int d, u;
+            bool in_range;
+            if (MODE == kVertGpu) {
+                const float c = (float)s_c[q], s = (float)s_s[q];
+                const float p1 = x32 * c, p2 = y32 * s;
+                const float f = (p1 + p2) + cen32;
+                in_range = (f >= 0.0f) && ((double)f <= (double)width1);
+                d = (int)floorf(f);
+                u = (int)ceilf(f);
+                fpos = (double)f;
+                frac = (double)f - (double)d;
+            } else {
+                const double f = (x * s_c[q] + y * s_s[q]) + a.center;
+                in_range = (f >= 0.0) && (f <= (double)width1);
+                d = (int)floor(f);
+                u = (int)ceil(f);
+                fpos = f;
+                frac = f - (double)d;
+            }
+            const float* pn = a.proj + (size_t)n * plane;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                const int i = i0 + 4 * k;
+                if (!valid_j || i >= a.height) continue;
+                const float* row = pn + (size_t)i * a.width;
+                double val;
+                if (in_range) {
+                    const float yd = row[d];
+                    if (u != d) {
+                        const float yu = row[u];
+                        val = (double)yd + (double)(yu - yd) * frac;
+                    } else {
+                        val = (double)yd;
+                    }
+                } else if (a.edge_pad) {
+                    val = (double)(fpos < 0.0 ? row[0] : row[width1]);
+                } else {
+                    continue;
+                }
+                if (MODE == kVertGpu)
+                    chunk64[k] += val;
+                else
+                    chunk32[k] = (float)((double)chunk32[k] + val);
+            }
+            if ((n + 1) % a.chunk == 0 || n + 1 == a.nproj) {
+#pragma unroll
+                for (int k = 0; k < kRows; ++k) {
+                    if (MODE == kVertGpu) {
+                        total[k] = total[k] + (float)chunk64[k];
+                        chunk64[k] = 0.0;
+                    } else {
+                        total[k] = total[k] + chunk32[k];
+                        chunk32[k] = 0.0f;
+                    }
+                }
+            }
+        }
+    }
+    if (!valid_j) return;
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const int i = i0 + 4 * k;
+        if (i >= a.height) continue;
+        float* o = a.out + ((size_t)sl * a.height + i) * a.width + j;
+        *o = a.accumulate ? *o + total[k] : total[k];
+    }
+}
+
+// (img - dark) / flat_dark in float32 (the caller has replaced zeros of
+// flat_dark by 1), then -log with values <= 0 set to 1 and flagged.
+template <typename Tin>
+__global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
+    const Tin* __restrict__ proj, long long nproj, int height, int width,
+    long long proj_stride, long long row_stride,
+    const float* __restrict__ dark, const float* __restrict__ flat_dark,
+    int apply_log, float* __restrict__ out, int* __restrict__ flag) {
+    const long long total = nproj * height * (long long)width;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        const int jx = (int)(e % width);
+        const long long t = e / width;
+        const int iy = (int)(t % height);
+        const long long n = t / height;
+        float v = (float)proj[n * proj_stride + iy * row_stride + jx];
+        if (dark) {
+            const size_t p = (size_t)iy * width + jx;
+            v = (v - dark[p]) / flat_dark[p];
+        }
+        if (apply_log) {
+            if (v <= 0.0f) {
+                v = 1.0f;
+                *flag = 1;
+            }
+            v = (float)(-log((double)v));
+        }
+        out[e] = v;
+    }
+}
+
+// In place: columns outside [lo, hi) zeroed (lo < 0: no mask), then the
+// factor pi / (nrow - 1) (nrow < 2: none), in the array's precision with
+// numpy's order (x * mask, * pi, / (nrow - 1)).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void vertical_finish_kernel(
+    T* __restrict__ data, long long total, int width, int lo, int hi,
+    int nrow) {
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        const int jx = (int)(e % width);
+        T v = data[e];
+        if (lo >= 0) v = v * ((jx >= lo && jx < hi) ? T(1) : T(0));
+        if (nrow >= 2) {
+            if constexpr (sizeof(T) == 8) {
+                v = v * 3.141592653589793;
+                v = v / (double)(nrow - 1);
+            } else {
+                v = v * (float)3.141592653589793;
+                v = v / (float)(nrow - 1);
+            }
+        }
+        data[e] = v;
+    }
+}
+
+long long stream_blocks(long long total) {
+    return std::max<long long>(
+        1, std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20));
+}
+
+}  // namespace
+
+int vertical_back_projection(const float* proj, int nproj, int height,
+                             int width, const double* angles,
+                             const double* xs, const double* ys, int nslice,
+                             double center, int edge_pad, int mode, int chunk,
+                             int accumulate, float* out, hipStream_t s) {
+    VertArgs a{proj, angles, xs, ys, out, nproj, height, width, nslice,
+               center, edge_pad, chunk, accumulate};
+    const dim3 grid((width + kPts - 1) / kPts,
+                    (height + 4 * kRows - 1) / (4 * kRows), nslice);
+    if (mode == kVertGpu)
+        hipLaunchKernelGGL(vertical_bp_kernel<kVertGpu>, grid, dim3(kThreads),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(vertical_bp_kernel<kVertCpu>, grid, dim3(kThreads),
+                           0, s, a);
+    return check_hip(hipGetLastError(), "vertical_bp_kernel launch");
+}
+
+int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
+                   int width, long long proj_stride, long long row_stride,
+                   const float* dark, const float* flat_dark, int apply_log,
+                   float* out, int* flag, hipStream_t s) {
+    const long long total = (long long)nproj * height * width;
+    const dim3 grid((unsigned)stream_blocks(total));
+    if (in_dtype == ATH_U16)
+        hipLaunchKernelGGL(flat_field_log_kernel<unsigned short>, grid,
+                           dim3(kThreads), 0, s,
+                           static_cast<const unsigned short*>(proj),
+                           (long long)nproj, height, width, proj_stride,
+                           row_stride, dark, flat_dark, apply_log, out, flag);
+    else
+        hipLaunchKernelGGL(flat_field_log_kernel<float>, grid, dim3(kThreads),
+                           0, s, static_cast<const float*>(proj),
+                           (long long)nproj, height, width, proj_stride,
+                           row_stride, dark, flat_dark, apply_log, out, flag);
+    return check_hip(hipGetLastError(), "flat_field_log_kernel launch");
+}
+
+int vertical_finish(void* data, int dtype, long long total, int width, int lo,
+                    int hi, int nrow, hipStream_t s) {
+    const dim3 grid((unsigned)stream_blocks(total));
+    if (dtype == ATH_F64)
+        hipLaunchKernelGGL(vertical_finish_kernel<double>, grid,
+                           dim3(kThreads), 0, s, static_cast<double*>(data),
+                           total, width, lo, hi, nrow);
+    else
+        hipLaunchKernelGGL(vertical_finish_kernel<float>, grid, dim3(kThreads),
+                           0, s, static_cast<float*>(data), total, width, lo,
+                           hi, nrow);
+    return check_hip(hipGetLastError(), "vertical_finish_kernel launch");
+}
+
+}  // namespace ath

@@ -101,13 +101,17 @@ def make_2d_ramp_window(height, width, filter_name=None):
     return np.tile(_ramp_row(int(width), filter_name), (height, 1))
 
 
-def _taps(rows):
+def _taps(rows, outer_shift=True):
     """Real circular-convolution taps equivalent to multiplying the shifted
     spectrum by ``rows`` and keeping the real part (reconstruction.py
-    :150-152): Re(ifftshift(ifft(ifftshift(R)))) per row."""
+    :150-152): Re(ifftshift(ifft(ifftshift(R)))) per row.  With
+    outer_shift=False, Re(ifft(ifftshift(R))): the form without the final
+    ifftshift, which vertrec's filter after back-projection uses with the
+    magnitude |R| (vertrec.py:621-627)."""
     rows = np.asarray(rows)
-    g = np.fft.ifftshift(np.fft.ifft(np.fft.ifftshift(rows, axes=-1),
-                                     axis=-1), axes=-1)
+    g = np.fft.ifft(np.fft.ifftshift(rows, axes=-1), axis=-1)
+    if outer_shift:
+        g = np.fft.ifftshift(g, axes=-1)
     return np.ascontiguousarray(np.real(g), dtype=np.float64)
 
 

@@ -35,6 +35,7 @@ extern "C" {
 #define ATH_F64 1
 #define ATH_F16 2
 #define ATH_BF16 3
+#define ATH_U16 4   /* raw projections (ath_flat_field_log input only) */
 
 /* numpy.pad modes applied inside the filter kernel */
 #define ATH_PAD_EDGE 0
@@ -289,6 +290,49 @@ int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
                              long long out_row_stride,
                              long long out_img_stride, double mask_radius,
                              int scale_nrow, void* stream);
+
+/* ---- vertical slices (algotom/rec/vertrec.py) ------------------------- */
+
+/* typing of the vertical back-projection (which reference kernel) */
+#define ATH_VERT_GPU 0  /* float32 trig/points/position, float64 chunk sums */
+#define ATH_VERT_CPU 1  /* float64 trig/position, float32 chunk images */
+
+/* Flat-field correction and -log of raw projections (vertrec.py:572-583):
+ * out = -log((proj - dark) / flat_dark) for proj[n*proj_stride +
+ * i*row_stride + j] (n < nproj, i < height, j < width; float32 or
+ * ATH_U16), out contiguous (nproj, height, width) float32.  dark /
+ * flat_dark: (height, width) float32 (flat - dark with zeros replaced by
+ * 1, as the reference does), or both NULL for no correction.  With
+ * apply_log, values <= 0 are set to 1 before the log and *flag is set to 1
+ * (the reference's warning); flag may be NULL only without apply_log. */
+int ath_flat_field_log(const void* proj, int in_dtype, int nproj, int height,
+                       int width, long long proj_stride, long long row_stride,
+                       const float* dark, const float* flat_dark,
+                       int apply_log, float* out, int* flag, void* stream);
+
+/* Vertical back-projection of proj (nproj, height, width) float32 onto
+ * nslice lines of width points (xs, ys: (nslice, width) float64, relative
+ * to the image centre): out[s][i][j] = sum over projections of proj[n][i]
+ * interpolated at x_sj cos(theta_n) + y_sj sin(theta_n) + center, summed
+ * in chunks of `chunk` projections as the reference's chunk loop does
+ * (vertrec.py:584-620).  mode ATH_VERT_GPU reproduces
+ * vertical_back_projection_gpu / _gpu_chunk (:176-335), ATH_VERT_CPU
+ * vertical_back_projection_cpu / _cpu_chunk (:53-173).  accumulate: add
+ * to out instead of overwriting (for projection stacks fed in pieces of
+ * whole chunks). */
+int ath_vertical_back_projection(const float* proj, int nproj, int height,
+                                 int width, const double* angles,
+                                 const double* xs, const double* ys,
+                                 int nslice, double center, int edge_pad,
+                                 int mode, int chunk, int accumulate,
+                                 float* out, void* stream);
+
+/* In place on nslice*height*width values (ATH_F32 or ATH_F64): columns
+ * outside [mask_lo, mask_hi) set to zero (mask_lo < 0: no mask), then
+ * times pi / (nrow - 1) (nrow < 2: no scale) -- the masking and scale of
+ * vertical_reconstruction (vertrec.py:686-695). */
+int ath_vertical_finish(void* data, int dtype, long long total, int width,
+                        int mask_lo, int mask_hi, int nrow, void* stream);
 
 /* Gaussian smoothing along rows (nrows x ncol float32), as
  * scipy.ndimage.gaussian_filter1d(x, sigma, axis=-1) with mode "reflect":

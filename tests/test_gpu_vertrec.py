@@ -1,0 +1,199 @@
+"""Vertical-slice reconstruction on the GPU against the oracle
+(oracle/vertrec_np.py).
+
+* the four vertical back-projectors: float32-position (reference GPU
+  kernel) typing equal to the oracle in >= 99.9 % of pixels and within
+  1e-6; float64-position (reference CPU kernel) typing within 1e-10 (the
+  device and host float64 cos/sin may differ in the last bit);
+* vertical_reconstruction / _multiple / _different_angles: every ramp-filter
+  placement, both typings, chunking, crop, projection range, flat/dark
+  fields, log, masking -- within 2e-6 of max;
+* the reference test's thresholds (tests/test_rec/test_vertrec.py:87-497).
+"""
+
+import numpy as np
+import pytest
+
+from tests.vertrec_case import vertrec_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(got, ref, tol, label):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape
+    err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
+    print("%s: max rel %.2e, equal %.5f" % (label, err, np.mean(got == ref)))
+    assert err <= tol
+    return np.mean(got == ref)
+
+
+@pytest.mark.parametrize("width,height,nproj", [(61, 40, 46), (200, 3, 90),
+                                                (130, 17, 181)])
+@pytest.mark.parametrize("edge_pad", [False, True])
+def test_backprojectors_vs_oracle(width, height, nproj, edge_pad):
+    from oracle import vertrec_np
+    from algotom_amd.rec import vertrec
+    rng = np.random.default_rng(width + nproj)
+    proj = (rng.random((nproj, height, width), dtype=np.float32) + 0.5)
+    ang = np.linspace(0.0, np.pi, nproj)
+    cen = (width - 1) / 2 + 1.37
+    x, y = vertrec._get_points_single_line(width // 3, 30.0, width)
+    g = vertrec.vertical_back_projection_gpu(proj, ang, x, y, cen, edge_pad)
+    r = vertrec_np.vertical_backproject(proj, ang, x, y, cen, edge_pad,
+                                        "gpu")[0]
+    assert _compare(g, r, 1e-6, "gpu single") >= 0.999
+    c = vertrec.vertical_back_projection_cpu(proj, ang, x, y, cen, edge_pad)
+    r = vertrec_np.vertical_backproject(proj, ang, x, y, cen, edge_pad,
+                                        "cpu")[0]
+    _compare(c, r, 1e-10, "cpu single")
+    xm, ym = vertrec._get_points_multiple_lines(width // 4, width // 4 + 4,
+                                                120.0, width)
+    g = vertrec.vertical_back_projection_gpu_chunk(proj, ang, xm, ym, cen,
+                                                   edge_pad)
+    r = vertrec_np.vertical_backproject(proj, ang, xm, ym, cen, edge_pad,
+                                        "gpu")
+    assert _compare(g, r, 1e-6, "gpu chunk") >= 0.999
+    c = vertrec.vertical_back_projection_cpu_chunk(proj, ang, xm, ym, cen,
+                                                   edge_pad)
+    r = vertrec_np.vertical_backproject(proj, ang, xm, ym, cen, edge_pad,
+                                        "cpu")
+    _compare(c, r, 1e-10, "cpu chunk")
+
+
+_CASES = [
+    dict(ramp_filter="before", gpu=True),
+    dict(ramp_filter="before", gpu=False, chunk_size=7),
+    dict(ramp_filter="after", gpu=True),
+    dict(ramp_filter="after", gpu=False, masking=True),
+    dict(ramp_filter=None, gpu=True, chunk_size=None),
+    dict(ramp_filter="before", gpu=True, crop=(10, 5, 6, 4), center_off=-10,
+         masking=True),
+    dict(ramp_filter="after", gpu=True, proj_start=3, proj_stop=40,
+         chunk_size=-1),
+    dict(ramp_filter="before", gpu=True, alpha=110.0, pad=20,
+         pad_mode="reflect"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(_CASES)))
+def test_vertical_reconstruction_vs_oracle(case):
+    from oracle import vertrec_np
+    from algotom_amd.rec import vertrec
+    c = vertrec_case()
+    kw = dict(_CASES[case])
+    off = kw.pop("center_off", 0)
+    if "proj_start" in kw:
+        n = kw["proj_stop"] - kw["proj_start"]
+        kw["angles"] = c["angles"][:n]
+    got = vertrec.vertical_reconstruction(
+        c["projs"], 25, c["center"] + off, apply_log=False,
+        show_progress=False, **kw)
+    ref = vertrec_np.vertical_reconstruction(c["projs"], 25, c["center"] + off,
+                                             apply_log=False, **kw)
+    assert got.dtype == ref.dtype
+    _compare(got, ref, 2e-6, "case %d" % case)
+
+
+def test_log_flat_dark_uint16_and_tensor_io():
+    import torch
+    from oracle import vertrec_np
+    from algotom_amd.rec import vertrec
+    rng = np.random.default_rng(11)
+    nproj, height, width = 60, 12, 70
+    raw = rng.integers(200, 4000, (nproj, height, width)).astype(np.uint16)
+    raw[5, 3, 7] = 0     # after dark subtraction <= 0 -> warning, set to 1
+    flat = rng.uniform(3000, 5000, (height, width)).astype(np.float32)
+    dark = np.full((height, width), 100.0, np.float32)
+    with pytest.warns(Warning):
+        got = vertrec.vertical_reconstruction(
+            raw, 30, 35.2, alpha=20.0, flat_field=flat, dark_field=dark,
+            ramp_filter="before", show_progress=False)
+    ref = vertrec_np.vertical_reconstruction(
+        raw, 30, 35.2, alpha=20.0, flat_field=flat, dark_field=dark,
+        ramp_filter="before")
+    _compare(got, ref, 2e-6, "uint16 + flat/dark + log")
+    t = torch.from_numpy(raw.astype(np.float32)).cuda()
+    out = vertrec.vertical_reconstruction_multiple(
+        t, 10, 20, 35.2, step_index=5, apply_log=False, ramp_filter=None,
+        show_progress=False)
+    assert isinstance(out, torch.Tensor) and out.shape == (3, height, width)
+    xm, ym = vertrec._get_points_multiple_lines(10, 20, 0.0, width, 5)
+    ref = vertrec_np.vertical_reconstruction(
+        raw.astype(np.float32), 0, 35.2, apply_log=False, ramp_filter=None,
+        points=(xm, ym))
+    _compare(out.cpu().numpy(), ref, 2e-6, "multiple, tensor in")
+
+
+def test_different_angles_vs_oracle():
+    from oracle import vertrec_np
+    from algotom_amd.rec import vertrec
+    c = vertrec_case()
+    idx, alphas = [20, 30, 40], [0.0, 45.0, 130.0]
+    got = vertrec.vertical_reconstruction_different_angles(
+        c["projs"], idx, alphas, c["center"], apply_log=False,
+        show_progress=False)
+    xm, ym = vertrec._get_points_multiple_lines_different_angles(idx, alphas,
+                                                                 c["width"])
+    ref = vertrec_np.vertical_reconstruction(c["projs"], 0, c["center"],
+                                             apply_log=False, points=(xm, ym))
+    assert got.dtype == np.float32
+    _compare(got, ref, 2e-6, "different angles")
+
+
+def test_reference_thresholds():
+    """tests/test_rec/test_vertrec.py:87-320 on the GPU path."""
+    import scipy.ndimage as ndi
+    from algotom_amd.rec import vertrec as vrec
+    c = vertrec_case()
+    projs, angles, w, h = c["projs"], c["angles"], c["width"], c["height"]
+    s0 = w // 2 - 20
+    x, y = vrec._get_points_single_line(s0, 0.0, w)
+    b1 = vrec.vertical_back_projection_cpu(projs + 1, angles, x, y,
+                                           c["center"], edge_pad=False)
+    b2 = vrec.vertical_back_projection_cpu(projs + 1, angles, x, y,
+                                           c["center"], edge_pad=True)
+    b5 = vrec.vertical_back_projection_gpu(projs + 1, angles, x, y,
+                                           c["center"], edge_pad=False)
+    assert b1.shape == (h, w) and np.min(b1[h // 2]) > 1.0
+    assert np.mean(b2) > np.mean(b1)
+    assert abs(np.mean(b1) - np.mean(b5)) < 0.05
+    f = vrec.vertical_reconstruction
+    idx = w // 2
+    ref = c["phantom"][:, idx, :]
+    kw = dict(chunk_size=10, apply_log=False, show_progress=False)
+    v1 = f(projs, idx, c["center"], ramp_filter="before", gpu=False, **kw)
+    assert np.mean(np.abs(ref - v1)) < 0.05
+    v2 = f(projs, idx, c["center"], ramp_filter="after", gpu=False, **kw)
+    assert np.mean(np.abs(ref - v2)) < 0.3
+    v4 = f(projs, idx, c["center"] - 4, ramp_filter="after", gpu=False,
+           masking=True, **kw)
+    assert np.sum(np.mean(v4, axis=0) == 0.0) == 9
+    v5 = f(projs, idx, c["center"] - 10, crop=(10, 5, 6, 4),
+           ramp_filter="after", gpu=False, masking=True, **kw)
+    assert v5.shape == (h - 15, w - 10)
+    g1 = f(projs, idx, c["center"], ramp_filter="before", gpu=True, **kw)
+    assert np.mean(np.abs(v1 - g1)) < 1e-5
+    g2 = f(projs, idx, c["center"], ramp_filter="after", gpu=True, **kw)
+    assert np.mean(np.abs(v2 - g2)) < 1e-5
+    for alpha, s in ((40.0, 27), (60.0, 24), (110.0, 24)):
+        v = f(projs, s, c["center"], alpha=alpha, angles=angles,
+              ramp_filter="before", apply_log=False, gpu=False,
+              show_progress=False)
+        rot = ndi.rotate(c["phantom"], -alpha, axes=(1, 2), order=1,
+                         reshape=False)
+        assert np.mean(np.abs(rot[:, s, :] - v)) < 0.05
+    m = vrec.vertical_reconstruction_multiple(
+        projs, idx - 10, idx - 5, c["center"], chunk_size=10,
+        ramp_filter="before", apply_log=False, gpu=True,
+        show_progress=False)
+    refs = np.moveaxis(c["phantom"][:, idx - 10:idx - 4, :], 1, 0)
+    assert m.shape == (6, h, w)
+    assert np.mean(np.abs(refs - m)) < 0.05
+    with pytest.raises(ValueError):
+        f(projs, idx, c["center"], ramp_filter="middle")
+    with pytest.raises(ValueError):
+        f(projs, w + 3, c["center"])
+    with pytest.raises(ValueError):
+        f(projs, idx, c["center"], angles=angles[:-1])

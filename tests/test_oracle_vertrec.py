@@ -1,0 +1,123 @@
+"""Vertical-slice oracle (oracle/vertrec_np.py) pinned by the geometry of
+the points, scalar loops, the standard back-projector and the reference
+test's thresholds."""
+
+import numpy as np
+import pytest
+
+from oracle import fbp_np, vertrec_np
+from tests.vertrec_case import vertrec_case
+
+
+@pytest.mark.parametrize("alpha", [0.0, 20.0, 45.0, 60.0, 90.0, 110.0,
+                                   135.0, 170.0, 180.0])
+@pytest.mark.parametrize("index", [0, 13, 30, 60])
+def test_points_geometry(alpha, index):
+    from algotom_amd.rec import vertrec
+    width = 61
+    x, y = vertrec_np.line_points(index, alpha, width)
+    assert len(x) == width
+    step = np.hypot(np.diff(x), np.diff(y))
+    assert np.allclose(step, 1.0, atol=1e-12)
+    mid = np.ceil(0.5 * (width - 1))
+    # distance of the line from the centre, direction of the line
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    dist = abs(x[0] * dy - y[0] * dx)
+    assert abs(dist - abs(index - mid)) < 1e-9
+    a = np.deg2rad(alpha)
+    assert abs(abs(dx * np.sin(a) - dy * np.cos(a))) < 1e-12
+    # the drop-in's host points are the same numbers
+    gx, gy = vertrec._get_points_single_line(index, alpha, width)
+    assert np.allclose(gx, x, rtol=0, atol=1e-12)
+    assert np.allclose(gy, y, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["gpu", "cpu"])
+@pytest.mark.parametrize("edge_pad", [False, True])
+def test_backprojector_vs_loops(mode, edge_pad):
+    rng = np.random.default_rng(3)
+    proj = rng.random((7, 3, 11), dtype=np.float32)
+    ang = np.linspace(0.0, np.pi, 7)
+    x, y = vertrec_np.line_points(7, 30.0, 11)
+    got = vertrec_np.vertical_backproject(proj, ang, x, y, 4.3, edge_pad,
+                                          mode)[0]
+    ref = vertrec_np.vertical_backproject_loops(proj, ang, x, y, 4.3,
+                                                edge_pad, mode)
+    assert np.array_equal(got, ref)
+
+
+def test_chunks_are_float32_sums_of_chunk_images():
+    rng = np.random.default_rng(4)
+    proj = rng.random((10, 2, 9), dtype=np.float32)
+    ang = np.linspace(0.0, np.pi, 10)
+    x, y = vertrec_np.line_points(3, 10.0, 9)
+    for mode in ("gpu", "cpu"):
+        whole = [vertrec_np.vertical_backproject(proj[a:b], ang[a:b], x, y,
+                                                 4.0, True, mode)[0]
+                 for a, b in ((0, 4), (4, 8), (8, 10))]
+        ref = (whole[0] + whole[1]) + whole[2]
+        got = vertrec_np.vertical_backproject(proj, ang, x, y, 4.0, True,
+                                              mode, chunk=4)[0]
+        assert np.array_equal(got, ref)
+
+
+def test_alpha0_slice_is_a_row_of_the_horizontal_slices():
+    rng = np.random.default_rng(5)
+    nproj, height, width = 40, 4, 31   # odd width: both grids share a centre
+    proj = rng.random((nproj, height, width), dtype=np.float32)
+    ang = np.linspace(0.0, np.pi, nproj, endpoint=False)
+    s = 9
+    x, y = vertrec_np.line_points(s, 0.0, width)
+    ver = vertrec_np.vertical_backproject(proj, ang, x, y, 15.2, False,
+                                          "cpu")[0]
+    for i in range(height):
+        bp = fbp_np.backproject_f64(proj[:, i, :].astype(np.float64), ang,
+                                    15.2)
+        assert np.allclose(ver[i], bp[s], rtol=1e-6, atol=1e-6)
+
+
+def test_reference_thresholds_on_oracle():
+    """tests/test_rec/test_vertrec.py:146-290 (CPU branch) on the oracle."""
+    c = vertrec_case()
+    f = vertrec_np.vertical_reconstruction
+    idx = c["width"] // 2
+    ref = c["phantom"][:, idx, :]
+    kw = dict(chunk_size=10, apply_log=False, gpu=False)
+    v1 = f(c["projs"], idx, c["center"], ramp_filter="before", **kw)
+    assert np.mean(np.abs(ref - v1)) < 0.05
+    v2 = f(c["projs"], idx, c["center"], ramp_filter="after", **kw)
+    assert v2.dtype == np.float64
+    assert np.mean(np.abs(ref - v2)) < 0.3
+    v4 = f(c["projs"], idx, c["center"] - 4, ramp_filter="after",
+           masking=True, **kw)
+    assert np.sum(np.mean(v4, axis=0) == 0.0) == 9
+    v7 = f(c["projs_360"], idx, c["center"], angles=c["angles_360"],
+           chunk_size=None, ramp_filter="before", apply_log=False, gpu=False)
+    assert np.mean(np.abs(ref - v7)) < 0.05
+    import scipy.ndimage as ndi
+    for alpha, s in ((40.0, 27), (60.0, 24), (110.0, 24)):
+        v = f(c["projs"], s, c["center"], alpha=alpha, angles=c["angles"],
+              ramp_filter="before", apply_log=False, gpu=False)
+        rot = ndi.rotate(c["phantom"], -alpha, axes=(1, 2), order=1,
+                         reshape=False)
+        assert np.mean(np.abs(rot[:, s, :] - v)) < 0.05
+    v13 = f(c["projs"], 30, c["center"], alpha=90.0, angles=c["angles"],
+            ramp_filter="before", apply_log=False, gpu=False)
+    assert np.mean(np.abs(np.fliplr(c["phantom"][:, :, 30]) - v13)) < 0.05
+
+
+@pytest.mark.parametrize("width,pad", [(61, 9), (40, 6), (128, 19)])
+def test_after_filter_taps_drive_the_gpu_filter(width, pad):
+    """The GPU filter (kernel model) with Re(ifft(ifftshift(|R|))) as taps
+    equals the literal after-filter of the oracle."""
+    from tests import kernel_model as km
+    from algotom_amd.rec import reconstruction as rec
+    rng = np.random.default_rng(width)
+    ver = rng.standard_normal((3, width))
+    rrow = np.abs(rec._ramp_row(width + 2 * pad, "hann"))
+    g = rec._taps(rrow, outer_shift=False)
+    L = km.fft_length(width, pad)
+    got = np.stack([km.ramp_filter_row(np.pad(v, pad, mode="edge"), g, pad, L)
+                    for v in ver])
+    ref = vertrec_np.after_filter(ver, rrow, pad, "edge")
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
