@@ -458,26 +458,37 @@ def make_circular_ramp_window(width, filter_name=None):
         Square float64 array, size (width, width): |windowed ramp spectrum|
         spun around the centre.
     """
+    return np.array(_circular_window(int(width), filter_name))
+
+
+@functools.lru_cache(maxsize=4)
+def _circular_window(width, filter_name):
     from algotom_amd.util import utility
     ramp_1d = np.abs(np.fft.fftshift(np.fft.fft(_ramp_kernel(width))))
     ramp_1d = ramp_1d * make_smoothing_window(filter_name, width)
-    return utility.transform_1d_window_to_2d(ramp_1d, order=1,
-                                             mode="nearest")
+    win = utility.transform_1d_window_to_2d(ramp_1d, order=1, mode="nearest")
+    win.setflags(write=False)
+    return win
 
 
 _circ_cache = {}
 
 
-def _circular_plan(ramp_win, dev):
+def _circular_plan(ramp_win, dev, name_key=None):
     """Host kernel of a W x W circular window, with the device plans built
     from it (one per FFT length): the filter is the circular convolution
     with g2 = Re(ifft2(ifftshift(R2))) (reconstruction.py:555-559),
     evaluated by ath_circular_ramp_filter.  One window is kept resident
-    (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192)."""
+    (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192).
+    Default windows are keyed by (width, filter_name); a caller's window by
+    a hash of its values."""
     import hashlib
     win = np.ascontiguousarray(ramp_win, dtype=np.float64)
     width = win.shape[0]
-    key = (dev.index, width, hashlib.sha1(win.tobytes()).hexdigest())
+    if name_key is not None:
+        key = (dev.index, width) + tuple(name_key)
+    else:
+        key = (dev.index, width, hashlib.sha1(win.tobytes()).hexdigest())
     hit = _circ_cache.get(key)
     if hit is not None:
         return hit
@@ -516,7 +527,7 @@ _CIRC_WORK_BYTES = 4 << 30
 
 def _circular_filter(images, ncol, nimg, row_stride, img_stride, pad,
                      pad_mode, ramp_win, out, out_row_stride, out_img_stride,
-                     mask_radius=-1.0, scale_nrow=0):
+                     mask_radius=-1.0, scale_nrow=0, name_key=None):
     """Run ath_circular_ramp_filter over nimg images of a device tensor,
     in groups that keep the workspace below 4 GiB."""
     import torch
@@ -528,7 +539,7 @@ def _circular_filter(images, ncol, nimg, row_stride, img_stride, pad,
     lib = nat.load()
     fft_len = nat.check(lib.ath_filter_fft_length(ncol, pad),
                         "ath_filter_fft_length")
-    entry = _circular_plan(ramp_win, dev)
+    entry = _circular_plan(ramp_win, dev, name_key)
     plan = _circular_plan_for(entry, fft_len, dev)
     per_img = lib.ath_circular_ramp_filter_workspace_bytes(ncol, pad,
                                                            fft_len, 1)
@@ -579,14 +590,16 @@ def apply_circular_ramp_filter(rec_img, ramp_win=None, filter_name=None,
     if pad is None:
         pad = min(int(0.15 * ncol), 150)
     width = ncol + 2 * pad
+    name_key = None
     if ramp_win is None or np.shape(ramp_win) != (width, width):
-        ramp_win = make_circular_ramp_window(width, filter_name)
+        ramp_win = _circular_window(width, filter_name)
+        name_key = ("window", filter_name)
     dev = nat.device()
     as_numpy = not _is_tensor(rec_img)
     img = _to_device(rec_img, dev, allow_f64=False)
     out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
     _circular_filter(img, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
-                     ramp_win, out, ncol, ncol * ncol)
+                     ramp_win, out, ncol, ncol * ncol, name_key=name_key)
     return _finish(out, as_numpy)
 
 
@@ -647,13 +660,16 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
     if pad is None:
         pad = min(int(0.15 * ncol), 150)
     width = ncol + 2 * pad
+    name_key = None
     if ramp_win is None:
-        ramp_win = make_circular_ramp_window(width, filter_name)
+        ramp_win = _circular_window(width, filter_name)
+        name_key = ("window", filter_name)
     elif np.shape(ramp_win) != (width, width):
         # apply_circular_ramp_filter regenerates the window; the 2D branch
         # calls it without filter_name (:662), the 3D branch with it (:675)
-        ramp_win = make_circular_ramp_window(
-            width, filter_name if num_sino > 1 else None)
+        regen = filter_name if num_sino > 1 else None
+        ramp_win = _circular_window(width, regen)
+        name_key = ("window", regen)
     sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
         nrow, num_sino, ncol)
     if apply_log:
@@ -663,7 +679,8 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
     if num_sino == 1:
         out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
         _circular_filter(bp, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
-                         ramp_win, out, ncol, ncol * ncol, mask_radius, nrow)
+                         ramp_win, out, ncol, ncol * ncol, mask_radius, nrow,
+                         name_key)
         if input_3d:
             out = out.reshape(ncol, 1, ncol)
     else:
@@ -671,7 +688,7 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
                           device=dev)
         _circular_filter(bp, ncol, num_sino, num_sino * ncol, ncol, pad,
                          pad_mode, ramp_win, out, num_sino * ncol, ncol,
-                         mask_radius, nrow)
+                         mask_radius, nrow, name_key)
     return _finish(out, as_numpy)
 
 

@@ -46,6 +46,18 @@ metric="entropy") -- a coarse integer pass over 41 centres plus the reference
 slice, then a sub-pixel pass; every pass smooths, logs and filters once and
 back-projects all its centres in one batched launch.  value = candidate
 slices reconstructed and scored per second.
+
+--workload bpf: BPF (back-projection filtering) of a 16 x 1801 x 2560
+transmission stack (seed 1000): log, edge-padded back-projection, the 2D
+circular ramp filter (FFT length 8192) of every slice, mask and scale;
+value = GUPS of the whole step (U = 16 * 2560^2 * 1801).
+
+--workload vertrec: vertical slices straight from a 1801 x 1024 x 2560
+projection stack (uniform random data on the device, seed 0): one
+vertical_reconstruction (ramp filter after, reference GPU typing, chunks of
+30 projections) and one vertical_reconstruction_multiple of 16 parallel
+slices (ramp filter before); value = GUPS of the 16-slice call
+(U = 16 * 1024 * 2560 * 1801).
 """
 
 import argparse
@@ -387,6 +399,82 @@ def run_center(args, dev):
     print(json.dumps(line))
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, out
+
+
+def run_bpf(args, dev):
+    import torch
+    from algotom_amd.rec import reconstruction as rec
+    nsl = 16
+    sino, angles = make_stack(dev, nslice=nsl)
+    updates = nsl * NCOL * NCOL * NROW
+    ms, out = _timed(lambda: rec.bpf_reconstruction(sino, CENTER, angles),
+                     args.steps, args.warmup)
+    one = sino[:, 0, :].contiguous()
+    ms2d, _ = _timed(lambda: rec.bpf_reconstruction(one, CENTER, angles),
+                     args.steps, args.warmup)
+    line = {
+        "metric": METRIC, "value": updates / (ms * 1e-3) / 1e9,
+        "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (f64 filter)",
+        "data": "synthetic",
+        "config": {"workload": "bpf_stack_16x1801x2560", "nslice": nsl,
+                   "nrow": NROW, "ncol": NCOL, "center": CENTER,
+                   "filter_fft_length": 8192, "parallelism": "single-gpu"},
+        "cpu_baseline": None,
+        "detail": {"ms_per_slice": ms / nsl,
+                   "single_sinogram_ms (float64 output)": ms2d,
+                   "finite": bool(torch.isfinite(out).all().item())},
+    }
+    print(json.dumps(line))
+
+
+def run_vertrec(args, dev):
+    import torch
+    from algotom_amd.rec import vertrec
+    nproj, height, width = NROW, 1024, NCOL
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    proj = torch.rand((nproj, height, width), generator=gen, device=dev)
+    proj = proj * 0.5 + 0.25
+    cen = CENTER
+    ms1, _ = _timed(lambda: vertrec.vertical_reconstruction(
+        proj, width // 2 - 100, cen, alpha=30.0, apply_log=True,
+        ramp_filter="after", show_progress=False), args.steps, args.warmup)
+    nsl = 16
+    ms16, out = _timed(lambda: vertrec.vertical_reconstruction_multiple(
+        proj, 1000, 1000 + 5 * (nsl - 1), cen, alpha=30.0, step_index=5,
+        apply_log=True, ramp_filter="before", show_progress=False),
+        args.steps, args.warmup)
+    u1 = height * width * nproj
+    line = {
+        "metric": METRIC, "value": nsl * u1 / (ms16 * 1e-3) / 1e9,
+        "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms16,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (f64 chunk sums)", "data": "synthetic",
+        "config": {"workload": "vertrec_1801x1024x2560", "nproj": nproj,
+                   "height": height, "width": width, "slices": nsl,
+                   "chunk_size": 30, "parallelism": "single-gpu"},
+        "cpu_baseline": None,
+        "detail": {"single_slice_after_filter_ms": ms1,
+                   "single_slice_gups": u1 / (ms1 * 1e-3) / 1e9,
+                   "multiple_16_before_filter_ms": ms16,
+                   "finite": bool(torch.isfinite(out).all().item())},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,7 +485,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="stack",
                     choices=["stack", "uneven", "refine", "volume",
-                             "center"])
+                             "center", "bpf", "vertrec"])
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -422,6 +510,12 @@ def main():
         return
     if args.workload == "center":
         run_center(args, dev)
+        return
+    if args.workload == "bpf":
+        run_bpf(args, dev)
+        return
+    if args.workload == "vertrec":
+        run_vertrec(args, dev)
         return
     sino, angles = make_stack(dev)
     updates = NSLICE * NCOL * NCOL * NROW
