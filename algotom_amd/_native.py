@@ -62,7 +62,7 @@ SIGNATURES = (
     ("ath_flat_field_log", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong,
                                 ctypes.c_longlong, _P, _P, _I, _P, _P, _P]),
     ("ath_vertical_back_projection", _I, [_P, _I, _I, _I, _P, _P, _P, _I, _D,
-                                          _I, _I, _I, _I, _P, _P]),
+                                          _P, _I, _I, _I, _I, _P, _P]),
     ("ath_vertical_finish", _I, [_P, _I, ctypes.c_longlong, _I, _I, _I, _I,
                                  _P]),
     ("ath_circular_filter_plan_bytes", _S, [_I]),
@@ -75,9 +75,10 @@ SIGNATURES = (
                                       ctypes.c_longlong, _D, _I, _P]),
     ("ath_gaussian_filter_rows", _I, [_P, _P, _I, _I, _P, _I, _P]),
     ("ath_zoom_linear", _I, [_P, _I, _I, _P, _I, _I, _P]),
-    ("ath_slice_metrics", _I, [_P, _I, _I, ctypes.c_longlong, _F, _F, _I, _P,
-                               _P, _S, _P]),
-    ("ath_slice_metrics_workspace_bytes", _S, [_I, _I, _I]),
+    ("ath_slice_metrics", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong, _D, _D,
+                               _I, _P, _P, _S, _P]),
+    ("ath_slice_metrics_workspace_bytes", _S, [_I, _I, _I, _I]),
+    ("ath_gaussian_filter_axis", _I, [_P, _P, _I, _I, _I, _P, _I, _P]),
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),
     ("ath_fbp_reconstruction", _I, [_P, _I, _I, _I, _I, _P, _F, _D, _P, _I,
                                     _I, _I, _I, _I, _I, _P, _S, _P, _P]),

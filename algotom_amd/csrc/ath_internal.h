@@ -118,8 +118,9 @@ constexpr int kVertCpu = ATH_VERT_CPU;
 int vertical_back_projection(const float* proj, int nproj, int height,
                              int width, const double* angles,
                              const double* xs, const double* ys, int nslice,
-                             double center, int edge_pad, int mode, int chunk,
-                             int accumulate, float* out, hipStream_t s);
+                             double center, const double* centers,
+                             int edge_pad, int mode, int chunk, int accumulate,
+                             float* out, hipStream_t s);
 int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
                    int width, long long proj_stride, long long row_stride,
                    const float* dark, const float* flat_dark, int apply_log,
@@ -129,10 +130,13 @@ int vertical_finish(void* data, int dtype, long long total, int width, int lo,
 int minus_log(const float* in, float* out, long long n, hipStream_t s);
 int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s);
-size_t slice_metrics_workspace(int nimg, int ncol, int metric);
-int slice_metrics(const float* images, int nimg, int ncol, long long stride,
-                  float nmin, float nmax, int metric, double* out,
-                  void* workspace, hipStream_t s);
+size_t slice_metrics_workspace(int nimg, int nrows, int ncols, int metric);
+int slice_metrics(const void* images, int dtype, int nimg, int nrows,
+                  int ncols, long long stride, double nmin, double nmax,
+                  int metric, double* out, void* workspace, hipStream_t s);
+int gaussian_axis(const float* in, float* out, int nouter, int length,
+                  int ninner, const double* weights, int radius,
+                  hipStream_t s);
 int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
                       int overlap, int side, int norm, int width,
                       float* factors, float* out, hipStream_t s);

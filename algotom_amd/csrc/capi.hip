@@ -399,7 +399,8 @@ int ath_flat_field_log(const void* proj, int in_dtype, int nproj, int height,
 int ath_vertical_back_projection(const float* proj, int nproj, int height,
                                  int width, const double* angles,
                                  const double* xs, const double* ys,
-                                 int nslice, double center, int edge_pad,
+                                 int nslice, double center,
+                                 const double* centers, int edge_pad,
                                  int mode, int chunk, int accumulate,
                                  float* out, void* stream) {
     clear_error();
@@ -414,8 +415,8 @@ int ath_vertical_back_projection(const float* proj, int nproj, int height,
     if (!std::isfinite(center))
         return set_error(ATH_ERR_INVALID_ARG, "center must be finite");
     return vertical_back_projection(proj, nproj, height, width, angles, xs, ys,
-                                    nslice, center, edge_pad, mode, chunk,
-                                    accumulate, out,
+                                    nslice, center, centers, edge_pad, mode,
+                                    chunk, accumulate, out,
                                     static_cast<hipStream_t>(stream));
 }
 
@@ -539,37 +540,57 @@ int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
                        static_cast<hipStream_t>(stream));
 }
 
-size_t ath_slice_metrics_workspace_bytes(int nimg, int ncol, int metric) {
-    if (nimg < 1 || ncol < 1 || (metric != 0 && metric != 1) ||
-        (long long)ncol * ncol > (1LL << 30))
+size_t ath_slice_metrics_workspace_bytes(int nimg, int nrows, int ncols,
+                                         int metric) {
+    if (nimg < 1 || nrows < 1 || ncols < 1 || metric < 0 || metric > 2 ||
+        (long long)nrows * ncols > (1LL << 30))
         return 0;
-    return slice_metrics_workspace(nimg, ncol, metric);
+    return slice_metrics_workspace(nimg, nrows, ncols, metric);
 }
 
-int ath_slice_metrics(const float* images, int nimg, int ncol,
-                      long long stride, float nmin, float nmax, int metric,
-                      double* out, void* workspace, size_t workspace_bytes,
-                      void* stream) {
+int ath_slice_metrics(const void* images, int dtype, int nimg, int nrows,
+                      int ncols, long long stride, double nmin, double nmax,
+                      int metric, double* out, void* workspace,
+                      size_t workspace_bytes, void* stream) {
     clear_error();
-    if (!images || !out || !workspace)
+    if (!images || !out)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
-    if (nimg < 1 || ncol < 1 || stride < (long long)ncol * ncol)
+    if (dtype != ATH_F32 && dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG, "float32 or float64 images");
+    if (nimg < 1 || nrows < 1 || ncols < 1 ||
+        stride < (long long)nrows * ncols)
         return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
     if (nimg > 65535)
         return set_error(ATH_ERR_UNSUPPORTED, "more than 65535 images a call");
-    if (metric != 0 && metric != 1)
-        return set_error(ATH_ERR_INVALID_ARG, "metric must be 0 or 1");
-    if ((long long)ncol * ncol > (1LL << 30))
+    if ((long long)nrows * ncols > (1LL << 30))
         return set_error(ATH_ERR_UNSUPPORTED, "image above 2^30 pixels");
-    if (metric == 1 && ncol < 2)
+    if (metric < 0 || metric > 2)
+        return set_error(ATH_ERR_INVALID_ARG, "metric must be 0, 1 or 2");
+    if (metric == 1 && (nrows < 2 || ncols < 2))
         return set_error(ATH_ERR_INVALID_ARG,
                          "gradient needs at least 2 pixels per axis");
     if (metric == 0 && !(nmax > nmin))
         return set_error(ATH_ERR_INVALID_ARG, "Empty image!!!");
-    if (workspace_bytes < slice_metrics_workspace(nimg, ncol, metric))
+    const size_t need = slice_metrics_workspace(nimg, nrows, ncols, metric);
+    if (need > 0 && (!workspace || workspace_bytes < need))
         return set_error(ATH_ERR_WORKSPACE, "slice metrics: workspace too small");
-    return slice_metrics(images, nimg, ncol, stride, nmin, nmax, metric, out,
-                         workspace, static_cast<hipStream_t>(stream));
+    return slice_metrics(images, dtype, nimg, nrows, ncols, stride, nmin, nmax,
+                         metric, out, workspace,
+                         static_cast<hipStream_t>(stream));
+}
+
+int ath_gaussian_filter_axis(const float* in, float* out, int nouter,
+                             int length, int ninner, const double* weights,
+                             int radius, void* stream) {
+    clear_error();
+    if (!in || !out || !weights)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nouter < 1 || length < 1 || ninner < 1 || radius < 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (nouter > 65535 && ninner > 1)
+        return set_error(ATH_ERR_UNSUPPORTED, "more than 65535 outer lines");
+    return gaussian_axis(in, out, nouter, length, ninner, weights, radius,
+                         static_cast<hipStream_t>(stream));
 }
 
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,

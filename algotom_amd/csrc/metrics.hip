@@ -116,29 +116,44 @@ __device__ T np_pairwise_serial(const T* a, int n) {
     return val[0];
 }
 
+// Sample of an image as the reference's float32 arithmetic sees it
+// (np.float32(image) for float64 slices).
+template <typename T>
+__device__ __forceinline__ float as_f32(T v) { return (float)v; }
+
+// uint8 level of one sample: clip((v - nmin) / (nmax - nmin), 0, 1) * 255,
+// truncated, in the image's precision (float32 slices: float32 scalars;
+// float64 slices: float64, as numpy evaluates it).
+__device__ __forceinline__ int level_of(float v, double nmin, double nmax) {
+    const float lo = (float)nmin;
+    float r = (v - lo) / ((float)nmax - lo);
+    r = fminf(fmaxf(r, 0.0f), 1.0f);
+    return (int)(r * 255.0f);
+}
+__device__ __forceinline__ int level_of(double v, double nmin, double nmax) {
+    double r = (v - nmin) / (nmax - nmin);
+    r = fmin(fmax(r, 0.0), 1.0);
+    return (int)(r * 255.0);
+}
+
 // Histogram of the uint8 levels: block (b, k) counts its contiguous share
 // of image k into LDS and writes the 256 counts to
 // partial[(k * nblk + b) * 256 + bin] (integers: the split changes nothing).
+template <typename T>
 __global__ __launch_bounds__(kThreads) void level_histogram_kernel(
-    const float* __restrict__ images, long long stride, int ncol, float nmin,
-    float nmax, unsigned int* __restrict__ partial) {
+    const T* __restrict__ images, long long stride, long long n, double nmin,
+    double nmax, unsigned int* __restrict__ partial) {
     __shared__ unsigned int hist[256];
     const int tid = threadIdx.x;
     const int nblk = gridDim.x;
-    const float* img = images + (size_t)blockIdx.y * (size_t)stride;
+    const T* img = images + (size_t)blockIdx.y * (size_t)stride;
     hist[tid] = 0u;
     __syncthreads();
-    const long long n = (long long)ncol * ncol;
     const long long per = (n + nblk - 1) / nblk;
     const long long e0 = (long long)blockIdx.x * per;
     const long long e1 = min(n, e0 + per);
-    const float range = nmax - nmin;
-    for (long long e = e0 + tid; e < e1; e += kThreads) {
-        float r = (img[e] - nmin) / range;
-        r = fminf(fmaxf(r, 0.0f), 1.0f);
-        const int v = (int)(r * 255.0f);  // np.uint8 truncation
-        atomicAdd(&hist[v], 1u);
-    }
+    for (long long e = e0 + tid; e < e1; e += kThreads)
+        atomicAdd(&hist[level_of(img[e], nmin, nmax)], 1u);
     __syncthreads();
     partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 256 + tid] = hist[tid];
 }
@@ -166,18 +181,21 @@ __global__ __launch_bounds__(kThreads) void entropy_finish_kernel(
     if (tid == 0) out[blockIdx.x] = -np_pairwise_serial(h7, 250);
 }
 
-// |np.gradient| of pixel e of a (ncol, ncol) image: axis 0 then axis 1,
-// central differences halved inside, one-sided at the borders, float32.
-__device__ __forceinline__ float grad_magnitude(const float* img, int ncol,
-                                                int e) {
-    const int y = e / ncol, x = e - y * ncol;
+// |np.gradient| of pixel e of a (nrows, ncols) image converted to float32:
+// axis 0 then axis 1, central differences halved inside, one-sided at the
+// borders, float32.
+template <typename T>
+__device__ __forceinline__ float grad_magnitude(const T* img, int nrows,
+                                                int ncols, int e) {
+    const int y = e / ncols, x = e - y * ncols;
     float gy, gx;
-    if (y == 0) gy = img[e + ncol] - img[e];
-    else if (y == ncol - 1) gy = img[e] - img[e - ncol];
-    else gy = (img[e + ncol] - img[e - ncol]) / 2.0f;
-    if (x == 0) gx = img[e + 1] - img[e];
-    else if (x == ncol - 1) gx = img[e] - img[e - 1];
-    else gx = (img[e + 1] - img[e - 1]) / 2.0f;
+    const float v = as_f32(img[e]);
+    if (y == 0) gy = as_f32(img[e + ncols]) - v;
+    else if (y == nrows - 1) gy = v - as_f32(img[e - ncols]);
+    else gy = (as_f32(img[e + ncols]) - as_f32(img[e - ncols])) / 2.0f;
+    if (x == 0) gx = as_f32(img[e + 1]) - v;
+    else if (x == ncols - 1) gx = v - as_f32(img[e - 1]);
+    else gx = (as_f32(img[e + 1]) - as_f32(img[e - 1])) / 2.0f;
     return sqrtf(gy * gy + gx * gx);
 }
 
@@ -189,18 +207,19 @@ __device__ __forceinline__ float grad_magnitude(const float* img, int ncol,
 // 64 blocks of 128 (64 threads) combined as the perfect binary tree the
 // recursion forms, a shorter tail chunk is summed by one thread.  The chunk
 // sum goes to partial[k * nchunk + c].
+template <typename T>
 __global__ __launch_bounds__(kThreads) void sharpness_chunk_kernel(
-    const float* __restrict__ images, long long stride, int ncol,
+    const T* __restrict__ images, long long stride, int nrows, int ncols,
     float* __restrict__ partial) {
     __shared__ float mag[kChunk];
     __shared__ float part[64];
-    const float* img = images + (size_t)blockIdx.y * (size_t)stride;
+    const T* img = images + (size_t)blockIdx.y * (size_t)stride;
     const int tid = threadIdx.x;
-    const int n = ncol * ncol;
+    const int n = nrows * ncols;
     const int c0 = blockIdx.x * kChunk;
     const int m = min(kChunk, n - c0);
     for (int k = tid; k < m; k += kThreads)
-        mag[k] = grad_magnitude(img, ncol, c0 + k);
+        mag[k] = grad_magnitude(img, nrows, ncols, c0 + k);
     __syncthreads();
     float chunk;
     if (m == kChunk) {
@@ -220,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void sharpness_chunk_kernel(
 
 // One workgroup per image: the chunk sums added in order, staged in LDS.
 __global__ __launch_bounds__(kThreads) void sharpness_finish_kernel(
-    const float* __restrict__ partial, int nchunk, int ncol,
+    const float* __restrict__ partial, int nchunk, long long n,
     double* __restrict__ out) {
     __shared__ float buf[4096];
     const int tid = threadIdx.x;
@@ -234,8 +253,65 @@ __global__ __launch_bounds__(kThreads) void sharpness_finish_kernel(
             for (int k = 0; k < m; ++k) total = total + buf[k];
         __syncthreads();
     }
-    const double n = (double)ncol * (double)ncol;
-    if (tid == 0) out[blockIdx.x] = (double)(float)((double)total / n);
+    if (tid == 0) out[blockIdx.x] = (double)(float)((double)total / (double)n);
+}
+
+// vertrec's autocorrelation metric (vertrec.py:1255-1261): the mean over
+// rows of max |ifft(|fft(row)|^2)|.  The circular autocorrelation of a real
+// row peaks at lag 0, where it equals sum(row^2) (Parseval), so each row
+// contributes its float64 sum of squares of the float32 samples.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void autocorr_kernel(
+    const T* __restrict__ images, long long stride, int nrows, int ncols,
+    double* __restrict__ out) {
+    __shared__ double red[kThreads];
+    const T* img = images + (size_t)blockIdx.x * (size_t)stride;
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int r = tid; r < nrows; r += kThreads) {
+        const T* row = img + (size_t)r * ncols;
+        double s = 0.0;
+        for (int c = 0; c < ncols; ++c) {
+            const double v = (double)as_f32(row[c]);
+            s += v * v;
+        }
+        acc += s;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) out[blockIdx.x] = red[0] / (double)nrows;
+}
+
+// scipy.ndimage.gaussian_filter1d along the middle axis of a contiguous
+// (nouter, length, ninner) float32 array, ninner > 1: one thread per inner
+// index (coalesced across lanes), looping along the axis; same sums and
+// boundary as gaussian_rows_kernel.
+__global__ __launch_bounds__(kThreads) void gaussian_inner_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int length,
+    int ninner, const double* __restrict__ weights, int radius) {
+    __shared__ double w[2 * kMaxRadius + 1];
+    for (int k = threadIdx.x; k <= 2 * radius; k += kThreads)
+        w[k] = weights[k];
+    __syncthreads();
+    const int b = blockIdx.x * kThreads + threadIdx.x;
+    if (b >= ninner) return;
+    const size_t base = (size_t)blockIdx.y * length * ninner + b;
+    const float* src = in + base;
+    float* dst = out + base;
+    for (int k = 0; k < length; ++k) {
+        double acc = (double)src[(size_t)k * ninner] * w[radius];
+        for (int j = 1; j <= radius; ++j) {
+            const int kp = pad_source(k + j, length, ATH_PAD_SYMMETRIC);
+            const int km = pad_source(k - j, length, ATH_PAD_SYMMETRIC);
+            acc += ((double)src[(size_t)kp * ninner] +
+                    (double)src[(size_t)km * ninner]) * w[radius + j];
+        }
+        dst[(size_t)k * ninner] = (float)acc;
+    }
 }
 
 // scipy.ndimage.zoom(x, zoom, order=1, mode="nearest") of a 2D float32
@@ -286,27 +362,32 @@ int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
     return check_hip(hipGetLastError(), "zoom_linear_kernel launch");
 }
 
-static int entropy_blocks(int ncol) {
-    const long long n = (long long)ncol * ncol;
+static int entropy_blocks(long long n) {
     return (int)std::min<long long>(
         64, std::max<long long>(1, (n + 65535) / 65536));
 }
 
-size_t slice_metrics_workspace(int nimg, int ncol, int metric) {
+size_t slice_metrics_workspace(int nimg, int nrows, int ncols, int metric) {
+    const long long n = (long long)nrows * ncols;
     if (metric == 0)
-        return (size_t)nimg * entropy_blocks(ncol) * 256 * sizeof(unsigned);
-    const long long n = (long long)ncol * ncol;
-    return (size_t)nimg * (size_t)((n + kChunk - 1) / kChunk) * sizeof(float);
+        return (size_t)nimg * entropy_blocks(n) * 256 * sizeof(unsigned);
+    if (metric == 1)
+        return (size_t)nimg * (size_t)((n + kChunk - 1) / kChunk) *
+               sizeof(float);
+    return 0;
 }
 
-int slice_metrics(const float* images, int nimg, int ncol, long long stride,
-                  float nmin, float nmax, int metric, double* out,
-                  void* workspace, hipStream_t s) {
+template <typename T>
+static int metrics_typed(const T* images, int nimg, int nrows, int ncols,
+                         long long stride, double nmin, double nmax,
+                         int metric, double* out, void* workspace,
+                         hipStream_t s) {
+    const long long n = (long long)nrows * ncols;
     if (metric == 0) {
-        const int nblk = entropy_blocks(ncol);
+        const int nblk = entropy_blocks(n);
         auto* partial = static_cast<unsigned int*>(workspace);
-        hipLaunchKernelGGL(level_histogram_kernel, dim3(nblk, nimg),
-                           dim3(kThreads), 0, s, images, stride, ncol, nmin,
+        hipLaunchKernelGGL(level_histogram_kernel<T>, dim3(nblk, nimg),
+                           dim3(kThreads), 0, s, images, stride, n, nmin,
                            nmax, partial);
         const int st = check_hip(hipGetLastError(), "level_histogram_kernel");
         if (st != ATH_OK) return st;
@@ -314,16 +395,46 @@ int slice_metrics(const float* images, int nimg, int ncol, long long stride,
                            0, s, partial, nblk, out);
         return check_hip(hipGetLastError(), "entropy_finish_kernel");
     }
-    const int n = ncol * ncol;
-    const int nchunk = (n + kChunk - 1) / kChunk;
+    if (metric == 2) {
+        hipLaunchKernelGGL(autocorr_kernel<T>, dim3(nimg), dim3(kThreads), 0,
+                           s, images, stride, nrows, ncols, out);
+        return check_hip(hipGetLastError(), "autocorr_kernel");
+    }
+    const int nchunk = (int)((n + kChunk - 1) / kChunk);
     auto* partial = static_cast<float*>(workspace);
-    hipLaunchKernelGGL(sharpness_chunk_kernel, dim3(nchunk, nimg),
-                       dim3(kThreads), 0, s, images, stride, ncol, partial);
+    hipLaunchKernelGGL(sharpness_chunk_kernel<T>, dim3(nchunk, nimg),
+                       dim3(kThreads), 0, s, images, stride, nrows, ncols,
+                       partial);
     const int st = check_hip(hipGetLastError(), "sharpness_chunk_kernel");
     if (st != ATH_OK) return st;
     hipLaunchKernelGGL(sharpness_finish_kernel, dim3(nimg), dim3(kThreads), 0,
-                       s, partial, nchunk, ncol, out);
+                       s, partial, nchunk, n, out);
     return check_hip(hipGetLastError(), "sharpness_finish_kernel");
+}
+
+int slice_metrics(const void* images, int dtype, int nimg, int nrows,
+                  int ncols, long long stride, double nmin, double nmax,
+                  int metric, double* out, void* workspace, hipStream_t s) {
+    if (dtype == ATH_F64)
+        return metrics_typed(static_cast<const double*>(images), nimg, nrows,
+                             ncols, stride, nmin, nmax, metric, out,
+                             workspace, s);
+    return metrics_typed(static_cast<const float*>(images), nimg, nrows, ncols,
+                         stride, nmin, nmax, metric, out, workspace, s);
+}
+
+int gaussian_axis(const float* in, float* out, int nouter, int length,
+                  int ninner, const double* weights, int radius,
+                  hipStream_t s) {
+    if (radius > kMaxRadius)
+        return set_error(ATH_ERR_UNSUPPORTED, "gaussian radius above 64");
+    if (ninner == 1) return gaussian_rows(in, out, nouter, length, weights,
+                                          radius, s);
+    hipLaunchKernelGGL(gaussian_inner_kernel,
+                       dim3((ninner + kThreads - 1) / kThreads, nouter),
+                       dim3(kThreads), 0, s, in, out, length, ninner, weights,
+                       radius);
+    return check_hip(hipGetLastError(), "gaussian_inner_kernel launch");
 }
 
 }  // namespace ath

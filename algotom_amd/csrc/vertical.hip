@@ -37,6 +37,7 @@ struct VertArgs {
     const double* angles; // nproj radians
     const double* xs;     // (nslice, width)
     const double* ys;
+    const double* centers;  // per-line centres, or nullptr
     float* out;           // (nslice, height, width)
     int nproj, height, width, nslice;
     double center;
@@ -58,7 +59,8 @@ __global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a) {
         x = a.xs[(size_t)sl * a.width + j];
         y = a.ys[(size_t)sl * a.width + j];
     }
-    const float x32 = (float)x, y32 = (float)y, cen32 = (float)a.center;
+    const double cen = a.centers ? a.centers[sl] : a.center;
+    const float x32 = (float)x, y32 = (float)y, cen32 = (float)cen;
     double chunk64[kRows];
     float chunk32[kRows], total[kRows];
 #pragma unroll
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a) {
                 fpos = (double)f;
                 frac = (double)f - (double)d;
             } else {
-                const double f = (x * s_c[q] + y * s_s[q]) + a.center;
+                const double f = (x * s_c[q] + y * s_s[q]) + cen;
                 in_range = (f >= 0.0) && (f <= (double)width1);
                 d = (int)floor(f);
                 u = (int)ceil(f);
@@ -220,10 +222,11 @@ long long stream_blocks(long long total) {
 int vertical_back_projection(const float* proj, int nproj, int height,
                              int width, const double* angles,
                              const double* xs, const double* ys, int nslice,
-                             double center, int edge_pad, int mode, int chunk,
-                             int accumulate, float* out, hipStream_t s) {
-    VertArgs a{proj, angles, xs, ys, out, nproj, height, width, nslice,
-               center, edge_pad, chunk, accumulate};
+                             double center, const double* centers,
+                             int edge_pad, int mode, int chunk, int accumulate,
+                             float* out, hipStream_t s) {
+    VertArgs a{proj, angles, xs, ys, centers, out, nproj, height, width,
+               nslice, center, edge_pad, chunk, accumulate};
     const dim3 grid((width + kPts - 1) / kPts,
                     (height + 4 * kRows - 1) / (4 * kRows), nslice);
     if (mode == kVertGpu)

@@ -156,10 +156,10 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
             continue
         code = 0 if metric == "entropy" else 1
         out = torch.empty(len(cs), dtype=torch.float64, device=dev)
-        nws = lib.ath_slice_metrics_workspace_bytes(len(cs), ncol, code)
+        nws = lib.ath_slice_metrics_workspace_bytes(len(cs), ncol, ncol, code)
         ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
         nat.check(lib.ath_slice_metrics(
-            nat.ptr(imgs), len(cs), ncol, ncol * ncol,
+            nat.ptr(imgs), nat.ATH_F32, len(cs), ncol, ncol, ncol * ncol,
             float(np.float32(nmin)), float(np.float32(nmax)), code,
             nat.ptr(out), nat.ptr(ws), nws, nat.stream_ptr(dev)),
             "ath_slice_metrics")

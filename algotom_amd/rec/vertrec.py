@@ -109,15 +109,17 @@ def _angles64(angles, dev):
 
 
 def _vbp(proj_t, ang_t, x_t, y_t, center, edge_pad, mode, chunk, out,
-         accumulate):
+         accumulate, centers_t=None):
     nproj, height, width = proj_t.shape
     if x_t.shape[1] != width:
         raise ValueError("the lines need one point per projection column")
     nat.check(nat.load().ath_vertical_back_projection(
         nat.ptr(proj_t), nproj, height, width, nat.ptr(ang_t), nat.ptr(x_t),
-        nat.ptr(y_t), x_t.shape[0], float(center), int(bool(edge_pad)),
-        int(mode), int(chunk), int(bool(accumulate)), nat.ptr(out),
-        nat.stream_ptr(proj_t.device)), "ath_vertical_back_projection")
+        nat.ptr(y_t), x_t.shape[0], float(center),
+        nat.ptr(centers_t) if centers_t is not None else None,
+        int(bool(edge_pad)), int(mode), int(chunk), int(bool(accumulate)),
+        nat.ptr(out), nat.stream_ptr(proj_t.device)),
+        "ath_vertical_back_projection")
     return out
 
 
@@ -288,7 +290,12 @@ def _prepare(raw, fd, apply_log, flag, dev):
 def _vertical_pipeline(projections, xs, ys, single, center, flat_field,
                        dark_field, angles, crop, proj_start, proj_stop,
                        chunk_size, ramp_filter, filter_name, pad, pad_mode,
-                       apply_log, gpu, show_progress, masking):
+                       apply_log, gpu, show_progress, masking, centers=None,
+                       mask_lohi=None, keep_device=False):
+    """The reconstruction of vertical_reconstruction (vertrec.py:450-695)
+    for one or more lines.  centers: candidate rotation centres of one line
+    (find_center_vertical_slice), each reconstructed as its own slice with
+    the single-slice output dtype; mask_lohi: explicit mask columns."""
     import torch
     (proj_stop, num_proj, angles, box, height, width, height0,
      width0) = _check_common(projections, angles, crop, proj_start,
@@ -306,6 +313,12 @@ def _vertical_pipeline(projections, xs, ys, single, center, flat_field,
     if ramp_filter == "before":
         taps = rec._window_taps(None, height, width + 2 * pad, filter_name)
     x_t, y_t = _points_device(xs, ys, dev)
+    centers_t = None
+    if centers is not None:
+        centers_t = torch.from_numpy(np.ascontiguousarray(
+            np.asarray(centers, dtype=np.float64))).to(dev)
+        x_t = x_t.expand(len(centers), -1).contiguous()
+        y_t = y_t.expand(len(centers), -1).contiguous()
     nslice = x_t.shape[0]
     ang_t = _angles64(angles, dev)
     mode = nat.VERT_GPU if gpu else nat.VERT_CPU
@@ -327,7 +340,7 @@ def _vertical_pipeline(projections, xs, ys, single, center, flat_field,
             img = rec._filter_stack(img, pad, pad_mode, False, taps,
                                     torch.float32)
         _vbp(img, ang_t[g0:g1], x_t, y_t, center, edge_pad, mode, chunk, ver,
-             g0 > 0)
+             g0 > 0, centers_t)
         if show_progress:
             sys.stdout.write("\rProcessed {}/{} images ({:.0f}%) - Time "
                              "elapsed: {:.2f} seconds".format(
@@ -341,22 +354,27 @@ def _vertical_pipeline(projections, xs, ys, single, center, flat_field,
         # |ramp window| along every slice row (vertrec.py:621-627, 698-706)
         row = np.abs(rec._ramp_row(width + 2 * pad, filter_name))
         taps_a = rec._taps(row, outer_shift=False)[None, :]
-        out_dtype = torch.float64 if single else torch.float32
+        out_dtype = torch.float64 if (single or centers is not None) \
+            else torch.float32
         result = rec._filter_stack(ver.reshape(nslice * height, 1, width),
                                    pad, pad_mode, False, taps_a,
                                    out_dtype).reshape(nslice, height, width)
     if show_progress:
         print("\nDone! Total time elapsed: {0:.2f}".format(time.time() - t0))
     lo = hi = -1
-    if masking:
+    if mask_lohi is not None:
+        lo, hi = mask_lohi
+    elif masking:
         rad = int(min(center, width - center))
         lo = (width - 2 * rad) // 2
         hi = lo + 2 * rad
     nat.check(nat.load().ath_vertical_finish(
         nat.ptr(result), nat.dtype_code(result.dtype), result.numel(), width,
         lo, hi, num_proj, nat.stream_ptr(dev)), "ath_vertical_finish")
-    if single:
+    if single and centers is None:
         result = result[0]
+    if keep_device:
+        return result
     return rec._finish(result, as_numpy)
 
 
@@ -446,3 +464,136 @@ def vertical_reconstruction_different_angles(projections, slice_indices,
                               proj_stop, chunk_size, ramp_filter,
                               filter_name, pad, pad_mode, apply_log, gpu,
                               show_progress, masking)
+
+
+# --------------------------------------------------------------------------
+# centre search on a vertical slice (vertrec.py:1255-1428)
+# --------------------------------------------------------------------------
+def _gaussian_2d(p, sigma):
+    """ndi.gaussian_filter(p, (0, sigma, sigma)) of a (n, height, width)
+    float32 device tensor: axis 1 then axis 2, float32 between the passes
+    (vertrec.py:1331-1332)."""
+    import torch
+    from algotom_amd.rec import center as ctr
+    w, radius = ctr._gaussian_weights(sigma)
+    w_d = torch.from_numpy(np.ascontiguousarray(w)).to(p.device)
+    n, height, width = p.shape
+    lib = nat.load()
+    mid = torch.empty_like(p)
+    nat.check(lib.ath_gaussian_filter_axis(
+        nat.ptr(p), nat.ptr(mid), n, height, width, nat.ptr(w_d), radius,
+        nat.stream_ptr(p.device)), "ath_gaussian_filter_axis")
+    out = torch.empty_like(p)
+    nat.check(lib.ath_gaussian_filter_axis(
+        nat.ptr(mid), nat.ptr(out), n * height, width, 1, nat.ptr(w_d),
+        radius, nat.stream_ptr(p.device)), "ath_gaussian_filter_axis")
+    return out
+
+
+_METRICS = {"entropy": 0, "sharpness": 1, "autocorrelation": 2}
+_SEARCH_BYTES = 2 << 30
+
+
+def find_center_vertical_slice(projections, slice_index, start, stop,
+                               step=1.0, metric="entropy", alpha=0.0,
+                               angles=None, chunk_size=30,
+                               ramp_filter="after", sigma=3, apply_log=True,
+                               gpu=True, block=(16, 16), ncore=None,
+                               prefer="threads", show_progress=True,
+                               masking=True, return_metric=False,
+                               invert_metric=False, metric_function=None,
+                               **kwargs):
+    """Find the centre of rotation from metrics of a vertical slice
+    reconstructed at centres arange(start, stop + step, step)
+    (vertrec.py:1264-1428); the minimum metric gives the centre.
+
+    The projections are smoothed (Gaussian over rows and columns) and
+    logged once; every candidate centre is then reconstructed as its own
+    line in one vertical back-projection launch (per pass of whole chunks)
+    and all metrics -- "entropy", "sharpness", "autocorrelation" -- are
+    computed on the GPU.  A custom metric_function is called on each slice
+    as a numpy array.  Returns the centre, or (list_center, list_metric)
+    if return_metric.
+    """
+    import torch
+    if (metric not in _METRICS) and metric_function is None:
+        raise ValueError("Please select one of three options: 'entropy', "
+                         "'sharpness', 'autocorrelation'; or provide a "
+                         "custom method.")
+    if ramp_filter is None:
+        ramp_filter = "after"
+    list_center = np.arange(start, stop + step, step)
+    (num_proj, height, width) = projections.shape
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, num_proj))
+    dev = nat.device()
+    p = rec._to_device(projections, dev, allow_f64=False)
+    if sigma > 0:
+        p = _gaussian_2d(p, sigma)
+    if apply_log:
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        p = _prepare(p, None, True, flag, dev)
+        if int(flag.item()) != 0:
+            warnings.warn(_LOG_WARNING)
+    lohi = None
+    if masking:
+        cmin = np.min(list_center)
+        rad = int(min(cmin, width - cmin))
+        lo = (width - 2 * rad) // 2
+        lohi = (lo, lo + 2 * rad)
+    pad = min(int(0.15 * width), 150)
+    xs, ys = _get_points_single_line(slice_index, alpha, width)
+    if np.any(list_center < 0) or np.any(list_center > width - 1):
+        raise ValueError("Center (relative to the cropped image) is out of "
+                         "range {})".format((0, width - 1)))
+
+    def recon(cs, mask):
+        return _vertical_pipeline(
+            p, xs, ys, True, float(cs[0]), None, None, angles, (0, 0, 0, 0),
+            0, -1, chunk_size, ramp_filter, "hann", pad, "edge", False, gpu,
+            False, False, centers=cs, mask_lohi=mask, keep_device=True)
+
+    nmin, nmax = 0.0, 1.0
+    if metric_function is None and metric == "entropy":
+        ref = recon([np.mean(list_center)], None)
+        nmin = float(torch.min(ref).item())
+        nmax = float(torch.max(ref).item())
+    esize = 8 if ramp_filter == "after" else 4
+    per_batch = max(1, _SEARCH_BYTES // (esize * height * width))
+    lib = nat.load()
+    values = []
+    t0 = time.time()
+    for b0 in range(0, len(list_center), per_batch):
+        cs = list_center[b0:b0 + per_batch]
+        imgs = recon(cs, lohi)
+        if metric_function is not None:
+            for k in range(len(cs)):
+                values.append(metric_function(imgs[k].cpu().numpy(),
+                                              **kwargs))
+        else:
+            code = _METRICS[metric]
+            out = torch.empty(len(cs), dtype=torch.float64, device=dev)
+            nws = lib.ath_slice_metrics_workspace_bytes(len(cs), height,
+                                                        width, code)
+            ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
+            nat.check(lib.ath_slice_metrics(
+                nat.ptr(imgs), nat.dtype_code(imgs.dtype), len(cs), height,
+                width, height * width, nmin, nmax, code, nat.ptr(out),
+                nat.ptr(ws), nws, nat.stream_ptr(dev)), "ath_slice_metrics")
+            values.extend(out.cpu().numpy().tolist())
+        if show_progress:
+            sys.stdout.write("\rReconstructed {}/{} centres - Time elapsed: "
+                             "{:.2f} seconds".format(
+                                 min(b0 + per_batch, len(list_center)),
+                                 len(list_center), time.time() - t0))
+            sys.stdout.flush()
+    list_metric = np.zeros_like(list_center)
+    list_metric[:] = values
+    if invert_metric:
+        list_metric = np.max(list_metric) - list_metric
+    center = list_center[np.argmin(list_metric)]
+    if show_progress:
+        print("\nDone! Optimal center: {0}.".format(center))
+    if return_metric is True:
+        return list_center, list_metric
+    return center

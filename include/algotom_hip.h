@@ -319,11 +319,14 @@ int ath_flat_field_log(const void* proj, int in_dtype, int nproj, int height,
  * vertical_back_projection_gpu / _gpu_chunk (:176-335), ATH_VERT_CPU
  * vertical_back_projection_cpu / _cpu_chunk (:53-173).  accumulate: add
  * to out instead of overwriting (for projection stacks fed in pieces of
- * whole chunks). */
+ * whole chunks).  centers: NULL, or one rotation centre per line (device,
+ * float64) used instead of `center` -- every candidate centre of
+ * find_center_vertical_slice (vertrec.py:1264-1428) in one launch. */
 int ath_vertical_back_projection(const float* proj, int nproj, int height,
                                  int width, const double* angles,
                                  const double* xs, const double* ys,
-                                 int nslice, double center, int edge_pad,
+                                 int nslice, double center,
+                                 const double* centers, int edge_pad,
                                  int mode, int chunk, int accumulate,
                                  float* out, void* stream);
 
@@ -351,21 +354,39 @@ int ath_gaussian_filter_rows(const float* in, float* out, int nrows,
 int ath_zoom_linear(const float* in, int nrow, int ncol, float* out,
                     int out_nrow, int out_ncol, void* stream);
 
-/* Centre-search metrics of nimg square images (ncol x ncol float32, image
- * k at images + k*stride): metric 0 = histogram entropy of
- * clip((img - nmin)/(nmax - nmin), 0, 1) (reconstruction.py:1211-1220,
- * 1243-1246), metric 1 = mean gradient magnitude (:1223-1228; ncol >= 2).
+/* Centre-search metrics of nimg images (nrows x ncols, ATH_F32 or ATH_F64,
+ * image k at images + k*stride elements):
+ *   metric 0  histogram entropy of clip((img - nmin)/(nmax - nmin), 0, 1)
+ *             (reconstruction.py:1211-1220, 1243-1246), evaluated in the
+ *             image's precision (nmin/nmax rounded to float32 for float32
+ *             images);
+ *   metric 1  mean gradient magnitude of the float32 image (:1223-1228;
+ *             nrows, ncols >= 2);
+ *   metric 2  vertrec's autocorrelation metric (vertrec.py:1255-1261): the
+ *             mean over rows of sum(row^2), the lag-0 peak of each row's
+ *             circular autocorrelation.
  * Sums follow numpy's order (pairwise within 8192-value chunks).  out: nimg
  * float64 (device); workspace: caller device memory of at least
- * ath_slice_metrics_workspace_bytes(nimg, ncol, metric) bytes. */
-int ath_slice_metrics(const float* images, int nimg, int ncol,
-                      long long stride, float nmin, float nmax, int metric,
-                      double* out, void* workspace, size_t workspace_bytes,
-                      void* stream);
+ * ath_slice_metrics_workspace_bytes(...) bytes (may be NULL when that is 0). */
+int ath_slice_metrics(const void* images, int dtype, int nimg, int nrows,
+                      int ncols, long long stride, double nmin, double nmax,
+                      int metric, double* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
 
 /* Device workspace bytes ath_slice_metrics needs (per-block partial
- * histograms or per-chunk partial sums); 0 for invalid arguments. */
-size_t ath_slice_metrics_workspace_bytes(int nimg, int ncol, int metric);
+ * histograms or per-chunk partial sums); 0 when none is needed or for
+ * invalid arguments. */
+size_t ath_slice_metrics_workspace_bytes(int nimg, int nrows, int ncols,
+                                         int metric);
+
+/* scipy.ndimage.gaussian_filter1d(x, sigma, axis=1) of a contiguous
+ * (nouter, length, ninner) float32 array (mode "reflect", float64 sums,
+ * float32 out; weights as for ath_gaussian_filter_rows).  Two calls make
+ * ndi.gaussian_filter(projections, (0, sigma, sigma)) of
+ * find_center_vertical_slice (vertrec.py:1331-1332). */
+int ath_gaussian_filter_axis(const float* in, float* out, int nouter,
+                             int length, int ninner, const double* weights,
+                             int radius, void* stream);
 
 /* Device workspace bytes needed by ath_fbp_reconstruction. */
 size_t ath_fbp_workspace_bytes(int nrow, int nslice, int ncol, int pad,

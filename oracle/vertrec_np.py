@@ -223,3 +223,66 @@ def vertical_reconstruction(proj, slice_index, center, alpha=0.0,
     else:
         out = out * np.pi / (num_proj - 1)
     return out[0] if single else out
+
+
+def autocorrelation_metric(image):
+    """vertrec.py:1255-1261: mean over rows of max |ifft(|fft(row)|^2)|,
+    complex128 (numpy 1.x semantics for the float32 image)."""
+    rows = np.fft.fft(np.float32(image).astype(np.float64))
+    ac = np.abs(np.fft.ifft(np.abs(rows) ** 2))
+    return np.mean(np.max(ac, axis=1))
+
+
+def find_center_vertical(proj, slice_index, start, stop, step=1.0,
+                         metric="entropy", alpha=0.0, angles=None,
+                         chunk_size=30, ramp_filter="after", sigma=3,
+                         apply_log=True, gpu=True, masking=True,
+                         invert_metric=False, metric_function=None, **kw):
+    """find_center_vertical_slice (vertrec.py:1264-1428) with
+    return_metric=True: (list_center, list_metric)."""
+    import scipy.ndimage as ndi
+    from oracle import center_np
+    if ramp_filter is None:
+        ramp_filter = "after"
+    centers = np.arange(start, stop + step, step)
+    num_proj, height, width = proj.shape
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, num_proj))
+    p = np.asarray(proj, dtype=np.float32)
+    if sigma > 0:
+        p = ndi.gaussian_filter(p, (0, sigma, sigma))
+    if apply_log:
+        p = p.copy()
+        p[p <= 0.0] = 1.0
+        p = (-np.log(p.astype(np.float64))).astype(np.float32)
+    mask = None
+    if masking:
+        cmin = np.min(centers)
+        rad = int(min(cmin, width - cmin))
+        lo = (width - 2 * rad) // 2
+        mask = np.zeros(width, np.float32)
+        mask[lo:lo + 2 * rad] = 1.0
+    pad = min(int(0.15 * width), 150)
+    kwr = dict(alpha=alpha, angles=angles, chunk_size=chunk_size,
+               ramp_filter=ramp_filter, pad=pad, pad_mode="edge",
+               filter_name="hann", apply_log=False, gpu=gpu, masking=False)
+    nmin, nmax = 0.0, 1.0
+    if metric_function is None and metric == "entropy":
+        ref = vertical_reconstruction(p, slice_index, np.mean(centers), **kwr)
+        nmin, nmax = np.min(ref), np.max(ref)
+    values = np.zeros_like(centers)
+    for i, c in enumerate(centers):
+        rec = vertical_reconstruction(p, slice_index, c, **kwr)
+        if mask is not None:
+            rec = rec * mask
+        if metric_function is not None:
+            values[i] = metric_function(rec, **kw)
+        elif metric == "entropy":
+            values[i] = center_np.slice_metric(rec, "entropy", nmin, nmax)
+        elif metric == "autocorrelation":
+            values[i] = autocorrelation_metric(rec)
+        else:
+            values[i] = center_np.edge_sharpness(rec)
+    if invert_metric:
+        values = np.max(values) - values
+    return centers, values

@@ -56,10 +56,10 @@ def _metrics_gpu(imgs, metric, nmin=0.0, nmax=1.0):
     code = 0 if metric == "entropy" else 1
     lib = nat.load()
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    nws = lib.ath_slice_metrics_workspace_bytes(n, c, code)
+    nws = lib.ath_slice_metrics_workspace_bytes(n, c, c, code)
     ws = torch.full((nws,), 0xAB, dtype=torch.uint8, device="cuda")
     nat.check(lib.ath_slice_metrics(
-        nat.ptr(t), n, c, c * c, float(np.float32(nmin)),
+        nat.ptr(t), nat.ATH_F32, n, c, c, c * c, float(np.float32(nmin)),
         float(np.float32(nmax)), code, nat.ptr(out), nat.ptr(ws), nws,
         nat.stream_ptr(t.device)), "ath_slice_metrics")
     return out.cpu().numpy()

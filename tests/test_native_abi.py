@@ -130,10 +130,14 @@ def test_every_module_imports_first_in_a_fresh_process():
 def test_slice_metrics_workspace_sizes():
     lib = nat.load()
     # entropy: 64 partial 256-bin histograms per 2560^2 image
-    assert lib.ath_slice_metrics_workspace_bytes(3, 2560, 0) == 3 * 64 * 1024
-    assert lib.ath_slice_metrics_workspace_bytes(1, 100, 0) == 1024
+    ws = lib.ath_slice_metrics_workspace_bytes
+    assert ws(3, 2560, 2560, 0) == 3 * 64 * 1024
+    assert ws(1, 100, 100, 0) == 1024
     # sharpness: one float per 8192-pixel chunk (2560^2 = 800 chunks)
-    assert lib.ath_slice_metrics_workspace_bytes(2, 2560, 1) == 2 * 800 * 4
-    assert lib.ath_slice_metrics_workspace_bytes(1, 97, 1) == 2 * 4
-    assert lib.ath_slice_metrics_workspace_bytes(0, 97, 1) == 0
-    assert lib.ath_slice_metrics_workspace_bytes(1, 97, 2) == 0
+    assert ws(2, 2560, 2560, 1) == 2 * 800 * 4
+    assert ws(1, 97, 97, 1) == 2 * 4
+    assert ws(1, 40, 61, 1) == 4
+    # autocorrelation needs none; invalid arguments give 0
+    assert ws(5, 40, 61, 2) == 0
+    assert ws(0, 97, 97, 1) == 0
+    assert ws(1, 97, 97, 3) == 0

@@ -121,3 +121,38 @@ def test_after_filter_taps_drive_the_gpu_filter(width, pad):
                     for v in ver])
     ref = vertrec_np.after_filter(ver, rrow, pad, "edge")
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_autocorrelation_metric_is_mean_row_energy():
+    """The GPU evaluates the metric as the mean over rows of sum(row^2):
+    the lag-0 peak of each row's circular autocorrelation (Parseval)."""
+    rng = np.random.default_rng(8)
+    for shape in ((40, 61), (7, 128), (100, 33)):
+        img = rng.standard_normal(shape)
+        lit = vertrec_np.autocorrelation_metric(img)
+        x = np.float32(img).astype(np.float64)
+        closed = np.mean(np.sum(x * x, axis=1))
+        assert abs(lit - closed) <= 1e-12 * closed
+
+
+def _negative(mat, n=2):
+    return np.abs(np.mean(mat[mat < 0.0])) ** n
+
+
+@pytest.mark.parametrize("metric", ["autocorrelation", "entropy",
+                                    "sharpness", "custom"])
+def test_vertical_centre_search_on_oracle(metric):
+    """tests/test_rec/test_vertrec.py:498-526 (CPU branch) on the oracle."""
+    c = vertrec_case()
+    kw = dict(step=1.0, alpha=0.0, chunk_size=30, ramp_filter="after",
+              apply_log=True, gpu=False, invert_metric=True)
+    if metric == "custom":
+        cs, vs = vertrec_np.find_center_vertical(
+            c["projs"] + 1, c["width"] // 2, c["center"] - 5,
+            c["center"] + 9, masking=True, metric_function=_negative, n=2,
+            **kw)
+    else:
+        cs, vs = vertrec_np.find_center_vertical(
+            c["projs"] + 1, c["width"] // 2, c["center"] - 5,
+            c["center"] + 9, metric=metric, masking=False, **kw)
+    assert abs(cs[np.argmin(vs)] - c["center"]) < 0.5
