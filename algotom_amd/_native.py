@@ -59,6 +59,10 @@ SIGNATURES = (
     ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _I, _I, _I, _I,
                                              _P, _P, _P]),
     ("ath_minus_log", _I, [_P, _P, ctypes.c_longlong, _P]),
+    ("ath_extend_sinogram", _I, [_P, _I, _I, _I, _I, _I, _I, _P, _P]),
+    ("ath_flat_field_correction", _I, [_P, _I, ctypes.c_longlong, _I, _I,
+                                       ctypes.c_longlong, ctypes.c_longlong,
+                                       _P, _P, _I, _P, _P]),
     ("ath_flat_field_log", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong,
                                 ctypes.c_longlong, _P, _P, _I, _P, _P, _P]),
     ("ath_vertical_back_projection", _I, [_P, _I, _I, _I, _P, _P, _P, _I, _D,

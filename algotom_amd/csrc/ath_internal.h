@@ -127,6 +127,14 @@ int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
                    float* out, int* flag, hipStream_t s);
 int vertical_finish(void* data, int dtype, long long total, int width, int lo,
                     int hi, int nrow, hipStream_t s);
+int extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
+                    int overlap, int side, int apply_log, float* out,
+                    hipStream_t s);
+int flat_field_correction(const void* proj, int in_dtype, long long nouter,
+                          int height, int width, long long outer_stride,
+                          long long row_stride, const void* dark,
+                          const void* denom, int field_dtype, void* out,
+                          hipStream_t s);
 int minus_log(const float* in, float* out, long long n, hipStream_t s);
 int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s);

@@ -433,6 +433,44 @@ int ath_vertical_finish(void* data, int dtype, long long total, int width,
                            static_cast<hipStream_t>(stream));
 }
 
+int ath_extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
+                        int overlap, int side, int apply_log, float* out,
+                        void* stream) {
+    clear_error();
+    if (!sino || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (nrow < 1 || nslice < 1 || ncol < 1)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (overlap < 1 || overlap > ncol)
+        return set_error(ATH_ERR_INVALID_ARG, "overlap must be in [1, ncol]");
+    if (side != 0 && side != 1)
+        return set_error(ATH_ERR_INVALID_ARG, "side must be 0 or 1");
+    return extend_sinogram(sino, nrow, nslice, ncol, overlap, side, apply_log,
+                           out, static_cast<hipStream_t>(stream));
+}
+
+int ath_flat_field_correction(const void* proj, int in_dtype,
+                              long long nouter, int height, int width,
+                              long long outer_stride, long long row_stride,
+                              const void* dark, const void* denom,
+                              int field_dtype, void* out, void* stream) {
+    clear_error();
+    if (!proj || !denom || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (in_dtype != ATH_F32 && in_dtype != ATH_F64 && in_dtype != ATH_U16)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "projections must be float32, float64 or uint16");
+    if (field_dtype != ATH_F32 && field_dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG, "fields must be float32/64");
+    if (nouter < 1 || height < 1 || width < 1 || row_stride < width ||
+        outer_stride < (long long)(height - 1) * row_stride + width)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    return flat_field_correction(proj, in_dtype, nouter, height, width,
+                                 outer_stride, row_stride, dark, denom,
+                                 field_dtype, out,
+                                 static_cast<hipStream_t>(stream));
+}
+
 int ath_minus_log(const float* in, float* out, long long n, void* stream) {
     clear_error();
     if (!in || !out) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");

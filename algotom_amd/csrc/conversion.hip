@@ -114,6 +114,62 @@ __global__ __launch_bounds__(kThreads) void stitch_kernel(
     }
 }
 
+// numpy.linspace(0.0, 1.0, ov)[t]: t * step, the last value exactly 1
+__device__ __forceinline__ double ramp_up(int t, int ov) {
+    if (ov == 1) return 0.0;
+    if (t == ov - 1) return 1.0;
+    return (double)t * (1.0 / (double)(ov - 1)) + 0.0;
+}
+
+// extend_sinogram (conversion.py:395-454) for an integer overlap: (log),
+// the overlap columns weighted by a linspace ramp (float64 product rounded
+// to float32), edge padding to 2*ncol - overlap columns, times 2.
+// side 1: ramp 1 -> 0 over the last ov columns, padding on the right;
+// side 0: ramp 0 -> 1 over the first ov columns, padding on the left.
+__global__ __launch_bounds__(kThreads) void extend_kernel(
+    const float* __restrict__ sino, int nrow, int nslice, int ncol,
+    int overlap, int side, int apply_log, float* __restrict__ out) {
+    const int width = 2 * ncol - overlap;
+    const int pad = ncol - overlap;
+    const long long total = (long long)nrow * nslice * width;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        const int k = (int)(e % width);
+        const long long rj = e / width;
+        // source column of the padded array (edge padding)
+        int c = side == 1 ? min(k, ncol - 1) : max(k - pad, 0);
+        float v = sino[rj * ncol + c];
+        if (apply_log) v = (float)(-log((double)v));
+        if (side == 1 && c >= ncol - overlap)
+            v = (float)((double)v * ramp_down(c - (ncol - overlap), overlap));
+        else if (side == 0 && c < overlap)
+            v = (float)((double)v * ramp_up(c, overlap));
+        out[e] = 2.0f * v;
+    }
+}
+
+// flat_field_correction (correction.py:45-98): out = (float32(proj) - dark)
+// / denom, or float32(proj) / denom without dark, in the fields' type T
+// (float32 fields: float32 arithmetic; float64 fields: float64, as numpy
+// promotes).  dark / denom: (height, width), broadcast over nouter.
+template <typename Tin, typename T>
+__global__ __launch_bounds__(kThreads) void flat_field_kernel(
+    const Tin* __restrict__ proj, long long nouter, int height, int width,
+    long long outer_stride, long long row_stride, const T* __restrict__ dark,
+    const T* __restrict__ denom, T* __restrict__ out) {
+    const long long total = nouter * height * (long long)width;
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+         e < total; e += (long long)gridDim.x * kThreads) {
+        const int jx = (int)(e % width);
+        const long long t = e / width;
+        const int iy = (int)(t % height);
+        const long long n = t / height;
+        const T v = (T)(float)proj[n * outer_stride + iy * row_stride + jx];
+        const size_t p = (size_t)iy * width + jx;
+        out[e] = dark ? (v - dark[p]) / denom[p] : v / denom[p];
+    }
+}
+
 // out = -log(in) elementwise, float32 <- float64 log rounded (the filter
 // kernel's convention for the same step)
 __global__ __launch_bounds__(kThreads) void minus_log_kernel(
@@ -131,6 +187,53 @@ int minus_log(const float* in, float* out, long long n, hipStream_t s) {
     hipLaunchKernelGGL(minus_log_kernel, dim3((unsigned)blocks),
                        dim3(kThreads), 0, s, in, out, n);
     return check_hip(hipGetLastError(), "minus_log_kernel launch");
+}
+
+int extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
+                    int overlap, int side, int apply_log, float* out,
+                    hipStream_t s) {
+    const long long total = (long long)nrow * nslice * (2LL * ncol - overlap);
+    const long long blocks =
+        std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20);
+    hipLaunchKernelGGL(extend_kernel, dim3((unsigned)blocks), dim3(kThreads),
+                       0, s, sino, nrow, nslice, ncol, overlap, side,
+                       apply_log, out);
+    return check_hip(hipGetLastError(), "extend_kernel launch");
+}
+
+template <typename Tin, typename T>
+static int launch_flat(const void* proj, long long nouter, int height,
+                       int width, long long outer_stride, long long row_stride,
+                       const void* dark, const void* denom, void* out,
+                       hipStream_t s) {
+    const long long total = nouter * height * (long long)width;
+    const long long blocks = std::max<long long>(
+        1, std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20));
+    hipLaunchKernelGGL((flat_field_kernel<Tin, T>), dim3((unsigned)blocks),
+                       dim3(kThreads), 0, s, static_cast<const Tin*>(proj),
+                       nouter, height, width, outer_stride, row_stride,
+                       static_cast<const T*>(dark),
+                       static_cast<const T*>(denom), static_cast<T*>(out));
+    return check_hip(hipGetLastError(), "flat_field_kernel launch");
+}
+
+int flat_field_correction(const void* proj, int in_dtype, long long nouter,
+                          int height, int width, long long outer_stride,
+                          long long row_stride, const void* dark,
+                          const void* denom, int field_dtype, void* out,
+                          hipStream_t s) {
+    const bool f64 = field_dtype == ATH_F64;
+#define ATH_FLAT(TIN)                                                         \
+    return f64 ? launch_flat<TIN, double>(proj, nouter, height, width,        \
+                                          outer_stride, row_stride, dark,     \
+                                          denom, out, s)                      \
+               : launch_flat<TIN, float>(proj, nouter, height, width,         \
+                                         outer_stride, row_stride, dark,      \
+                                         denom, out, s);
+    if (in_dtype == ATH_U16) { ATH_FLAT(unsigned short) }
+    if (in_dtype == ATH_F64) { ATH_FLAT(double) }
+    ATH_FLAT(float)
+#undef ATH_FLAT
 }
 
 int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,

@@ -83,3 +83,55 @@ def convert_sinogram_360_to_180(sino_360, cor, wei_mat1=None, wei_mat2=None,
     if as_numpy:
         out = out.cpu().numpy()
     return out, new_cor
+
+
+def extend_sinogram(sino_360, cor, apply_log=True):
+    """Extend a 360-degree sinogram with an offset centre of rotation for
+    reconstruction (conversion.py:395-454): (log), linear weights over the
+    overlap, edge padding to 2*ncol - overlap columns, times 2.
+
+    Parameters
+    ----------
+    sino_360 : array_like
+        2D (nrow, ncol) or 3D (nrow, nslice, ncol) 360-degree sinogram(s);
+        numpy or a torch tensor on the GPU.
+    cor : float or tuple of float
+        Center-of-rotation or (Overlap_area, overlap_side).
+    apply_log : bool, optional
+        Apply the logarithm function if True.
+
+    Returns
+    -------
+    sino_pad : array_like
+        Extended sinogram(s), float32.
+    cor : float
+        Updated center-of-rotation referred to the converted sinogram.
+    """
+    import torch
+    from algotom_amd.rec import reconstruction as rec
+    shape = tuple(sino_360.shape)
+    two_d = len(shape) == 2
+    nrow, ncol = shape[0], shape[-1]
+    overlap, side = _overlap_side(ncol, cor)
+    overlap_int = int(np.round(overlap + 1.0e-3))
+    if overlap - overlap_int != 0.0:
+        raise NotImplementedError(
+            "sub-pixel overlap %r needs a spline shift; use a centre at x.5 "
+            "or an integer overlap" % (overlap,))
+    dev = nat.device()
+    as_numpy = not rec._is_tensor(sino_360)
+    src = rec._to_device(sino_360, dev, allow_f64=False).reshape(nrow, -1,
+                                                                 ncol)
+    nslice = src.shape[1]
+    out = torch.empty((nrow, nslice, 2 * ncol - overlap_int),
+                      dtype=torch.float32, device=dev)
+    nat.check(nat.load().ath_extend_sinogram(
+        nat.ptr(src), nrow, nslice, ncol, overlap_int, int(side),
+        int(bool(apply_log)), nat.ptr(out), nat.stream_ptr(dev)),
+        "ath_extend_sinogram")
+    new_cor = ncol - overlap_int / 2.0 - 0.5
+    if two_d:
+        out = out[:, 0, :]
+    if as_numpy:
+        out = out.cpu().numpy()
+    return out, new_cor

@@ -240,6 +240,31 @@ int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
                                     int side, int norm, int width,
                                     float* factors, float* out, void* stream);
 
+/* extend_sinogram (algotom/prep/conversion.py:395-454) for an integer
+ * overlap: sino (nrow, nslice, ncol) float32 -> out (nrow, nslice,
+ * 2*ncol - overlap) float32: optional -log, the overlap columns weighted
+ * by numpy.linspace(1, 0, overlap) (side 1, last columns) or
+ * linspace(0, 1, overlap) (side 0, first columns), edge padding on the
+ * other side, times 2. */
+int ath_extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
+                        int overlap, int side, int apply_log, float* out,
+                        void* stream);
+
+/* flat_field_correction (algotom/prep/correction.py:45-98) without its
+ * artifact-removal options: out = (float32(proj) - dark) / denom, or
+ * float32(proj) / denom when dark is NULL.  proj element (n, i, j) at
+ * proj[n*outer_stride + i*row_stride + j] (float32, float64 or ATH_U16;
+ * n < nouter, i < height, j < width); dark / denom (height, width) of
+ * field_dtype ATH_F32 or ATH_F64, which is also the type of the arithmetic
+ * and of the contiguous out, as numpy's promotion makes it.  denom is the
+ * caller's ratio*flat - dark (or ratio*flat) with zeros replaced as the
+ * reference does (:78-98). */
+int ath_flat_field_correction(const void* proj, int in_dtype,
+                              long long nouter, int height, int width,
+                              long long outer_stride, long long row_stride,
+                              const void* dark, const void* denom,
+                              int field_dtype, void* out, void* stream);
+
 /* out = -log(in) for n float32 values (in == out allowed), the log taken
  * in float64 and rounded.  Replaces the apply-log step of
  * bpf_reconstruction (reconstruction.py:634-639), which back-projects the

@@ -68,3 +68,45 @@ def convert_360_to_180(sino, cor, norm=True, total_width=None):
     assert ov == overlap, "integer overlaps only"
     out = stitch(top, bot, ov, side, norm, total_width)
     return out, ncol - ov / 2.0 - 0.5
+
+
+def extend(sino_360, cor, apply_log=True):
+    """extend_sinogram (conversion.py:395-454) for an integer overlap,
+    2D float32 input; log in float64 rounded (the oracle's convention)."""
+    s = np.asarray(sino_360, dtype=np.float32)
+    if apply_log:
+        s = (-np.log(s.astype(np.float64))).astype(np.float32)
+    else:
+        s = s.copy()
+    nrow, ncol = s.shape
+    xc = (ncol - 1.0) * 0.5
+    if isinstance(cor, tuple):
+        overlap, side = cor
+    elif cor <= xc:
+        overlap, side = 2 * cor + 1, 0
+    else:
+        overlap, side = 2 * (ncol - cor) - 1, 1
+    ov = int(np.round(overlap + 1.0e-3))
+    pad = ncol - ov
+    if side == 1:
+        s[:, -ov:] = s[:, -ov:] * np.linspace(1.0, 0.0, ov)[None, :]
+        out = np.pad(s, ((0, 0), (0, pad)), mode="edge")
+    else:
+        s[:, :ov] = s[:, :ov] * np.linspace(0.0, 1.0, ov)[None, :]
+        out = np.pad(s, ((0, 0), (pad, 0)), mode="edge")
+    return 2 * out, ncol - ov / 2.0 - 0.5
+
+
+def flat_field(proj, flat, dark, ratio=1.0, use_dark=True):
+    """flat_field_correction (correction.py:45-98) without options."""
+    flat = ratio * np.asarray(flat)
+    if use_dark:
+        den = flat - dark
+        if 0.0 in den:
+            m = np.mean(den)
+            den[den == 0.0] = m if m != 0.0 else 1
+        return (np.float32(proj) - dark) / den
+    if 0.0 in flat:
+        m = np.mean(flat)
+        flat[flat == 0.0] = m if m != 0.0 else 1
+    return np.float32(proj) / flat
