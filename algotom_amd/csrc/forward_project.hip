@@ -30,9 +30,12 @@ __device__ __forceinline__ double fp_det_pos(double xc_c, double yc, double s,
 // angles' candidate windows over the 64 (+2) bins is staged into LDS once
 // (line-major, FSB slices interleaved) and every lane then gathers its
 // candidates from LDS.  Each staged pixel serves up to 4 angles x 2 bins.
-// Chunks whose union window is wider than kMaxMinor columns (only for
-// angles close to 45 degrees with very small images) read global memory.
-constexpr int kMaxMinor = 192;
+// Chunks whose union window is wider than kMaxMinor columns read global
+// memory instead.  The walking axis keeps |slope| >= ~0.69, so 64 bins span
+// <= 93 columns plus the margins and the 4 angles' spread: 128 covers every
+// block of closely spaced angles, and the small tile (16 KB at 8 slices x 4
+// lines) lets 6+ workgroups share a CU to hide the staging loads.
+constexpr int kMaxMinor = 128;
 // below this |slope| a line runs (almost) along the ray: take it whole
 constexpr double kMinSlope = 1.0 / 1024.0;
 
@@ -41,7 +44,8 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     FpArgs a, int by_rows_block) {
     __shared__ float tile[kRows * kMaxMinor * FSB];
     __shared__ double s_cs[4][2];
-    __shared__ int s_win[2];
+    __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
+    __shared__ double s_u[16];    // window ends: (angle, line end, bin end)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
     const int d0 = blockIdx.x * 64;
@@ -64,6 +68,10 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     const bool by_rows = by_rows_block < 0
                              ? fabs(s_cs[0][0]) >= fabs(s_cs[0][1])
                              : by_rows_block != 0;
+    if (tid < 4) {
+        const double sl = by_rows ? s_cs[tid][0] : s_cs[tid][1];
+        s_inv[tid] = fabs(sl) >= kMinSlope ? 1.0 / sl : 0.0;
+    }
     const double slope = by_rows ? c : s;
     // windows only need to be a superset (every candidate is tested with
     // the exact f), so a reciprocal replaces the per-line divisions
@@ -76,50 +84,34 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     for (int m0 = 0; m0 < C; m0 += kRows) {
         const int m1 = min(m0 + kRows, C) - 1;
         // union window of the minor coordinate over 4 angles, bins
-        // [d0 - 1, d0 + 64] and lines m0, m1 (windows are linear in m)
-        if (tid == 0) {
-            double lo = 1e300, hi = -1e300;
-            for (int q = 0; q < 4; ++q) {
-                const double cq = s_cs[q][0], sq = s_cs[q][1];
-                const double sl = by_rows ? cq : sq;
-                if (fabs(sl) < kMinSlope) {  // line nearly along the ray
-                    lo = -1e300;
-                    hi = 1e300;
-                    continue;
-                }
-                for (int e = 0; e < 2; ++e) {
-                    const double mc = (double)(e ? m1 : m0) - ic;
-                    const double base = (by_rows ? mc * sq : mc * cq) + cen;
-                    const double u0 = ((double)d0 - 2.0 - base) / sl;
-                    const double u1 = ((double)d0 + 65.0 - base) / sl;
-                    lo = fmin(lo, fmin(u0, u1));
-                    hi = fmax(hi, fmax(u0, u1));
-                }
-            }
-            s_win[0] = lo <= -1e299 ? 0 : max((int)floor(lo + ic) - 1, 0);
-            s_win[1] = hi >= 1e299 ? C - 1 : min((int)ceil(hi + ic) + 1, C - 1);
+        // [d0 - 1, d0 + 64] and lines m0, m1 (windows are linear in m):
+        // 16 threads each evaluate one end, every thread takes min / max
+        if (tid < 16) {
+            const int q = tid >> 2, e = (tid >> 1) & 1, b = tid & 1;
+            const double mc = (double)(e ? m1 : m0) - ic;
+            const double base =
+                (by_rows ? mc * s_cs[q][1] : mc * s_cs[q][0]) + cen;
+            const double bin = b ? (double)d0 + 65.0 : (double)d0 - 2.0;
+            // a near-parallel line (inverse 0) takes the whole line
+            s_u[tid] = s_inv[q] != 0.0 ? (bin - base) * s_inv[q]
+                                       : (b ? 1e300 : -1e300);
         }
         __syncthreads();
-        const int ka = s_win[0], kb = s_win[1];
+        double lo = 1e300, hi = -1e300;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            lo = fmin(lo, s_u[t]);
+            hi = fmax(hi, s_u[t]);
+        }
+        // a negative inverse swaps the two bin ends: min / max over all
+        // sixteen covers both orders
+        const int ka = lo <= -1e299 ? 0 : max((int)floor(lo + ic) - 1, 0);
+        const int kb = hi >= 1e299 ? C - 1 : min((int)ceil(hi + ic) + 1, C - 1);
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
         if (wid > 0 && staged) {
             const int nrows = m1 - m0 + 1;
-            const int total = nrows * wid * FSB;
-            for (int e = tid; e < total; e += 256) {
-                // coalesced along x for rows, along the slice-row for columns
-                int js, k, r;
-                if (by_rows) {
-                    k = e % wid;
-                    const int t = e / wid;
-                    js = t % FSB;
-                    r = t / FSB;
-                } else {
-                    r = e % nrows;
-                    const int t = e / nrows;
-                    js = t % FSB;
-                    k = t / FSB;
-                }
+            auto stage = [&](int r, int js, int k) {
                 const int x = by_rows ? ka + k : m0 + r;
                 const int y = by_rows ? m0 + r : ka + k;
                 float v = 0.0f;
@@ -132,6 +124,22 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
                     }
                 }
                 tile[(r * kMaxMinor + k) * FSB + js] = v;
+            };
+            if (by_rows) {
+                // lanes along x (contiguous in the image)
+                for (int rj = wv; rj < nrows * FSB; rj += 4) {
+                    const int r = rj / FSB, js = rj % FSB;
+                    for (int k = lane; k < wid; k += 64) stage(r, js, k);
+                }
+            } else {
+                // lanes along the line index (x contiguous); kRows and FSB
+                // are compile-time, so the index split is shifts
+                for (int e = tid; e < kRows * FSB * wid; e += 256) {
+                    const int r = e % kRows;
+                    const int t = e / kRows;
+                    const int js = t % FSB, k = t / FSB;
+                    if (r < nrows) stage(r, js, k);
+                }
             }
         }
         __syncthreads();
@@ -149,8 +157,12 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
                         u0 = u1;
                         u1 = t;
                     }
-                    k0 = max((int)floor(u0 + ic) - 1, 0);
-                    k1 = min((int)ceil(u1 + ic) + 1, C - 1);
+                    // contributors have f in [d-1, d+1), i.e. u in
+                    // [u0, u1]; u carries a relative error of a few ulp
+                    // (reciprocal slope), so a 1e-6 px margin keeps the
+                    // candidate list a superset (each is tested exactly)
+                    k0 = max((int)floor(u0 + ic - 1e-6), 0);
+                    k1 = min((int)ceil(u1 + ic + 1e-6), C - 1);
                 }
                 if (staged) {
                     k0 = max(k0, ka);
@@ -211,14 +223,20 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
 int forward_projection(const FpArgs& a, hipStream_t stream) {
     // host copy of the walking direction per block needs the angles: they
     // live on the device, so the kernel decides per block instead
-    const int fsb = a.nslice >= 8 ? 8 : (a.nslice >= 4 ? 4 : 1);
+    // slices per workgroup and lines per chunk measured on MI355X
+    // (16 x 901 x 1024: 16/4 15.3 ms, 8/4 16.0 ms, 8/8 20.9 ms)
+    const int fsb = a.nslice >= 16 ? 16
+                  : a.nslice >= 8 ? 8 : (a.nslice >= 4 ? 4 : 1);
     const dim3 grid((a.ncol + 63) / 64, (a.nrow + 3) / 4,
                     (a.nslice + fsb - 1) / fsb);
-    if (fsb == 8)
-        hipLaunchKernelGGL((forward_project_lds_kernel<8, 8>), grid,
+    if (fsb == 16)
+        hipLaunchKernelGGL((forward_project_lds_kernel<16, 4>), grid,
+                           dim3(256), 0, stream, a, -1);
+    else if (fsb == 8)
+        hipLaunchKernelGGL((forward_project_lds_kernel<8, 4>), grid,
                            dim3(256), 0, stream, a, -1);
     else if (fsb == 4)
-        hipLaunchKernelGGL((forward_project_lds_kernel<4, 16>), grid,
+        hipLaunchKernelGGL((forward_project_lds_kernel<4, 8>), grid,
                            dim3(256), 0, stream, a, -1);
     else
         hipLaunchKernelGGL((forward_project_lds_kernel<1, 16>), grid,
