@@ -33,13 +33,35 @@ __device__ __forceinline__ double2 rot(double2 a) {
     return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
+// Quarter twiddle table for a length-M transform: tq[m] = exp(-2 pi i m/M)
+// for m < M/4, taken from the plan's length-2M table (tw[2m]).  All threads.
+__device__ __forceinline__ void stage_quarter(double2* tq,
+                                              const double2* __restrict__ tw,
+                                              int M) {
+    for (int m = threadIdx.x; m < (M >> 2); m += kFftThreads) tq[m] = tw[2 * m];
+}
+
+// exp(-2 pi i j/M) for j < M/2 from the quarter table: the second quarter
+// is the first turned by -i (exact).
+__device__ __forceinline__ double2 twiddle_half(const double2* tq, int j,
+                                                int M) {
+    const int q = M >> 2;
+    if (j < q) return tq[j];
+    const double2 t = tq[j - q];
+    return make_double2(t.y, -t.x);
+}
+
 // In-LDS complex FFT of length M = 2^m, natural order in and out (Stockham
 // autosort: each pass reads all operands into registers, barriers, then
 // writes them to their sorted place in the same buffer).  DIR = -1 is the
 // forward transform, +1 the unscaled inverse.  tw[n] = exp(-2 pi i n / L),
 // L = 2M.  Each of the 256 threads handles up to MAXB butterflies a pass.
-template <int DIR, int MAXB>
-__device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw) {
+// A radix-4 butterfly fetches one twiddle w and forms w^2, w^3 by complex
+// products (one table access instead of three).  With QT the twiddles come
+// from the LDS quarter table tq (stage_quarter) instead of global memory.
+template <int DIR, int MAXB, bool QT = false>
+__device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
+                        const double2* tq = nullptr) {
     const int tid = threadIdx.x;
     const int L = 2 * M;
     int ns = 1;
@@ -55,12 +77,13 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw) {
                 for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * nb];
                 const int k = j & (ns - 1);
                 if (k != 0) {
-#pragma unroll
-                    for (int r = 1; r < 4; ++r) {
-                        double2 w = tw[r * k * tstep];
-                        if (DIR > 0) w = conjg(w);
-                        v[b][r] = cmul(v[b][r], w);
-                    }
+                    double2 w1 = QT ? tq[k * (tstep >> 1)] : tw[k * tstep];
+                    if (DIR > 0) w1 = conjg(w1);
+                    const double2 w2 = cmul(w1, w1);
+                    const double2 w3 = cmul(w1, w2);
+                    v[b][1] = cmul(v[b][1], w1);
+                    v[b][2] = cmul(v[b][2], w2);
+                    v[b][3] = cmul(v[b][3], w3);
                 }
                 const double2 a0 = cadd(v[b][0], v[b][2]);
                 const double2 a1 = csub(v[b][0], v[b][2]);
@@ -96,7 +119,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw) {
                 v[b][0] = buf[j];
                 double2 x1 = buf[j + nb];
                 if (j != 0) {
-                    double2 w = tw[2 * j];
+                    double2 w = QT ? twiddle_half(tq, j, M) : tw[2 * j];
                     if (DIR > 0) w = conjg(w);
                     x1 = cmul(x1, w);
                 }
@@ -149,6 +172,26 @@ inline int maxb_for(int M) {
 // Allows dynamic LDS above 64 KB for `kernel`, once per kernel (a host-side
 // attribute, so later launches, including graph-captured ones, enqueue only
 // the kernel).  `bytes` is the largest size the kernel is launched with.
+// Workgroups of `kernel` resident on the device at once with `lds` bytes of
+// dynamic LDS each (cached per kernel): the grid of a persistent launch.
+template <typename K>
+int resident_blocks(K kernel, size_t lds) {
+    static int cached = 0;  // one value per kernel instantiation
+    if (cached > 0) return cached;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(kernel), kFftThreads,
+            lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    cached = std::max(1, per_cu) * std::max(1, cus);
+    return cached;
+}
+
 template <typename K>
 void prepare_lds(K kernel, size_t bytes) {
     static bool done = false;  // one flag per kernel instantiation

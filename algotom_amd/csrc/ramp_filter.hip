@@ -46,9 +46,10 @@ __device__ __forceinline__ double load_sample<double>(const double* row,
 // index map instead of cropped.
 template <int MAXB, typename Tin, typename Tout, bool ADJ = false>
 __device__ __forceinline__ void filter_row(
-    double2* buf, const Tin* __restrict__ src_row, Tout* __restrict__ dst,
-    int ncol, int pad, int pad_mode, int apply_log,
+    double2* buf, const double2* tq, const Tin* __restrict__ src_row,
+    Tout* __restrict__ dst, int ncol, int pad, int pad_mode, int apply_log,
     const double2* __restrict__ tw, const double2* __restrict__ G, int M) {
+    constexpr bool QT = MAXB <= 4;  // quarter table in LDS (M <= 4096)
     const int tid = threadIdx.x;
     const int width = ncol + 2 * pad;
     // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
@@ -67,7 +68,7 @@ __device__ __forceinline__ void filter_row(
         buf[n] = make_double2(v[0], v[1]);
     }
     __syncthreads();
-    lds_fft<-1, MAXB>(buf, M, tw);
+    lds_fft<-1, MAXB, QT>(buf, M, tw, tq);
 
     // rfft split, multiply by the kernel spectrum, merge for the inverse
     for (int k = tid; k <= M / 2; k += kThreads) {
@@ -93,7 +94,7 @@ __device__ __forceinline__ void filter_row(
         }
     }
     __syncthreads();
-    lds_fft<1, MAXB>(buf, M, tw);
+    lds_fft<1, MAXB, QT>(buf, M, tw, tq);
 
     const double* y = reinterpret_cast<const double*>(buf);
     if (!ADJ) {
@@ -123,57 +124,80 @@ __device__ __forceinline__ void filter_row(
     __syncthreads();  // buf free for the caller's next row
 }
 
-// One workgroup per row (angle i, slice j) of a (nrow, nslice, ncol) stack.
+// LDS of the row kernels: the row (M complex) then, for M <= 4096, the
+// quarter twiddle table (M/4 complex).
+template <int MAXB>
+constexpr size_t row_lds_bytes(int M) {
+    return (size_t)(M + (MAXB <= 4 ? M / 4 : 0)) * sizeof(double2);
+}
+
+// Persistent: each workgroup stages the twiddles once, then filters rows
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the (nrow, nslice, ncol) stack.
 template <int MAXB, typename Tin, typename Tout>
 __global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
-    const Tin* __restrict__ sino, Tout* __restrict__ out, int nslice,
-    int ncol, int pad, int pad_mode, int apply_log,
+    const Tin* __restrict__ sino, Tout* __restrict__ out, size_t rows,
+    int nslice, int ncol, int pad, int pad_mode, int apply_log,
     const double2* __restrict__ tw, const double2* __restrict__ spec,
     int nwin, int M) {
     extern __shared__ double2 buf[];
-    const size_t row = blockIdx.x;
-    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
-    filter_row<MAXB, Tin, Tout>(buf, sino + row * (size_t)ncol,
-                                out + row * (size_t)ncol, ncol, pad, pad_mode,
-                                apply_log, tw, G, M);
+    double2* tq = buf + M;
+    if (MAXB <= 4) stage_quarter(tq, tw, M);  // read after the row's barrier
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const double2* G =
+            spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+        filter_row<MAXB, Tin, Tout>(buf, tq, sino + row * (size_t)ncol,
+                                    out + row * (size_t)ncol, ncol, pad,
+                                    pad_mode, apply_log, tw, G, M);
+    }
 }
 
-// Transpose of ramp_filter_kernel (float32 in and out): one workgroup per row.
+// Transpose of ramp_filter_kernel (float32 in and out), persistent.
 template <int MAXB>
 __global__ __launch_bounds__(kThreads) void ramp_filter_adjoint_kernel(
-    const float* __restrict__ in, float* __restrict__ out, int nslice,
-    int ncol, int pad, int pad_mode, const double2* __restrict__ tw,
-    const double2* __restrict__ spec, int nwin, int M) {
+    const float* __restrict__ in, float* __restrict__ out, size_t rows,
+    int nslice, int ncol, int pad, int pad_mode,
+    const double2* __restrict__ tw, const double2* __restrict__ spec,
+    int nwin, int M) {
     extern __shared__ double2 buf[];
-    const size_t row = blockIdx.x;
-    const double2* G = spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
-    filter_row<MAXB, float, float, true>(buf, in + row * (size_t)ncol,
-                                         out + row * (size_t)ncol, ncol, pad,
-                                         pad_mode, 0, tw, G, M);
+    double2* tq = buf + M;
+    if (MAXB <= 4) stage_quarter(tq, tw, M);
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const double2* G =
+            spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+        filter_row<MAXB, float, float, true>(buf, tq, in + row * (size_t)ncol,
+                                             out + row * (size_t)ncol, ncol,
+                                             pad, pad_mode, 0, tw, G, M);
+    }
 }
 
-// One workgroup per row of many stacks sharing the FFT length L = 2M.
+// Rows of many stacks sharing the FFT length L = 2M (hence the twiddles),
+// persistent; a row's stack is found by binary search over row_begin.
 template <int MAXB, typename Tin, typename Tout>
 __global__ __launch_bounds__(kThreads) void ramp_filter_batched_kernel(
-    const ath_filter_job* __restrict__ jobs, int njobs, int pad_mode,
-    int apply_log, int M) {
+    const ath_filter_job* __restrict__ jobs, int njobs, long long total_rows,
+    int pad_mode, int apply_log, int M) {
     extern __shared__ double2 buf[];
-    const long long grow = blockIdx.x;
-    int lo = 0, hi = njobs - 1;  // last job with row_begin <= grow
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].row_begin <= grow) lo = mid;
-        else hi = mid - 1;
+    double2* tq = buf + M;
+    if (MAXB <= 4)
+        stage_quarter(tq, static_cast<const double2*>(jobs[0].plan), M);
+    for (long long grow = blockIdx.x; grow < total_rows; grow += gridDim.x) {
+        int lo = 0, hi = njobs - 1;  // last job with row_begin <= grow
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].row_begin <= grow) lo = mid;
+            else hi = mid - 1;
+        }
+        const ath_filter_job jb = jobs[lo];
+        const size_t row = (size_t)(grow - jb.row_begin);
+        const double2* tw = static_cast<const double2*>(jb.plan);
+        const double2* G =
+            tw + 2 * M +
+            (jb.nwin > 1 ? (row / jb.nslice) : 0) * (size_t)(M + 1);
+        filter_row<MAXB, Tin, Tout>(
+            buf, tq, static_cast<const Tin*>(jb.sino) + row * (size_t)jb.ncol,
+            static_cast<Tout*>(jb.out) + row * (size_t)jb.ncol, jb.ncol,
+            jb.pad, pad_mode, apply_log, tw, G, M);
     }
-    const ath_filter_job jb = jobs[lo];
-    const size_t row = (size_t)(grow - jb.row_begin);
-    const double2* tw = static_cast<const double2*>(jb.plan);
-    const double2* G = tw + 2 * M +
-                       (jb.nwin > 1 ? (row / jb.nslice) : 0) * (size_t)(M + 1);
-    filter_row<MAXB, Tin, Tout>(
-        buf, static_cast<const Tin*>(jb.sino) + row * (size_t)jb.ncol,
-        static_cast<Tout*>(jb.out) + row * (size_t)jb.ncol, jb.ncol, jb.pad,
-        pad_mode, apply_log, tw, G, M);
 }
 
 // spec[r][k] = rfft_L([g_r, g_r, 0...])[k] / M, k = 0..M, one block per row.
@@ -213,14 +237,17 @@ int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
     const int M = fft_len / 2;
     const double2* tw = static_cast<const double2*>(plan);
     const double2* spec = tw + fft_len;
-    const size_t lds = (size_t)M * sizeof(double2);
+    const size_t lds = row_lds_bytes<MAXB>(M);
     auto kern = ramp_filter_kernel<MAXB, Tin, Tout>;
-    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
+    prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
     const size_t rows = (size_t)nrow * nslice;
-    hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kThreads), lds, stream,
+    const size_t grid =
+        std::min(rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(
+                                                         1024 * MAXB)));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds, stream,
                        static_cast<const Tin*>(sino), static_cast<Tout*>(out),
-                       nslice, ncol, pad, pad_mode, apply_log, tw, spec, nwin,
-                       M);
+                       rows, nslice, ncol, pad, pad_mode, apply_log, tw, spec,
+                       nwin, M);
     return check_hip(hipGetLastError(), "ramp_filter_kernel launch");
 }
 
@@ -254,11 +281,14 @@ int launch_filter_batched(const ath_filter_job* jobs, int njobs,
                           long long total_rows, int fft_len, int pad_mode,
                           int apply_log, hipStream_t stream) {
     const int M = fft_len / 2;
-    const size_t lds = (size_t)M * sizeof(double2);
+    const size_t lds = row_lds_bytes<MAXB>(M);
     auto kern = ramp_filter_batched_kernel<MAXB, Tin, Tout>;
-    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
-    hipLaunchKernelGGL(kern, dim3((unsigned)total_rows), dim3(kThreads), lds,
-                       stream, jobs, njobs, pad_mode, apply_log, M);
+    prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
+    const long long grid = std::min<long long>(
+        total_rows, resident_blocks(kern, row_lds_bytes<MAXB>(1024 * MAXB)));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds,
+                       stream, jobs, njobs, total_rows, pad_mode, apply_log,
+                       M);
     return check_hip(hipGetLastError(), "ramp_filter_batched_kernel launch");
 }
 
@@ -386,13 +416,16 @@ int launch_adjoint(const float* in, int nrow, int nslice, int ncol, int pad,
     const int M = fft_len / 2;
     const double2* tw = static_cast<const double2*>(plan);
     const double2* spec = tw + fft_len;
-    const size_t lds = (size_t)M * sizeof(double2);
+    const size_t lds = row_lds_bytes<MAXB>(M);
     auto kern = ramp_filter_adjoint_kernel<MAXB>;
-    prepare_lds(kern, (size_t)1024 * MAXB * sizeof(double2));
+    prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
     const size_t rows = (size_t)nrow * nslice;
-    hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kThreads), lds, stream,
-                       in, out, nslice, ncol, pad, pad_mode, tw, spec, nwin,
-                       M);
+    const size_t grid =
+        std::min(rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(
+                                                         1024 * MAXB)));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds, stream,
+                       in, out, rows, nslice, ncol, pad, pad_mode, tw, spec,
+                       nwin, M);
     return check_hip(hipGetLastError(), "ramp_filter_adjoint_kernel launch");
 }
 }  // namespace
