@@ -44,14 +44,28 @@ struct VertArgs {
     int edge_pad, chunk, accumulate;
 };
 
+// Workgroup order (1D grid): id = (q * nslice + line) * 8 + x8 with tile
+// t = 8q + x8.  Workgroups are dealt to the 8 XCDs round-robin by id, so
+// every line of one tile lands on the same XCD (id % 8 = x8) and is
+// dispatched within 8 * nslice ids of the others: the lines, which read
+// overlapping columns of the same projection rows, share that XCD's L2
+// instead of each streaming the stack from HBM.
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a) {
+__global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a,
+                                                               int tiles_x,
+                                                               int tiles) {
     __shared__ double s_c[kAngBlock], s_s[kAngBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int j = blockIdx.x * kPts + lane;
-    const int i0 = blockIdx.y * (4 * kRows) + wave;
-    const int sl = blockIdx.z;
+    const unsigned id = blockIdx.x;
+    const int x8 = id & 7;
+    const unsigned rest = id >> 3;
+    const int sl = (int)(rest % (unsigned)a.nslice);
+    const int tile = (int)(rest / (unsigned)a.nslice) * 8 + x8;
+    if (tile >= tiles) return;  // padding of the last group of 8 tiles
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int j = tx * kPts + lane;
+    const int i0 = ty * (4 * kRows) + wave;
     const bool valid_j = j < a.width;
     const int width1 = a.width - 1;
     double x = 0.0, y = 0.0;
@@ -227,14 +241,18 @@ int vertical_back_projection(const float* proj, int nproj, int height,
                              float* out, hipStream_t s) {
     VertArgs a{proj, angles, xs, ys, centers, out, nproj, height, width,
                nslice, center, edge_pad, chunk, accumulate};
-    const dim3 grid((width + kPts - 1) / kPts,
-                    (height + 4 * kRows - 1) / (4 * kRows), nslice);
+    const int tiles_x = (width + kPts - 1) / kPts;
+    const int tiles = tiles_x * ((height + 4 * kRows - 1) / (4 * kRows));
+    const long long blocks = (long long)((tiles + 7) / 8) * 8 * nslice;
+    if (blocks > 0x7fffffffLL)
+        return set_error(ATH_ERR_UNSUPPORTED, "vertical grid too large");
+    const dim3 grid((unsigned)blocks);
     if (mode == kVertGpu)
         hipLaunchKernelGGL(vertical_bp_kernel<kVertGpu>, grid, dim3(kThreads),
-                           0, s, a);
+                           0, s, a, tiles_x, tiles);
     else
         hipLaunchKernelGGL(vertical_bp_kernel<kVertCpu>, grid, dim3(kThreads),
-                           0, s, a);
+                           0, s, a, tiles_x, tiles);
     return check_hip(hipGetLastError(), "vertical_bp_kernel launch");
 }
 
