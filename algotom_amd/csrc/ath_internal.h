@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -54,6 +55,19 @@ __host__ __device__ __forceinline__ int pad_source(int i, int n, int mode) {
         return j < n ? j : p - 1 - j; }
     default: return -1;
     }
+}
+
+// Launch shape of a rows x width elementwise pass: grid.x spans the width in
+// `threads`-wide blocks and grid.y strides over the rows, so splitting a row
+// index into its parts is block-uniform (scalar unit, once per row) and no
+// lane divides a 64-bit element index.  Kernels loop
+// `for (r = blockIdx.y; r < rows; r += gridDim.y)` and
+// `for (k = blockIdx.x * threads + threadIdx.x; k < width; k += gridDim.x * threads)`.
+inline dim3 row_grid(long long rows, int width, int threads) {
+    const long long gx = width > 0 ? (width + threads - 1) / threads : 1;
+    long long gy = rows < 1 ? 1 : (rows < 65535 ? rows : 65535);
+    if (gx * gy > (1LL << 22)) gy = std::max<long long>(1, (1LL << 22) / gx);
+    return dim3((unsigned)std::min<long long>(gx, 1LL << 16), (unsigned)gy);
 }
 
 // ---- ramp filter (ramp_filter.hip) ----------------------------------------
@@ -113,6 +127,18 @@ int circular_ramp_filter(const float* images, int nimg, int ncol,
                          void* out, int out_dtype, long long out_row_stride,
                          long long out_img_stride, double mask_radius,
                          int scale_nrow, hipStream_t s);
+size_t dfi_plan_bytes(int width);
+int dfi_plan_build(const double* window, int width, void* plan,
+                   hipStream_t s);
+size_t dfi_workspace_bytes(int nrow_in, int ncol, int nrow, int width);
+int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
+                       int ncol, long long row_stride, long long slice_stride,
+                       double xshift, int pad, int pad_mode, int apply_log,
+                       int angle_mode, double row_shift, int flip, int nrow,
+                       const void* plan, int width, void* workspace,
+                       void* out, int out_dtype, long long out_row_stride,
+                       long long out_slice_stride, double mask_r2, int* flag,
+                       hipStream_t s);
 constexpr int kVertGpu = ATH_VERT_GPU;
 constexpr int kVertCpu = ATH_VERT_CPU;
 int vertical_back_projection(const float* proj, int nproj, int height,

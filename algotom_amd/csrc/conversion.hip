@@ -70,20 +70,20 @@ __global__ __launch_bounds__(kThreads) void stitch_kernel(
     const float* __restrict__ sino, int nrow, int nslice, int ncol,
     int nrow_180, int overlap, int side, int norm,
     const float* __restrict__ factors, int width, float* __restrict__ out) {
-    const long long total = (long long)nrow_180 * nslice * width;
+    const int rows = nrow_180 * nslice;
     const int w0 = 2 * ncol - overlap;
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        int k = (int)(e % width);
-        const long long rj = e / width;
-        const int j = (int)(rj % nslice), r = (int)(rj / nslice);
-        if (k >= w0) k = w0 - 1;  // edge extension to total_width
-        const float* top = sino + ((size_t)r * nslice + j) * ncol;
-        const float* bot =
-            sino + ((size_t)(nrow - nrow_180 + r) * nslice + j) * ncol;
-        // image 2 is scaled by factor1 / factor2 ((mat2 * f1) / f2)
-        const float f1 = norm ? factors[2 * j] : 1.0f;
-        const float f2 = norm ? factors[2 * j + 1] : 1.0f;
+    for (int rj = blockIdx.y; rj < rows; rj += gridDim.y) {
+      const int j = rj % nslice, r = rj / nslice;  // block-uniform
+      const float* top = sino + ((size_t)r * nslice + j) * ncol;
+      const float* bot =
+          sino + ((size_t)(nrow - nrow_180 + r) * nslice + j) * ncol;
+      // image 2 is scaled by factor1 / factor2 ((mat2 * f1) / f2)
+      const float f1 = norm ? factors[2 * j] : 1.0f;
+      const float f2 = norm ? factors[2 * j + 1] : 1.0f;
+      float* dst = out + (size_t)rj * width;
+      for (int kk = blockIdx.x * kThreads + threadIdx.x; kk < width;
+           kk += gridDim.x * kThreads) {
+        const int k = kk < w0 ? kk : w0 - 1;  // edge extension to width
         // first image (placed at 0) and second image (placed at ncol - ov)
         const int a_off = 0, b_off = ncol - overlap;
         // side 1: first = img1 (top), second = img2; side 0: swapped
@@ -110,7 +110,8 @@ __global__ __launch_bounds__(kThreads) void stitch_kernel(
             }
             v += x * wgt;
         }
-        out[e] = v;
+        dst[kk] = v;
+      }
     }
 }
 
@@ -131,20 +132,23 @@ __global__ __launch_bounds__(kThreads) void extend_kernel(
     int overlap, int side, int apply_log, float* __restrict__ out) {
     const int width = 2 * ncol - overlap;
     const int pad = ncol - overlap;
-    const long long total = (long long)nrow * nslice * width;
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        const int k = (int)(e % width);
-        const long long rj = e / width;
-        // source column of the padded array (edge padding)
-        int c = side == 1 ? min(k, ncol - 1) : max(k - pad, 0);
-        float v = sino[rj * ncol + c];
-        if (apply_log) v = (float)(-log((double)v));
-        if (side == 1 && c >= ncol - overlap)
-            v = (float)((double)v * ramp_down(c - (ncol - overlap), overlap));
-        else if (side == 0 && c < overlap)
-            v = (float)((double)v * ramp_up(c, overlap));
-        out[e] = 2.0f * v;
+    const long long rows = (long long)nrow * nslice;
+    for (long long rj = blockIdx.y; rj < rows; rj += gridDim.y) {
+        const float* src = sino + rj * ncol;
+        float* dst = out + rj * width;
+        for (int k = blockIdx.x * kThreads + threadIdx.x; k < width;
+             k += gridDim.x * kThreads) {
+            // source column of the padded array (edge padding)
+            const int c = side == 1 ? min(k, ncol - 1) : max(k - pad, 0);
+            float v = src[c];
+            if (apply_log) v = (float)(-log((double)v));
+            if (side == 1 && c >= ncol - overlap)
+                v = (float)((double)v *
+                            ramp_down(c - (ncol - overlap), overlap));
+            else if (side == 0 && c < overlap)
+                v = (float)((double)v * ramp_up(c, overlap));
+            dst[k] = 2.0f * v;
+        }
     }
 }
 
@@ -157,16 +161,19 @@ __global__ __launch_bounds__(kThreads) void flat_field_kernel(
     const Tin* __restrict__ proj, long long nouter, int height, int width,
     long long outer_stride, long long row_stride, const T* __restrict__ dark,
     const T* __restrict__ denom, T* __restrict__ out) {
-    const long long total = nouter * height * (long long)width;
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        const int jx = (int)(e % width);
-        const long long t = e / width;
-        const int iy = (int)(t % height);
-        const long long n = t / height;
-        const T v = (T)(float)proj[n * outer_stride + iy * row_stride + jx];
-        const size_t p = (size_t)iy * width + jx;
-        out[e] = dark ? (v - dark[p]) / denom[p] : v / denom[p];
+    const long long rows = nouter * height;
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+        const long long n = r / height;  // block-uniform
+        const int iy = (int)(r - n * height);
+        const Tin* src = proj + n * outer_stride + iy * row_stride;
+        const T* dk = dark ? dark + (size_t)iy * width : nullptr;
+        const T* dn = denom + (size_t)iy * width;
+        T* dst = out + r * width;
+        for (int jx = blockIdx.x * kThreads + threadIdx.x; jx < width;
+             jx += gridDim.x * kThreads) {
+            const T v = (T)(float)src[jx];
+            dst[jx] = dk ? (v - dk[jx]) / dn[jx] : v / dn[jx];
+        }
     }
 }
 
@@ -192,10 +199,9 @@ int minus_log(const float* in, float* out, long long n, hipStream_t s) {
 int extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
                     int overlap, int side, int apply_log, float* out,
                     hipStream_t s) {
-    const long long total = (long long)nrow * nslice * (2LL * ncol - overlap);
-    const long long blocks =
-        std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20);
-    hipLaunchKernelGGL(extend_kernel, dim3((unsigned)blocks), dim3(kThreads),
+    const dim3 grid =
+        row_grid((long long)nrow * nslice, 2 * ncol - overlap, kThreads);
+    hipLaunchKernelGGL(extend_kernel, grid, dim3(kThreads),
                        0, s, sino, nrow, nslice, ncol, overlap, side,
                        apply_log, out);
     return check_hip(hipGetLastError(), "extend_kernel launch");
@@ -206,10 +212,8 @@ static int launch_flat(const void* proj, long long nouter, int height,
                        int width, long long outer_stride, long long row_stride,
                        const void* dark, const void* denom, void* out,
                        hipStream_t s) {
-    const long long total = nouter * height * (long long)width;
-    const long long blocks = std::max<long long>(
-        1, std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20));
-    hipLaunchKernelGGL((flat_field_kernel<Tin, T>), dim3((unsigned)blocks),
+    const dim3 grid = row_grid(nouter * height, width, kThreads);
+    hipLaunchKernelGGL((flat_field_kernel<Tin, T>), grid,
                        dim3(kThreads), 0, s, static_cast<const Tin*>(proj),
                        nouter, height, width, outer_stride, row_stride,
                        static_cast<const T*>(dark),
@@ -247,10 +251,8 @@ int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
         const int st = check_hip(hipGetLastError(), "stitch_means_kernel");
         if (st != ATH_OK) return st;
     }
-    const long long total = (long long)nrow_180 * nslice * width;
-    const long long blocks = std::min<long long>((total + kThreads - 1) /
-                                                     kThreads, 1 << 20);
-    hipLaunchKernelGGL(stitch_kernel, dim3((unsigned)blocks), dim3(kThreads),
+    const dim3 grid = row_grid((long long)nrow_180 * nslice, width, kThreads);
+    hipLaunchKernelGGL(stitch_kernel, grid, dim3(kThreads),
                        0, s, sino, nrow, nslice, ncol, nrow_180, overlap, side,
                        norm, factors, width, out);
     return check_hip(hipGetLastError(), "stitch_kernel launch");

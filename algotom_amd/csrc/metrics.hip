@@ -321,10 +321,9 @@ __global__ __launch_bounds__(kThreads) void gaussian_inner_kernel(
 __global__ __launch_bounds__(kThreads) void zoom_linear_kernel(
     const float* __restrict__ in, int h, int w, float* __restrict__ out,
     int oh, int ow, double zy, double zx) {
-    const long long total = (long long)oh * ow;
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        const int i = (int)(e / ow), j = (int)(e % ow);
+    for (int i = blockIdx.y; i < oh; i += gridDim.y)
+      for (int j = blockIdx.x * kThreads + threadIdx.x; j < ow;
+           j += gridDim.x * kThreads) {
         const double yy = (double)i * zy, xx = (double)j * zx;
         int y0 = (int)floor(yy), x0 = (int)floor(xx);
         y0 = min(max(y0, 0), h - 1);
@@ -335,8 +334,8 @@ __global__ __launch_bounds__(kThreads) void zoom_linear_kernel(
         const double f10 = in[(size_t)y1 * w + x0], f11 = in[(size_t)y1 * w + x1];
         const double top = (1.0 - tx) * f00 + tx * f01;
         const double bot = (1.0 - tx) * f10 + tx * f11;
-        out[e] = (float)((1.0 - ty) * top + ty * bot);
-    }
+        out[(size_t)i * ow + j] = (float)((1.0 - ty) * top + ty * bot);
+      }
 }
 
 }  // namespace
@@ -354,10 +353,7 @@ int zoom_linear(const float* in, int h, int w, float* out, int oh, int ow,
                 hipStream_t s) {
     const double zy = oh > 1 ? (double)(h - 1) / (double)(oh - 1) : 1.0;
     const double zx = ow > 1 ? (double)(w - 1) / (double)(ow - 1) : 1.0;
-    const long long total = (long long)oh * ow;
-    const long long blocks =
-        std::min<long long>((total + kThreads - 1) / kThreads, 1 << 16);
-    hipLaunchKernelGGL(zoom_linear_kernel, dim3((unsigned)blocks),
+    hipLaunchKernelGGL(zoom_linear_kernel, row_grid(oh, ow, kThreads),
                        dim3(kThreads), 0, s, in, h, w, out, oh, ow, zy, zx);
     return check_hip(hipGetLastError(), "zoom_linear_kernel launch");
 }

@@ -178,26 +178,27 @@ __global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
     long long proj_stride, long long row_stride,
     const float* __restrict__ dark, const float* __restrict__ flat_dark,
     int apply_log, float* __restrict__ out, int* __restrict__ flag) {
-    const long long total = nproj * height * (long long)width;
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        const int jx = (int)(e % width);
-        const long long t = e / width;
-        const int iy = (int)(t % height);
-        const long long n = t / height;
-        float v = (float)proj[n * proj_stride + iy * row_stride + jx];
-        if (dark) {
-            const size_t p = (size_t)iy * width + jx;
-            v = (v - dark[p]) / flat_dark[p];
-        }
-        if (apply_log) {
-            if (v <= 0.0f) {
-                v = 1.0f;
-                *flag = 1;
+    const long long rows = nproj * height;
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+        const long long n = r / height;  // block-uniform
+        const int iy = (int)(r - n * height);
+        const Tin* src = proj + n * proj_stride + iy * row_stride;
+        float* dst = out + r * width;
+        const float* dk = dark ? dark + (size_t)iy * width : nullptr;
+        const float* fd = dark ? flat_dark + (size_t)iy * width : nullptr;
+        for (int jx = blockIdx.x * kThreads + threadIdx.x; jx < width;
+             jx += gridDim.x * kThreads) {
+            float v = (float)src[jx];
+            if (dk) v = (v - dk[jx]) / fd[jx];
+            if (apply_log) {
+                if (v <= 0.0f) {
+                    v = 1.0f;
+                    *flag = 1;
+                }
+                v = (float)(-log((double)v));
             }
-            v = (float)(-log((double)v));
+            dst[jx] = v;
         }
-        out[e] = v;
     }
 }
 
@@ -206,29 +207,26 @@ __global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
 // numpy's order (x * mask, * pi, / (nrow - 1)).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void vertical_finish_kernel(
-    T* __restrict__ data, long long total, int width, int lo, int hi,
+    T* __restrict__ data, long long rows, int width, int lo, int hi,
     int nrow) {
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-         e < total; e += (long long)gridDim.x * kThreads) {
-        const int jx = (int)(e % width);
-        T v = data[e];
-        if (lo >= 0) v = v * ((jx >= lo && jx < hi) ? T(1) : T(0));
-        if (nrow >= 2) {
-            if constexpr (sizeof(T) == 8) {
-                v = v * 3.141592653589793;
-                v = v / (double)(nrow - 1);
-            } else {
-                v = v * (float)3.141592653589793;
-                v = v / (float)(nrow - 1);
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+        T* row = data + r * width;
+        for (int jx = blockIdx.x * kThreads + threadIdx.x; jx < width;
+             jx += gridDim.x * kThreads) {
+            T v = row[jx];
+            if (lo >= 0) v = v * ((jx >= lo && jx < hi) ? T(1) : T(0));
+            if (nrow >= 2) {
+                if constexpr (sizeof(T) == 8) {
+                    v = v * 3.141592653589793;
+                    v = v / (double)(nrow - 1);
+                } else {
+                    v = v * (float)3.141592653589793;
+                    v = v / (float)(nrow - 1);
+                }
             }
+            row[jx] = v;
         }
-        data[e] = v;
     }
-}
-
-long long stream_blocks(long long total) {
-    return std::max<long long>(
-        1, std::min<long long>((total + kThreads - 1) / kThreads, 1 << 20));
 }
 
 }  // namespace
@@ -260,8 +258,7 @@ int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
                    int width, long long proj_stride, long long row_stride,
                    const float* dark, const float* flat_dark, int apply_log,
                    float* out, int* flag, hipStream_t s) {
-    const long long total = (long long)nproj * height * width;
-    const dim3 grid((unsigned)stream_blocks(total));
+    const dim3 grid = row_grid((long long)nproj * height, width, kThreads);
     if (in_dtype == ATH_U16)
         hipLaunchKernelGGL(flat_field_log_kernel<unsigned short>, grid,
                            dim3(kThreads), 0, s,
@@ -278,14 +275,16 @@ int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
 
 int vertical_finish(void* data, int dtype, long long total, int width, int lo,
                     int hi, int nrow, hipStream_t s) {
-    const dim3 grid((unsigned)stream_blocks(total));
+    if (width <= 0) return ATH_OK;
+    const long long rows = total / width;
+    const dim3 grid = row_grid(rows, width, kThreads);
     if (dtype == ATH_F64)
         hipLaunchKernelGGL(vertical_finish_kernel<double>, grid,
                            dim3(kThreads), 0, s, static_cast<double*>(data),
-                           total, width, lo, hi, nrow);
+                           rows, width, lo, hi, nrow);
     else
         hipLaunchKernelGGL(vertical_finish_kernel<float>, grid, dim3(kThreads),
-                           0, s, static_cast<float*>(data), total, width, lo,
+                           0, s, static_cast<float*>(data), rows, width, lo,
                            hi, nrow);
     return check_hip(hipGetLastError(), "vertical_finish_kernel launch");
 }
