@@ -22,6 +22,7 @@ ATH_ERR_WORKSPACE = -5
 
 ATH_F32, ATH_F64, ATH_F16, ATH_BF16, ATH_U16 = 0, 1, 2, 3, 4
 VERT_GPU, VERT_CPU = 0, 1
+DFI_NO_ANGLES, DFI_ANGLES, DFI_FOLD_360 = 0, 1, 2
 PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
              "wrap": 4}
 ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV = 0, 1, 2, 3
@@ -77,6 +78,14 @@ SIGNATURES = (
                                       ctypes.c_longlong, _I, _I, _P, _I, _P,
                                       _S, _P, _I, ctypes.c_longlong,
                                       ctypes.c_longlong, _D, _I, _P]),
+    ("ath_dfi_plan_bytes", _S, [_I]),
+    ("ath_dfi_plan_build", _I, [_P, _I, _P, _S, _P]),
+    ("ath_dfi_workspace_bytes", _S, [_I, _I, _I, _I]),
+    ("ath_dfi_reconstruction", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong,
+                                    ctypes.c_longlong, _D, _I, _I, _I, _I,
+                                    _D, _I, _I, _P, _I, _P, _S, _P, _I,
+                                    ctypes.c_longlong, ctypes.c_longlong, _D,
+                                    _P, _P]),
     ("ath_gaussian_filter_rows", _I, [_P, _P, _I, _I, _P, _I, _P]),
     ("ath_zoom_linear", _I, [_P, _I, _I, _P, _I, _I, _P]),
     ("ath_slice_metrics", _I, [_P, _I, _I, _I, _I, ctypes.c_longlong, _D, _D,

@@ -13,8 +13,7 @@ LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libalgotom_hip.so")
 SOURCES = ("capi.hip", "ramp_filter.hip", "backproject.hip",
            "forward_project.hip", "conversion.hip", "filter2d.hip",
-           "vertical.hip",
-           "metrics.hip")
+           "vertical.hip", "metrics.hip", "dfi.hip")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

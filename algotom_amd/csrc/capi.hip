@@ -555,6 +555,82 @@ int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
                                 static_cast<hipStream_t>(stream));
 }
 
+size_t ath_dfi_plan_bytes(int width) { return dfi_plan_bytes(width); }
+
+int ath_dfi_plan_build(const double* window, int width, void* plan,
+                       size_t plan_bytes, void* stream) {
+    clear_error();
+    const size_t need = dfi_plan_bytes(width);
+    if (need == 0)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "DFI: the padded width must be odd and at most 4095");
+    if (!plan) return set_error(ATH_ERR_INVALID_ARG, "null plan pointer");
+    if (plan_bytes < need)
+        return set_error(ATH_ERR_WORKSPACE, "DFI plan buffer too small");
+    return dfi_plan_build(window, width, plan,
+                          static_cast<hipStream_t>(stream));
+}
+
+size_t ath_dfi_workspace_bytes(int nrow_in, int ncol, int nrow, int width) {
+    if (nrow_in < 1 || ncol < 1 || nrow < 1 || width < 1) return 0;
+    return dfi_workspace_bytes(nrow_in, ncol, nrow, width);
+}
+
+int ath_dfi_reconstruction(const void* sino, int in_dtype, int nrow_in,
+                           int nslice, int ncol, long long row_stride,
+                           long long slice_stride, double xshift, int pad,
+                           int pad_mode, int apply_log, int angle_mode,
+                           double row_shift, int flip, int nrow,
+                           const void* plan, int width, void* workspace,
+                           size_t workspace_bytes, void* out, int out_dtype,
+                           long long out_row_stride,
+                           long long out_slice_stride, double mask_r2,
+                           int* flag, void* stream) {
+    clear_error();
+    if (!sino || !plan || !workspace || !out)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    if (apply_log && !flag)
+        return set_error(ATH_ERR_INVALID_ARG, "apply_log needs a flag pointer");
+    if (in_dtype != ATH_F32 && in_dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "DFI input must be float32 or float64");
+    if (out_dtype != ATH_F32 && out_dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "DFI output must be float32 or float64");
+    if (nrow_in < 2 || nslice < 1 || ncol < 2 || pad < 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
+    if (row_stride < ncol || out_row_stride < ncol)
+        return set_error(ATH_ERR_INVALID_ARG, "row stride below ncol");
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+        return set_error(ATH_ERR_INVALID_ARG, "bad pad mode");
+    const int ncol1 = ncol + (ncol % 2 == 0 ? 1 : 0);
+    if (width != ncol1 + 2 * pad || dfi_plan_bytes(width) == 0)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "DFI: width must be the odd width + 2*pad and at "
+                         "most 4095");
+    if (angle_mode == ATH_DFI_NO_ANGLES || angle_mode == ATH_DFI_ANGLES) {
+        if (nrow != nrow_in)
+            return set_error(ATH_ERR_INVALID_ARG, "nrow must equal nrow_in");
+    } else if (angle_mode == ATH_DFI_FOLD_360) {
+        if (nrow != nrow_in / 2 + 1)
+            return set_error(ATH_ERR_INVALID_ARG,
+                             "folded nrow must be nrow_in // 2 + 1");
+    } else {
+        return set_error(ATH_ERR_INVALID_ARG, "bad angle mode");
+    }
+    if (!(xshift == xshift) || !(row_shift == row_shift) ||
+        fabs(xshift) > 1e6 || fabs(row_shift) > 1e9)
+        return set_error(ATH_ERR_INVALID_ARG, "shift out of range");
+    if (workspace_bytes < dfi_workspace_bytes(nrow_in, ncol, nrow, width))
+        return set_error(ATH_ERR_WORKSPACE, "DFI: workspace too small");
+    return dfi_reconstruction(sino, in_dtype, nrow_in, nslice, ncol,
+                              row_stride, slice_stride, xshift, pad, pad_mode,
+                              apply_log, angle_mode, row_shift, flip, nrow,
+                              plan, width, workspace, out, out_dtype,
+                              out_row_stride, out_slice_stride, mask_r2, flag,
+                              static_cast<hipStream_t>(stream));
+}
+
 int ath_gaussian_filter_rows(const float* in, float* out, int nrows,
                              int ncol, const double* weights, int radius,
                              void* stream) {

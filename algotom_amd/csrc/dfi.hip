@@ -1,0 +1,769 @@
+// Direct Fourier inversion (DFI) on the GPU: algotom's dfi_reconstruction
+// (algotom/rec/reconstruction.py:687-907) for one slice at a time.
+//
+// Per slice, in reference order:
+//   1. dfi_shift_kernel     the odd-width, centre-shifted sinogram
+//                           (scipy.ndimage.shift order 3, mode "nearest",
+//                           :749-781): one workgroup per row, the 12-sample
+//                           edge-padded line and its cubic B-spline
+//                           coefficients in LDS, values rounded to the
+//                           input's float type;
+//   2. dfi_angles_kernel    (only with angles, :731-746) 360-degree fold,
+//                           the stacked [rows; flipped] sinogram shifted
+//                           along the angle axis (order 3, mode "wrap");
+//   3. dfi_rows_fwd_kernel  pad (numpy.pad modes), -log, ifftshift, the
+//                           length-N DFT by Bluestein's chirp-z method over
+//                           a power-of-two LDS FFT, fftshift, smoothing
+//                           window, then the quintic B-spline prefilter
+//                           along the row (:787-790 and the first axis of
+//                           map_coordinates' prefilter);
+//   4. dfi_cols_kernel      the quintic prefilter along the angle axis;
+//   5. dfi_grid_kernel      generate_mapping_coordinate (:687-728) per
+//                           Cartesian frequency, order-5 evaluation on the
+//                           (theta, r) grid (mode "reflect"), the circle
+//                           mask, written ifftshifted and transposed;
+//   6. dfi_rows_inv_kernel  inverse DFT along the first axis (Bluestein),
+//                           keeping the rows the crop keeps; transpose;
+//      dfi_rows_inv_kernel  inverse DFT along the second axis, fftshift,
+//                           real part, crop [pad, pad + ncol), ratio mask.
+//
+// Spline conventions (scipy.ndimage 1.15): prefilter = the exact inverse
+// of the sampled B-spline, applied as its impulse response h (closed form
+// from the filter's poles; |h[k]| < 1e-18 h[0] past the stored half
+// lengths) to the symmetric extension of the line: half-sample symmetric
+// for modes "nearest" (after the 12-sample edge padding) and "reflect",
+// whole-sample for "wrap".  Evaluation sums the order + 1 nearest knots
+// weighted by the centred B-spline; knots outside the line are clamped
+// ("nearest"), folded half-sample ("reflect") or whole-sample ("wrap").
+// All arithmetic after the shift is float64, as numpy's complex128 FFTs.
+#include "ath_internal.h"
+#include "fft_lds.h"
+
+namespace ath {
+namespace {
+
+constexpr int kK3 = 32;         // cubic inverse-filter half length
+constexpr int kK5 = 52;         // quintic inverse-filter half length
+constexpr int kNearestPad = 12; // scipy's pre-padding for mode "nearest"
+constexpr int kThreads = 256;
+constexpr int kMaxLine = 8192 + 2 * kNearestPad;  // longest shifted line
+constexpr double kPi = 3.141592653589793;         // numpy.pi
+
+// ---- plan layout ---------------------------------------------------------
+struct DfiPlan {
+    int N, M;
+    size_t tw, chirp, bhat, window, h3, h5, total;  // byte offsets
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int bluestein_length(int N) {
+    int M = 1;
+    while (M < 2 * N - 1) M <<= 1;
+    return M < 4 ? 4 : M;
+}
+
+DfiPlan plan_layout(int N) {
+    DfiPlan p{};
+    p.N = N;
+    p.M = bluestein_length(N);
+    size_t o = 0;
+    p.tw = o;     o = align256(o + (size_t)2 * p.M * sizeof(double2));
+    p.chirp = o;  o = align256(o + (size_t)N * sizeof(double2));
+    p.bhat = o;   o = align256(o + (size_t)p.M * sizeof(double2));
+    p.window = o; o = align256(o + (size_t)N * sizeof(double));
+    p.h3 = o;     o = align256(o + (size_t)(kK3 + 1) * sizeof(double));
+    p.h5 = o;     o = align256(o + (size_t)(kK5 + 1) * sizeof(double));
+    p.total = o;
+    return p;
+}
+
+template <typename T>
+__host__ __device__ T* at(void* base, size_t off) {
+    return reinterpret_cast<T*>(static_cast<char*>(base) + off);
+}
+template <typename T>
+__host__ __device__ const T* at(const void* base, size_t off) {
+    return reinterpret_cast<const T*>(static_cast<const char*>(base) + off);
+}
+
+// ---- B-spline weights -----------------------------------------------------
+__device__ __forceinline__ double beta3(double t) {
+    t = fabs(t);
+    if (t < 1.0) return 2.0 / 3.0 - t * t + 0.5 * t * t * t;
+    if (t < 2.0) { const double u = 2.0 - t; return u * u * u / 6.0; }
+    return 0.0;
+}
+
+__device__ __forceinline__ double beta5(double t) {
+    t = fabs(t);
+    if (t < 1.0) {
+        const double t2 = t * t;
+        return 11.0 / 20.0 + t2 * (-0.5 + t2 * (0.25 - t / 12.0));
+    }
+    if (t < 2.0)
+        return 17.0 / 40.0 +
+               t * (5.0 / 8.0 +
+                    t * (-7.0 / 4.0 +
+                         t * (5.0 / 4.0 + t * (-3.0 / 8.0 + t / 24.0))));
+    if (t < 3.0) {
+        const double u = 3.0 - t, u2 = u * u;
+        return u2 * u2 * u / 120.0;
+    }
+    return 0.0;
+}
+
+// half-sample symmetric (scipy "reflect") and whole-sample ("mirror")
+// index folds onto [0, n)
+__device__ __forceinline__ int fold_half(int i, int n) {
+    return pad_source(i, n, ATH_PAD_SYMMETRIC);
+}
+__device__ __forceinline__ int fold_whole(int i, int n) {
+    return pad_source(i, n, ATH_PAD_REFLECT);
+}
+
+// numpy.remainder(a, b) for b > 0
+__device__ __forceinline__ double np_mod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0 && m < 0.0) m += b;
+    return m;
+}
+
+template <typename T>
+__device__ __forceinline__ double round_to(double v) {
+    return (double)(T)v;
+}
+
+// ---- plan kernels ---------------------------------------------------------
+// chirp[n] = exp(-i pi n^2 / N), the exponent reduced mod 2N exactly
+__global__ void chirp_kernel(double2* chirp, int N) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) {
+        const long long q = ((long long)n * n) % (2LL * N);
+        double s, c;
+        sincospi((double)q / (double)N, &s, &c);
+        chirp[n] = make_double2(c, -s);
+    }
+}
+
+// bhat = FFT_M(b) / M, b[m] = b[M - m] = exp(+i pi m^2 / N) for 0 <= m < N,
+// zero elsewhere (one workgroup)
+template <int MAXB>
+__global__ __launch_bounds__(kFftThreads) void bhat_kernel(
+    const double2* __restrict__ chirp, const double2* __restrict__ tw, int N,
+    int M, double2* __restrict__ bhat) {
+    extern __shared__ double2 lds[];
+    for (int k = threadIdx.x; k < M; k += kFftThreads) {
+        double2 v = make_double2(0.0, 0.0);
+        if (k < N) v = conjg(chirp[k]);
+        else if (k > M - N) v = conjg(chirp[M - k]);
+        lds[k] = v;
+    }
+    __syncthreads();
+    lds_fft<-1, MAXB>(lds, M, tw);
+    const double inv = 1.0 / (double)M;
+    for (int k = threadIdx.x; k < M; k += kFftThreads)
+        bhat[k] = cscale(lds[k], inv);
+}
+
+// window (ones without one) and the inverse-filter taps.  1 / B(q) over the
+// poles z_i of the sampled B-spline (|z_i| < 1, z_i + 1/z_i = a_i):
+//   cubic:   h[k] = (1 - z)^2 g_z[k]
+//   quintic: h[k] = C (z2 g_z2[k] - z1 g_z1[k]) / (z1 z2 (a1 - a2)),
+//            C = (1 - z1)^2 (1 - z2)^2,
+// with g_z[k] = z^|k| / (1 - z^2), the response of 1/((1 - zq)(1 - z/q)).
+__global__ void dfi_tables_kernel(const double* __restrict__ window, int N,
+                                  double* __restrict__ win,
+                                  double* __restrict__ h3,
+                                  double* __restrict__ h5) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) win[i] = window ? window[i] : 1.0;
+    if (i <= kK3) {
+        const double w = -4.0;
+        const double z = 2.0 / (w - sqrt(w * w - 4.0));
+        h3[i] = (1.0 - z) * (1.0 - z) * pow(z, (double)i) / (1.0 - z * z);
+    }
+    if (i <= kK5) {
+        const double w1 = -13.0 + sqrt(105.0), w2 = -13.0 - sqrt(105.0);
+        const double z1 = 2.0 / (w1 - sqrt(w1 * w1 - 4.0));
+        const double z2 = 2.0 / (w2 - sqrt(w2 * w2 - 4.0));
+        const double C = (1.0 - z1) * (1.0 - z1) * (1.0 - z2) * (1.0 - z2);
+        const double g1 = pow(z1, (double)i) / (1.0 - z1 * z1);
+        const double g2 = pow(z2, (double)i) / (1.0 - z2 * z2);
+        h5[i] = C * (z2 * g2 - z1 * g1) / (z1 * z2 * (w1 - w2));
+    }
+}
+
+// ---- 1. centre shift --------------------------------------------------------
+// in: (nrow, ncol) rows at row_stride; out S: (nrow, ncol1) float64 holding
+// values of the input's float type Tr.  line = the odd-width row edge-padded
+// by 12 on both sides (L1 = ncol1 + 24), coefficients by the half-sample
+// symmetric prefilter, out[j] = sum_q beta3(pos - k_q) c[clamp(k_q)] at
+// pos = (j - xshift) + 12.
+template <typename Tin, typename Tr>
+__global__ __launch_bounds__(kThreads) void dfi_shift_kernel(
+    const Tin* __restrict__ in, long long row_stride, int nrow, int ncol,
+    int ncol1, double xshift, const double* __restrict__ h3,
+    double* __restrict__ S) {
+    extern __shared__ double sh[];
+    const int L1 = ncol1 + 2 * kNearestPad;
+    double* line = sh;
+    double* coef = sh + L1;
+    const int tid = threadIdx.x;
+    for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+        const Tin* src = in + (long long)row * row_stride;
+        for (int p = tid; p < L1; p += kThreads) {
+            int q = min(max(p - kNearestPad, 0), ncol1 - 1);
+            q = min(q, ncol - 1);  // the odd-width edge column
+            line[p] = (double)src[q];
+        }
+        __syncthreads();
+        for (int p = tid; p < L1; p += kThreads) {
+            double c = h3[0] * line[p];
+            for (int t = 1; t <= kK3; ++t)
+                c += h3[t] * (line[fold_half(p - t, L1)] +
+                              line[fold_half(p + t, L1)]);
+            coef[p] = c;
+        }
+        __syncthreads();
+        double* dst = S + (size_t)row * ncol1;
+        for (int j = tid; j < ncol1; j += kThreads) {
+            const double pos = ((double)j - xshift) + (double)kNearestPad;
+            const int k0 = (int)floor(pos) - 1;
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + q;
+                v += beta3(pos - (double)k) * coef[min(max(k, 0), L1 - 1)];
+            }
+            dst[j] = round_to<Tr>(v);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- 2. angle handling ----------------------------------------------------
+// src(r, j) = the (folded) sinogram row r < rows; v360 = [src[:rows - 1];
+// fliplr(src)] (L = 2 rows - 1 rows); out[i] = sum_q beta3(pos - k_q)
+// c[fold_whole(k_q)] with pos = i - shift wrapped into [0, L - 1) when
+// outside [0, L - 1] and c the whole-sample prefilter of v360 along rows.
+template <typename Tr>
+__device__ __forceinline__ double angle_src(const double* S, int nrow_in,
+                                            int ncol1, int rows, int fold,
+                                            int r, int j) {
+    const double a = S[(size_t)r * ncol1 + j];
+    if (!fold) return a;
+    const double b = S[(size_t)(nrow_in - rows + r) * ncol1 + (ncol1 - 1 - j)];
+    return round_to<Tr>(round_to<Tr>(a + b) / 2.0);
+}
+
+template <typename Tr>
+__global__ __launch_bounds__(kThreads) void dfi_angles_kernel(
+    const double* __restrict__ S, int nrow_in, int ncol1, int rows, int fold,
+    double shift, int flip, const double* __restrict__ h3,
+    double* __restrict__ S2) {
+    const int L = 2 * rows - 1;
+    for (int i = blockIdx.y; i < rows; i += gridDim.y) {
+        double pos = (double)i - shift;
+        if (L > 1 && (pos < 0.0 || pos > (double)(L - 1)))
+            pos = np_mod(pos, (double)(L - 1));
+        const int k0 = (int)floor(pos) - 1;
+        for (int j = blockIdx.x * kThreads + threadIdx.x; j < ncol1;
+             j += gridDim.x * kThreads) {
+            double v = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                const int k = fold_whole(k0 + q, L);
+                double c = 0.0;
+                for (int t = -kK3; t <= kK3; ++t) {
+                    const int r = fold_whole(k + t, L);
+                    const double x =
+                        r < rows - 1
+                            ? angle_src<Tr>(S, nrow_in, ncol1, rows, fold, r, j)
+                            : angle_src<Tr>(S, nrow_in, ncol1, rows, fold,
+                                            r - (rows - 1), ncol1 - 1 - j);
+                    c += h3[t < 0 ? -t : t] * x;
+                }
+                v += beta3(pos - (double)(k0 + q)) * c;
+            }
+            v = round_to<Tr>(v);
+            if (flip)
+                S2[(size_t)(rows - 1 - i) * ncol1 + (ncol1 - 1 - j)] = v;
+            else
+                S2[(size_t)i * ncol1 + j] = v;
+        }
+    }
+}
+
+// ---- Bluestein DFT over the LDS FFT ----------------------------------------
+// nk = (n^2 + k^2 - (k - n)^2) / 2 turns the length-N DFT into a circular
+// convolution of length M >= 2N - 1:  DFT(a)[k] = chirp[k] * conv[k],
+// conv = IFFT_M(FFT_M(chirp * a) * bhat).  On entry buf[n] = chirp[n] a[n]
+// (n < N, zero up to M); on exit buf[k] = conv[k] (bhat carries the 1/M).
+template <int MAXB, bool QT>
+__device__ __forceinline__ void bluestein_core(double2* buf, int M,
+                                               const double2* __restrict__ tw,
+                                               const double2* tq,
+                                               const double2* __restrict__ bhat) {
+    lds_fft<-1, MAXB, QT>(buf, M, tw, tq);
+    for (int k = threadIdx.x; k < M; k += kFftThreads)
+        buf[k] = cmul(buf[k], bhat[k]);
+    __syncthreads();
+    lds_fft<+1, MAXB, QT>(buf, M, tw, tq);
+}
+
+// ---- 3. row spectra + row prefilter ----------------------------------------
+template <int MAXB, typename Tr>
+__global__ __launch_bounds__(kFftThreads) void dfi_rows_fwd_kernel(
+    const double* __restrict__ S, int rows, int ncol1, int pad, int pad_mode,
+    int apply_log, const void* __restrict__ plan, int N, int M, size_t o_tw,
+    size_t o_chirp, size_t o_bhat, size_t o_win, size_t o_h5,
+    double2* __restrict__ X, int* __restrict__ flag) {
+    constexpr bool QT = MAXB <= 4;
+    extern __shared__ double2 lds[];
+    double2* buf = lds;
+    double2* tq = lds + M;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const double* win = at<double>(plan, o_win);
+    const double* h5 = at<double>(plan, o_h5);
+    const int tid = threadIdx.x;
+    const int m = (N - 1) / 2;
+    if (QT) stage_quarter(tq, tw, M);
+    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+        const double* src = S + (size_t)row * ncol1;
+        for (int n = tid; n < M; n += kFftThreads) {
+            double2 a = make_double2(0.0, 0.0);
+            if (n < N) {
+                int u = n + m;  // ifftshift
+                if (u >= N) u -= N;
+                const int c = pad_source(u - pad, ncol1, pad_mode);
+                double v = c >= 0 ? src[c] : 0.0;
+                if (apply_log) {
+                    if (v <= 0.0) {
+                        v = 1.0;
+                        *flag = 1;
+                    }
+                    v = round_to<Tr>(-log(v));
+                }
+                a = cscale(chirp[n], v);
+            }
+            buf[n] = a;
+        }
+        __syncthreads();
+        bluestein_core<MAXB, QT>(buf, M, tw, tq, bhat);
+        // spectrum in fftshift order times the window, kept at buf[N + k']
+        // (M >= 2N, so it does not overlap buf[0, N))
+        for (int kp = tid; kp < N; kp += kFftThreads) {
+            int k = kp + m + 1;
+            if (k >= N) k -= N;
+            buf[N + kp] = cscale(cmul(chirp[k], buf[k]), win[kp]);
+        }
+        __syncthreads();
+        const double2* sp = buf + N;
+        double2* dst = X + (size_t)row * N;
+        for (int kp = tid; kp < N; kp += kFftThreads) {
+            double2 acc = cscale(sp[kp], h5[0]);
+            for (int t = 1; t <= kK5; ++t)
+                acc = cadd(acc, cscale(cadd(sp[fold_half(kp - t, N)],
+                                            sp[fold_half(kp + t, N)]),
+                                       h5[t]));
+            dst[kp] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- 4. column prefilter ----------------------------------------------------
+// out[r][c] = sum_t h5[|t|] X[fold_half(r + t)][c]: one column per lane,
+// kColRun consecutive rows per thread (each loaded row feeds every output
+// within kK5 of it).
+constexpr int kColRun = 16;
+
+__global__ __launch_bounds__(kThreads) void dfi_cols_kernel(
+    const double2* __restrict__ X, int rows, int N,
+    const double* __restrict__ h5, double2* __restrict__ out) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kColRun;
+    if (c >= N || r0 >= rows) return;
+    double2 acc[kColRun];
+#pragma unroll
+    for (int q = 0; q < kColRun; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int i = 0; i < kColRun + 2 * kK5; ++i) {
+        const int r = fold_half(r0 - kK5 + i, rows);
+        const double2 v = X[(size_t)r * N + c];
+#pragma unroll
+        for (int q = 0; q < kColRun; ++q) {
+            int d = i - kK5 - q;
+            d = d < 0 ? -d : d;
+            if (d <= kK5) {
+                const double h = h5[d];
+                acc[q].x += h * v.x;
+                acc[q].y += h * v.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kColRun; ++q)
+        if (r0 + q < rows) out[(size_t)(r0 + q) * N + c] = acc[q];
+}
+
+// ---- 5. gridding --------------------------------------------------------------
+// T[v][u] = G[(u + m) % N][(v + m) % N] (ifftshift, transposed), G[y][x] =
+// mask ? sum beta5 beta5 coef : 0 at the float32 (theta, r) of
+// generate_mapping_coordinate: X = m - x, Y = y - m.
+__global__ __launch_bounds__(kThreads) void dfi_grid_kernel(
+    const double2* __restrict__ coef, int rows, int N,
+    double2* __restrict__ T) {
+    const int m = (N - 1) / 2;
+    const double r_max = (double)m;
+    const double hs1 = (double)rows - 1.0;
+    for (int v = blockIdx.y; v < N; v += gridDim.y) {
+        int x = v + m;
+        if (x >= N) x -= N;
+        const int X = m - x;
+        double2* dst = T + (size_t)v * N;
+        for (int u = blockIdx.x * kThreads + threadIdx.x; u < N;
+             u += gridDim.x * kThreads) {
+            int y = u + m;
+            if (y >= N) y -= N;
+            const int Y = y - m;
+            double2 val = make_double2(0.0, 0.0);
+            if (X * X + Y * Y <= m * m) {
+                float r = (float)fmin(fmax(sqrt((double)(X * X + Y * Y)), 0.0),
+                                      r_max);
+                double theta = kPi + atan2((double)Y, (double)X);
+                if (theta > kPi) {
+                    r = -r;
+                    theta -= kPi;
+                }
+                const double rr = (double)(float)fmin(
+                    fmax((double)r + r_max, 0.0), (double)(N - 1));
+                const double tt = (double)(float)(theta * hs1 / kPi);
+                const int s0 = (int)floor(tt) - 2, s1 = (int)floor(rr) - 2;
+                double wx[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) wx[j] = beta5(rr - (double)(s1 + j));
+                int cx[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) cx[j] = fold_half(s1 + j, N);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    const double wy = beta5(tt - (double)(s0 + i));
+                    const double2* crow =
+                        coef + (size_t)fold_half(s0 + i, rows) * N;
+                    double2 rs = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        const double2 c = crow[cx[j]];
+                        rs.x += wx[j] * c.x;
+                        rs.y += wx[j] * c.y;
+                    }
+                    val.x += wy * rs.x;
+                    val.y += wy * rs.y;
+                }
+            }
+            dst[u] = val;
+        }
+    }
+}
+
+// ---- 6. inverse DFTs ----------------------------------------------------------
+// Row-wise inverse DFT (unnormalised transform / N) of `in` (nrows x N) by
+// Bluestein on the conjugate; each row keeps ncol outputs, output index
+// kc <- DFT index (kc + pad + m + 1) % N (fftshift, then the crop).
+// FINAL = false: out (nrows x ncol) complex.  FINAL = true: real part times
+// the ratio mask into the caller's output (row `row` of the slice).
+template <int MAXB, bool FINAL, typename To>
+__global__ __launch_bounds__(kFftThreads) void dfi_rows_inv_kernel(
+    const double2* __restrict__ in, int nrows, int ncol, int pad,
+    const void* __restrict__ plan, int N, int M, size_t o_tw, size_t o_chirp,
+    size_t o_bhat, double2* __restrict__ cout, To* __restrict__ rout,
+    long long out_row_stride, double mask_r2) {
+    constexpr bool QT = MAXB <= 4;
+    extern __shared__ double2 lds[];
+    double2* buf = lds;
+    double2* tq = lds + M;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const int tid = threadIdx.x;
+    const int m = (N - 1) / 2;
+    const double invN = 1.0 / (double)N;
+    const int mc = ncol / 2;
+    if (QT) stage_quarter(tq, tw, M);
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const double2* src = in + (size_t)row * N;
+        for (int n = tid; n < M; n += kFftThreads)
+            buf[n] = n < N ? cmul(chirp[n], conjg(src[n]))
+                           : make_double2(0.0, 0.0);
+        __syncthreads();
+        bluestein_core<MAXB, QT>(buf, M, tw, tq, bhat);
+        for (int kc = tid; kc < ncol; kc += kFftThreads) {
+            int k = kc + pad + m + 1;  // < 2N
+            if (k >= N) k -= N;
+            const double2 y = conjg(cmul(chirp[k], buf[k]));
+            if constexpr (FINAL) {
+                const double re = y.x * invN;
+                double v = re;
+                if (mask_r2 >= 0.0) {
+                    const int dy = row - mc, dx = kc - mc;
+                    v = (double)(dy * dy + dx * dx) <= mask_r2 ? re : re * 0.0;
+                }
+                rout[(long long)row * out_row_stride + kc] = (To)v;
+            } else {
+                cout[(size_t)row * ncol + kc] = cscale(y, invN);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// A (rows x cols) -> A^T (cols x rows), 32 x 32 LDS tiles
+__global__ __launch_bounds__(kThreads) void dfi_transpose_kernel(
+    const double2* __restrict__ A, int rows, int cols,
+    double2* __restrict__ At) {
+    __shared__ double2 tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int tr = blockIdx.y; tr * 32 < rows; tr += gridDim.y)
+        for (int tc = blockIdx.x; tc * 32 < cols; tc += gridDim.x) {
+            for (int k = ty; k < 32; k += 8) {
+                const int r = tr * 32 + k, c = tc * 32 + tx;
+                if (r < rows && c < cols)
+                    tile[k][tx] = A[(size_t)r * cols + c];
+            }
+            __syncthreads();
+            for (int k = ty; k < 32; k += 8) {
+                const int c = tc * 32 + k, r = tr * 32 + tx;
+                if (r < rows && c < cols)
+                    At[(size_t)c * rows + r] = tile[tx][k];
+            }
+            __syncthreads();
+        }
+}
+
+// ---- dispatch ---------------------------------------------------------------------
+template <int MAXB>
+size_t fft_lds_bytes(int M) {
+    return (size_t)M * sizeof(double2) +
+           (MAXB <= 4 ? (size_t)(M / 4) * sizeof(double2) : 0);
+}
+
+template <int MAXB, typename Tr>
+int launch_rows_fwd(const double* S, int rows, int ncol1, int pad,
+                    int pad_mode, int apply_log, const void* plan,
+                    const DfiPlan& p, double2* X, int* flag, hipStream_t s) {
+    auto k = dfi_rows_fwd_kernel<MAXB, Tr>;
+    const size_t lds = fft_lds_bytes<MAXB>(p.M);
+    prepare_lds(k, lds);
+    const int grid = std::min(rows, resident_blocks(k, lds));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kFftThreads), lds, s, S, rows,
+                       ncol1, pad, pad_mode, apply_log, plan, p.N, p.M, p.tw,
+                       p.chirp, p.bhat, p.window, p.h5, X, flag);
+    return check_hip(hipGetLastError(), "dfi_rows_fwd_kernel launch");
+}
+
+template <int MAXB, bool FINAL, typename To>
+int launch_rows_inv(const double2* in, int nrows, int ncol, int pad,
+                    const void* plan, const DfiPlan& p, double2* cout,
+                    To* rout, long long out_row_stride, double mask_r2,
+                    hipStream_t s) {
+    auto k = dfi_rows_inv_kernel<MAXB, FINAL, To>;
+    const size_t lds = fft_lds_bytes<MAXB>(p.M);
+    prepare_lds(k, lds);
+    const int grid = std::min(nrows, resident_blocks(k, lds));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kFftThreads), lds, s, in, nrows,
+                       ncol, pad, plan, p.N, p.M, p.tw, p.chirp, p.bhat, cout,
+                       rout, out_row_stride, mask_r2);
+    return check_hip(hipGetLastError(), "dfi_rows_inv_kernel launch");
+}
+
+template <typename Tr>
+int rows_fwd(int maxb, const double* S, int rows, int ncol1, int pad,
+             int pad_mode, int apply_log, const void* plan, const DfiPlan& p,
+             double2* X, int* flag, hipStream_t s) {
+    switch (maxb) {
+    case 1: return launch_rows_fwd<1, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    case 2: return launch_rows_fwd<2, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    case 4: return launch_rows_fwd<4, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    default: return launch_rows_fwd<8, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    }
+}
+
+template <bool FINAL, typename To>
+int rows_inv(int maxb, const double2* in, int nrows, int ncol, int pad,
+             const void* plan, const DfiPlan& p, double2* cout, To* rout,
+             long long out_row_stride, double mask_r2, hipStream_t s) {
+    switch (maxb) {
+    case 1: return launch_rows_inv<1, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
+    case 2: return launch_rows_inv<2, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
+    case 4: return launch_rows_inv<4, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
+    default: return launch_rows_inv<8, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
+    }
+}
+
+struct DfiWork {
+    size_t s1, s2, r1, r2, total;
+};
+
+DfiWork work_layout(int nrow_in, int ncol, int nrow, int N) {
+    const int ncol1 = ncol + (ncol % 2 == 0 ? 1 : 0);
+    DfiWork w{};
+    size_t o = 0;
+    w.s1 = o; o = align256(o + (size_t)nrow_in * ncol1 * sizeof(double));
+    w.s2 = o; o = align256(o + (size_t)nrow * ncol1 * sizeof(double));
+    const size_t r1 = std::max({(size_t)nrow * N, (size_t)N * N,
+                                (size_t)ncol * N});
+    const size_t r2 = std::max((size_t)nrow * N, (size_t)N * ncol);
+    w.r1 = o; o = align256(o + r1 * sizeof(double2));
+    w.r2 = o; o = align256(o + r2 * sizeof(double2));
+    w.total = o;
+    return w;
+}
+
+template <typename Tin, typename Tr>
+int shift_rows(const Tin* in, long long row_stride, int nrow, int ncol,
+               int ncol1, double xshift, const double* h3, double* S,
+               hipStream_t s) {
+    auto k = dfi_shift_kernel<Tin, Tr>;
+    const size_t lds = 2 * (size_t)(ncol1 + 2 * kNearestPad) * sizeof(double);
+    prepare_lds(k, 2 * (size_t)kMaxLine * sizeof(double));
+    const int grid = std::min(nrow, 4096);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, s, in, row_stride,
+                       nrow, ncol, ncol1, xshift, h3, S);
+    return check_hip(hipGetLastError(), "dfi_shift_kernel launch");
+}
+
+}  // namespace
+
+size_t dfi_plan_bytes(int width) {
+    if (width < 3 || width % 2 == 0 || bluestein_length(width) > 8192)
+        return 0;
+    return plan_layout(width).total;
+}
+
+int dfi_plan_build(const double* window, int width, void* plan,
+                   hipStream_t s) {
+    const DfiPlan p = plan_layout(width);
+    const int L = 2 * p.M;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((L + 255) / 256), dim3(256), 0, s,
+                       at<double2>(plan, p.tw), L);
+    hipLaunchKernelGGL(chirp_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s,
+                       at<double2>(plan, p.chirp), p.N);
+    const int tab = std::max(p.N, kK5 + 1);
+    hipLaunchKernelGGL(dfi_tables_kernel, dim3((tab + 255) / 256), dim3(256),
+                       0, s, window, p.N, at<double>(plan, p.window),
+                       at<double>(plan, p.h3), at<double>(plan, p.h5));
+    const size_t lds = (size_t)p.M * sizeof(double2);
+#define ATH_BHAT(B)                                                           \
+    {                                                                         \
+        prepare_lds(bhat_kernel<B>, lds);                                     \
+        hipLaunchKernelGGL(bhat_kernel<B>, dim3(1), dim3(kFftThreads), lds,   \
+                           s, at<double2>(plan, p.chirp),                     \
+                           at<double2>(plan, p.tw), p.N, p.M,                 \
+                           at<double2>(plan, p.bhat));                        \
+    }
+    switch (maxb_for(p.M)) {
+    case 1: ATH_BHAT(1) break;
+    case 2: ATH_BHAT(2) break;
+    case 4: ATH_BHAT(4) break;
+    default: ATH_BHAT(8) break;
+    }
+#undef ATH_BHAT
+    return check_hip(hipGetLastError(), "dfi plan kernels");
+}
+
+size_t dfi_workspace_bytes(int nrow_in, int ncol, int nrow, int width) {
+    return work_layout(nrow_in, ncol, nrow, width).total;
+}
+
+int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
+                       int ncol, long long row_stride, long long slice_stride,
+                       double xshift, int pad, int pad_mode, int apply_log,
+                       int angle_mode, double row_shift, int flip, int nrow,
+                       const void* plan, int width, void* workspace,
+                       void* out, int out_dtype, long long out_row_stride,
+                       long long out_slice_stride, double mask_r2, int* flag,
+                       hipStream_t s) {
+    const DfiPlan p = plan_layout(width);
+    const DfiWork w = work_layout(nrow_in, ncol, nrow, width);
+    const int ncol1 = ncol + (ncol % 2 == 0 ? 1 : 0);
+    const int maxb = maxb_for(p.M);
+    const bool f64 = dtype == ATH_F64;
+    double* S1 = at<double>(workspace, w.s1);
+    double* S2 = at<double>(workspace, w.s2);
+    double2* R1 = at<double2>(workspace, w.r1);
+    double2* R2 = at<double2>(workspace, w.r2);
+    const double* h3 = at<double>(plan, p.h3);
+    const double* h5 = at<double>(plan, p.h5);
+    const size_t esz = f64 ? sizeof(double) : sizeof(float);
+    for (int sl = 0; sl < nslice; ++sl) {
+        const void* in = static_cast<const char*>(sino) +
+                         (size_t)sl * (size_t)slice_stride * esz;
+        int st = f64 ? shift_rows<double, double>(
+                           static_cast<const double*>(in), row_stride,
+                           nrow_in, ncol, ncol1, xshift, h3, S1, s)
+                     : shift_rows<float, float>(
+                           static_cast<const float*>(in), row_stride, nrow_in,
+                           ncol, ncol1, xshift, h3, S1, s);
+        if (st != ATH_OK) return st;
+        const double* S = S1;
+        if (angle_mode != 0) {
+            const dim3 grid = row_grid(nrow, ncol1, kThreads);
+            const int fold = angle_mode == 2 ? 1 : 0;
+            if (f64)
+                hipLaunchKernelGGL(dfi_angles_kernel<double>, grid,
+                                   dim3(kThreads), 0, s, S1, nrow_in, ncol1,
+                                   nrow, fold, row_shift, flip, h3, S2);
+            else
+                hipLaunchKernelGGL(dfi_angles_kernel<float>, grid,
+                                   dim3(kThreads), 0, s, S1, nrow_in, ncol1,
+                                   nrow, fold, row_shift, flip, h3, S2);
+            st = check_hip(hipGetLastError(), "dfi_angles_kernel launch");
+            if (st != ATH_OK) return st;
+            S = S2;
+        }
+        st = f64 ? rows_fwd<double>(maxb, S, nrow, ncol1, pad, pad_mode,
+                                    apply_log, plan, p, R1, flag, s)
+                 : rows_fwd<float>(maxb, S, nrow, ncol1, pad, pad_mode,
+                                   apply_log, plan, p, R1, flag, s);
+        if (st != ATH_OK) return st;
+        {
+            const dim3 grid((p.N + 63) / 64,
+                            (nrow + 4 * kColRun - 1) / (4 * kColRun));
+            hipLaunchKernelGGL(dfi_cols_kernel, grid, dim3(kThreads), 0, s, R1,
+                               nrow, p.N, h5, R2);
+            st = check_hip(hipGetLastError(), "dfi_cols_kernel launch");
+            if (st != ATH_OK) return st;
+        }
+        hipLaunchKernelGGL(dfi_grid_kernel, row_grid(p.N, p.N, kThreads),
+                           dim3(kThreads), 0, s, R2, nrow, p.N, R1);
+        st = check_hip(hipGetLastError(), "dfi_grid_kernel launch");
+        if (st != ATH_OK) return st;
+        st = rows_inv<false, double>(maxb, R1, p.N, ncol, pad, plan, p, R2,
+                                     nullptr, 0, -1.0, s);
+        if (st != ATH_OK) return st;
+        {
+            const dim3 grid(std::min((ncol + 31) / 32, 65535),
+                            std::min((p.N + 31) / 32, 65535));
+            hipLaunchKernelGGL(dfi_transpose_kernel, grid, dim3(kThreads), 0,
+                               s, R2, p.N, ncol, R1);
+            st = check_hip(hipGetLastError(), "dfi_transpose_kernel launch");
+            if (st != ATH_OK) return st;
+        }
+        char* o = static_cast<char*>(out) +
+                  (size_t)sl * (size_t)out_slice_stride *
+                      (out_dtype == ATH_F64 ? sizeof(double) : sizeof(float));
+        st = out_dtype == ATH_F64
+                 ? rows_inv<true, double>(maxb, R1, ncol, ncol, pad, plan, p,
+                                          nullptr, reinterpret_cast<double*>(o),
+                                          out_row_stride, mask_r2, s)
+                 : rows_inv<true, float>(maxb, R1, ncol, ncol, pad, plan, p,
+                                         nullptr, reinterpret_cast<float*>(o),
+                                         out_row_stride, mask_r2, s);
+        if (st != ATH_OK) return st;
+    }
+    return ATH_OK;
+}
+
+}  // namespace ath

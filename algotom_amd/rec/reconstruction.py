@@ -692,6 +692,130 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
     return _finish(out, as_numpy)
 
 
+# --------------------------------------------------------------------------
+# DFI (direct Fourier inversion), reconstruction.py:687-907
+# --------------------------------------------------------------------------
+_dfi_plans = {}
+
+
+def _dfi_plan(width, filter_name, dev):
+    """Device plan of ath_dfi_reconstruction for padded width ``width``:
+    Bluestein tables, the smoothing window (reconstruction.py:875-878) and
+    the B-spline prefilter taps; a few are kept per device."""
+    import torch
+    key = (dev.index, width, filter_name)
+    plan = _dfi_plans.get(key)
+    if plan is not None:
+        return plan
+    lib = nat.load()
+    nbytes = lib.ath_dfi_plan_bytes(width)
+    if nbytes == 0:
+        raise NotImplementedError(
+            "dfi_reconstruction on the GPU supports padded widths "
+            "ncol + 2*int(pad_rate*ncol) up to 4095 (got %d)" % width)
+    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    win = None
+    if filter_name is not None:
+        win = torch.from_numpy(np.ascontiguousarray(
+            make_smoothing_window(filter_name, width),
+            dtype=np.float64)).to(dev)
+    nat.check(lib.ath_dfi_plan_build(
+        nat.ptr(win) if win is not None else None, width, nat.ptr(plan),
+        nbytes, nat.stream_ptr(dev)), "ath_dfi_plan_build")
+    if len(_dfi_plans) >= 8:
+        _dfi_plans.clear()
+    _dfi_plans[key] = plan
+    return plan
+
+
+def _dfi_angles(angles, nrow):
+    """Angle handling of __dfi_handle_angles (reconstruction.py:731-746):
+    (mode, rows, row shift, flip), the scalars in numpy's arithmetic."""
+    if len(angles) != nrow:
+        raise ValueError("!!! Number of angles is not the same as the row "
+                         "number of the sinogram !!!")
+    angles = np.asarray(angles)
+    t_ang = np.sum(np.abs(np.diff(angles * 180.0 / np.pi)))
+    fold = abs(t_ang - 360) < 10
+    rows = nrow // 2 + 1 if fold else nrow
+    step = np.mean(np.abs(np.diff(angles)))
+    b_ang = angles[0] - (angles[0] // (2 * np.pi)) * (2 * np.pi)
+    mode = nat.DFI_FOLD_360 if fold else nat.DFI_ANGLES
+    return mode, rows, float(b_ang / step), int(bool(angles[-1] < angles[0]))
+
+
+def dfi_reconstruction(sinogram, center, angles=None, ratio=1.0,
+                       filter_name="hann", pad_rate=0.25, pad_mode="edge",
+                       apply_log=True, ncore=None):
+    """Apply the DFI (direct Fourier inversion) reconstruction method to a
+    sinogram-image or a chunk of sinogram-images (reconstruction.py:809-907).
+    Angular axis is 0; for 3D input the slicing axis is 1.
+
+    Every slice runs on the GPU through ath_dfi_reconstruction: the odd-width
+    sinogram shifted to the centre (cubic B-spline), optional angle handling
+    (360-degree fold and first-angle offset), padding by int(pad_rate *
+    width), -log, row FFTs, the smoothing window, order-5 B-spline gridding
+    of the polar spectrum onto the Cartesian grid, and the inverse 2D FFT,
+    all float64 after the shift.  ``ncore`` is ignored.
+
+    Returns
+    -------
+    array_like
+        (ncol, ncol) float64, or (ncol, nslice, ncol) float64 for 3D input
+        ((ncol, 1, ncol) for a one-slice stack).
+    """
+    import torch
+    shape = tuple(sinogram.shape)
+    input_3d = len(shape) == 3
+    nrow, ncol = shape[0], shape[-1]
+    num_sino = shape[1] if input_3d else 1
+    if center < 0 or center >= ncol:
+        raise ValueError("Center is out of the range [0, {}]".format(ncol - 1))
+    if ncol / nrow > 5.0:
+        warnings.warn("!!!Sinogram is significantly undersampled!!! "
+                      "Considering to use the 'upsample_sinogram' method "
+                      "before reconstruction!")
+    if pad_mode not in nat.PAD_MODES:
+        raise NotImplementedError(
+            "pad_mode %r is not supported by the GPU path (supported: %s)"
+            % (pad_mode, ", ".join(nat.PAD_MODES)))
+    ncol1 = ncol + 1 if ncol % 2 == 0 else ncol
+    xshift = (ncol1 - 1) / 2.0 - center
+    pad = int(pad_rate * ncol1)
+    mode, rows, row_shift, flip = nat.DFI_NO_ANGLES, nrow, 0.0, 0
+    if angles is not None:
+        mode, rows, row_shift, flip = _dfi_angles(angles, nrow)
+    width = ncol1 + 2 * pad
+    mask_r2 = -1.0
+    if ratio is not None:
+        if ratio == 0.0:
+            ratio = min(center, ncol - center) / (0.5 * ncol)
+        radius = ratio * (ncol // 2)
+        mask_r2 = float(radius * radius)
+    dev = nat.device()
+    as_numpy = not _is_tensor(sinogram)
+    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
+    plan = _dfi_plan(width, filter_name, dev)
+    lib = nat.load()
+    nws = lib.ath_dfi_workspace_bytes(nrow, ncol, rows, width)
+    work = torch.empty(nws, dtype=torch.uint8, device=dev)
+    out = torch.empty((ncol, num_sino, ncol), dtype=torch.float64, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    nat.check(lib.ath_dfi_reconstruction(
+        nat.ptr(sino_t), nat.dtype_code(sino_t.dtype), nrow, num_sino, ncol,
+        num_sino * ncol, ncol, float(xshift), pad, nat.PAD_MODES[pad_mode],
+        int(bool(apply_log)), mode, row_shift, flip, rows, nat.ptr(plan),
+        width, nat.ptr(work), nws, nat.ptr(out), nat.ATH_F64,
+        num_sino * ncol, ncol, mask_r2, nat.ptr(flag), nat.stream_ptr(dev)),
+        "ath_dfi_reconstruction")
+    if apply_log and int(flag.item()):
+        warnings.warn("!!!Applying logarithm is enabled but there are "
+                      "values <= 0.0 in the data!!!")
+    if not input_3d:
+        out = out.reshape(ncol, ncol)
+    return _finish(out, as_numpy)
+
+
 # Centre search by slice metrics (reconstruction.py:1254-1458), implemented
 # in algotom_amd.rec.center and reachable here, where the reference has it.
 def _find_center_based_slice_metric(*args, **kwargs):

@@ -316,6 +316,61 @@ int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
                              long long out_img_stride, double mask_radius,
                              int scale_nrow, void* stream);
 
+/* ---- DFI: direct Fourier inversion (reconstruction.py:687-907) -------- */
+
+/* DFI angle handling (__dfi_handle_angles, reconstruction.py:731-746) */
+#define ATH_DFI_NO_ANGLES 0 /* angles=None: the sinogram is used as it is */
+#define ATH_DFI_ANGLES 1    /* shift by the first angle along the angle axis */
+#define ATH_DFI_FOLD_360 2  /* a ~360-degree scan: fold to nrow//2 + 1 rows,
+                               then shift */
+
+/* Bytes of a DFI plan for padded width `width` (odd; the Bluestein FFT
+ * length, the smallest power of two >= 2*width - 1, must be <= 8192, so
+ * width <= 4095); 0 if unsupported. */
+size_t ath_dfi_plan_bytes(int width);
+
+/* Builds the plan: twiddles, Bluestein chirp and kernel spectrum for the
+ * length-`width` DFT, the smoothing window (device float64, `width`
+ * values = make_smoothing_window(filter_name, width), reconstruction.py
+ * :877-878; NULL = none) and the cubic / quintic B-spline inverse-filter
+ * taps of scipy.ndimage's spline prefilter. */
+int ath_dfi_plan_build(const double* window, int width, void* plan,
+                       size_t plan_bytes, void* stream);
+
+/* Device workspace bytes of ath_dfi_reconstruction (independent of the
+ * slice count: slices are processed one after another). */
+size_t ath_dfi_workspace_bytes(int nrow_in, int ncol, int nrow, int width);
+
+/* dfi_reconstruction (reconstruction.py:809-907) below its Python argument
+ * handling, for nslice sinograms; sinogram j's element (angle i, column c)
+ * at sino[j*slice_stride + i*row_stride + c] (float32 or float64).
+ *   __dfi_handle_sinogram (:749-781): odd width ncol1 (edge column added
+ *     when ncol is even), scipy.ndimage.shift by xshift = (ncol1-1)/2 -
+ *     center (order 3, mode "nearest") in the input's precision; with
+ *     angle_mode != ATH_DFI_NO_ANGLES the angle handling (:731-746) with
+ *     row_shift = b_ang/step and flip = angles[-1] < angles[0], giving nrow
+ *     rows; numpy.pad by `pad` columns in pad_mode (width = ncol1 + 2*pad);
+ *   apply_log (:862-867): values <= 0 set to 1 (*flag = 1, the caller
+ *     warns), then -log in the input's precision;
+ *   __dfi_single_slice (:784-806): row FFT (ifftshift / fftshift), window,
+ *     order-5 map_coordinates (mode "reflect") at generate_mapping_coordinate
+ *     (:687-728), circle mask, inverse 2D FFT, fftshift, real part, crop to
+ *     ncol x ncol (float64 arithmetic throughout);
+ *   ratio mask (:896-904): zero where x^2 + y^2 > mask_r2 about ncol//2
+ *     (mask_r2 < 0: none).
+ * Output element (y, j, x) at out[j*out_slice_stride + y*out_row_stride +
+ * x], float64 (the reference's result type) or float32. */
+int ath_dfi_reconstruction(const void* sino, int in_dtype, int nrow_in,
+                           int nslice, int ncol, long long row_stride,
+                           long long slice_stride, double xshift, int pad,
+                           int pad_mode, int apply_log, int angle_mode,
+                           double row_shift, int flip, int nrow,
+                           const void* plan, int width, void* workspace,
+                           size_t workspace_bytes, void* out, int out_dtype,
+                           long long out_row_stride,
+                           long long out_slice_stride, double mask_r2,
+                           int* flag, void* stream);
+
 /* ---- vertical slices (algotom/rec/vertrec.py) ------------------------- */
 
 /* typing of the vertical back-projection (which reference kernel) */

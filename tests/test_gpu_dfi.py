@@ -1,0 +1,229 @@
+"""DFI on the GPU against the oracle (oracle/dfi_np.py).
+
+Tolerances (max |gpu - oracle| / max |oracle|):
+* float64 sinograms: 1e-10 -- both sides compute in float64; they differ
+  by the order of float64 rounding (Bluestein vs numpy's FFT, summation
+  order of the spline sums);
+* float32 sinograms: 1e-6 -- the shifted sinogram and its log are rounded
+  to float32 on both sides, and a float64 difference at a rounding boundary
+  moves one sample by a float32 ulp.
+At the full detector width (2560 columns, Bluestein length 8192) where the
+oracle would take minutes: linearity in the sinogram (float64, apply_log
+off) and the analytic phantom's reconstruction error.
+"""
+
+import warnings
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(got, ref):
+    return float(np.abs(np.asarray(got) - ref).max() / np.abs(ref).max())
+
+
+def _sino(nrow, ncol, dtype, seed, low=0.5):
+    rng = np.random.default_rng(seed)
+    return (rng.random((nrow, ncol)) + low).astype(dtype)
+
+
+def _reference_case():
+    import scipy.ndimage as ndi
+    size = 64
+    mat = np.zeros((size + 1, size + 1), dtype=np.float32)
+    mat[20:40, 30:45] = np.float32(1.0)
+    mat = ndi.gaussian_filter(mat, 2.0)
+    angles = np.linspace(0.0, 360.0, 73, dtype=np.float32)
+    sino = np.zeros((73, size + 1), dtype=np.float32)
+    for i, ang in enumerate(angles):
+        sino[i] = np.sum(ndi.rotate(mat, -ang, reshape=False), axis=0)
+    return mat, sino, angles, size // 2
+
+
+def test_reference_test_case():
+    """tests/test_rec/test_reconstruction.py:204-237 on the GPU, each call
+    also against the oracle."""
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    mat, sino_360, angles, center = _reference_case()
+    sino_180, angles2 = sino_360[:37], angles[:37]
+    r1 = rec.dfi_reconstruction(sino_180, center, ratio=0.0, apply_log=False)
+    assert r1.dtype == np.float64 and r1.shape == (65, 65)
+    assert np.abs(mat - r1).max() <= 0.1
+    assert _rel(r1, D.dfi(sino_180, center, ratio=0.0, apply_log=False)) \
+        <= 1e-6
+    a360 = np.deg2rad(angles)
+    r2 = rec.dfi_reconstruction(sino_360, center, angles=a360,
+                                apply_log=False, ratio=None)
+    assert np.abs(mat - r2).max() <= 0.1
+    assert _rel(r2, D.dfi(sino_360, center, angles=a360, apply_log=False,
+                          ratio=None)) <= 1e-6
+    stack = np.pad(sino_180[:, None, :], ((0, 0), (2, 2), (0, 0)),
+                   mode="edge")
+    a2 = np.deg2rad(angles2)
+    r3 = rec.dfi_reconstruction(stack, center, apply_log=False, ratio=0.0,
+                                angles=a2, ncore=1)
+    assert r3.shape == (65, 5, 65)
+    assert np.abs(mat - r3[:, 0, :]).max() <= 0.1
+    assert _rel(r3, D.dfi(stack, center, apply_log=False, ratio=0.0,
+                          angles=a2)) <= 1e-6
+    r4 = rec.dfi_reconstruction(stack, center, apply_log=False, ratio=0.0,
+                                ncore=None)
+    assert np.abs(mat - r4[:, 0, :]).max() <= 0.1
+    with pytest.raises(ValueError):
+        rec.dfi_reconstruction(sino_180, center, np.deg2rad(angles),
+                               apply_log=False)
+    with pytest.warns(Warning):
+        rec.dfi_reconstruction(sino_180.copy(), center, apply_log=True)
+    with pytest.raises(ValueError):
+        rec.dfi_reconstruction(sino_180, -1, apply_log=False)
+
+
+@pytest.mark.parametrize(
+    "nrow,ncol,center,dtype,pad_mode,filt,pad_rate,ratio", [
+        (121, 96, 46.3, np.float32, "edge", "hann", 0.25, 1.0),
+        (121, 96, 20.2, np.float64, "edge", "hann", 0.25, 0.0),
+        (181, 128, 60.0, np.float64, "constant", None, 0.25, None),
+        (100, 81, 41.0, np.float32, "reflect", "hamming", 0.1, 1.0),
+        (90, 65, 30.7, np.float64, "symmetric", "blackman", 0.5, 0.8),
+        (64, 77, 38.0, np.float32, "wrap", "parzen", 0.0, 1.0),
+        (37, 64, 33.9, np.float32, "edge", "triang", 0.25, 1.0),
+    ])
+def test_dfi_vs_oracle(nrow, ncol, center, dtype, pad_mode, filt, pad_rate,
+                       ratio):
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    sino = _sino(nrow, ncol, dtype, ncol)
+    kw = dict(ratio=ratio, filter_name=filt, pad_rate=pad_rate,
+              pad_mode=pad_mode, apply_log=False)
+    got = rec.dfi_reconstruction(sino, center, **kw)
+    ref = D.dfi(sino, center, **kw)
+    err = _rel(got, ref)
+    print("%dx%d c=%.1f %s %s: %.2e" % (nrow, ncol, center,
+                                        np.dtype(dtype).name, pad_mode, err))
+    assert got.dtype == np.float64 and got.shape == (ncol, ncol)
+    assert err <= (1e-10 if dtype == np.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dfi_apply_log_vs_oracle(dtype):
+    """-log after shift and padding (reconstruction.py:862-867): values
+    <= 0 of the shifted, padded sinogram (always the constant pad's zeros;
+    the planted zero only if the cubic shift leaves it <= 0) are set to 1
+    with one warning, as the oracle does."""
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    sino = _sino(91, 70, dtype, 7, low=0.05)
+    sino[10, 5] = 0.0
+    for mode in ("edge", "constant"):
+        with warnings.catch_warnings(record=True) as wg:
+            warnings.simplefilter("always")
+            got = rec.dfi_reconstruction(sino.copy(), 34.6, pad_mode=mode)
+        with warnings.catch_warnings(record=True) as wr:
+            warnings.simplefilter("always")
+            ref = D.dfi(sino.copy(), 34.6, pad_mode=mode)
+        assert len(wg) == len(wr) and (mode == "edge" or len(wg) == 1)
+        assert _rel(got, ref) <= (1e-10 if dtype == np.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("first,last,n", [
+    (0.0, 180.0, 91),      # no offset
+    (17.0, 197.0, 91),     # fractional row shift
+    (180.0, 0.0, 91),      # descending: flipped
+    (0.0, 360.0, 121),     # folded to 61 rows
+    (-30.0, 150.0, 73),    # negative start
+    (0.0, 200.0, 101),     # neither 180 nor 360
+])
+def test_dfi_angles_vs_oracle(first, last, n):
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    sino = _sino(n, 72, np.float32, n)
+    angles = np.deg2rad(np.linspace(first, last, n))
+    got = rec.dfi_reconstruction(sino, 35.4, angles=angles, apply_log=False)
+    ref = D.dfi(sino, 35.4, angles=angles, apply_log=False)
+    err = _rel(got, ref)
+    print("angles %.0f..%.0f (%d): %.2e" % (first, last, n, err))
+    assert err <= 1e-6
+
+
+def test_dfi_stack_and_tensors():
+    """A (nrow, 3, ncol) stack -> (ncol, 3, ncol); a device tensor in gives
+    a float64 device tensor out; a (nrow, 1, ncol) stack keeps its axis."""
+    import torch
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    rng = np.random.default_rng(11)
+    stack = (rng.random((80, 3, 66)) + 0.5).astype(np.float32)
+    got = rec.dfi_reconstruction(stack, 31.2, apply_log=False)
+    ref = D.dfi(stack, 31.2, apply_log=False)
+    assert got.shape == (66, 3, 66) and _rel(got, ref) <= 1e-6
+    t = torch.from_numpy(stack).cuda()
+    out = rec.dfi_reconstruction(t, 31.2, apply_log=False)
+    assert isinstance(out, torch.Tensor) and out.is_cuda
+    assert out.dtype == torch.float64
+    np.testing.assert_array_equal(out.cpu().numpy(), got)
+    one = rec.dfi_reconstruction(stack[:, 1:2, :], 31.2, apply_log=False)
+    assert one.shape == (66, 1, 66)
+    np.testing.assert_array_equal(one[:, 0, :], got[:, 1, :])
+
+
+@pytest.mark.parametrize("nrow,ncol,center,pad_rate", [
+    (361, 1024, 511.3, 0.25),   # width 1537, Bluestein length 4096
+    (48, 2049, 1024.3, 0.0),    # width 2049, Bluestein length 8192
+])
+def test_dfi_large_vs_oracle(nrow, ncol, center, pad_rate):
+    from oracle import dfi_np as D
+    from algotom_amd.rec import reconstruction as rec
+    sino = _sino(nrow, ncol, np.float64, ncol)
+    got = rec.dfi_reconstruction(sino, center, pad_rate=pad_rate,
+                                 apply_log=False)
+    ref = D.dfi(sino, center, pad_rate=pad_rate, apply_log=False)
+    err = _rel(got, ref)
+    print("%dx%d: %.2e" % (nrow, ncol, err))
+    assert err <= 1e-10
+
+
+def test_dfi_full_width_linearity_and_phantom():
+    """1801 x 2560 (width 3841, Bluestein length 8192): the float64 path is
+    linear in the sinogram (apply_log off), and an analytic ellipse
+    phantom's sinogram reconstructs to the phantom."""
+    import torch
+    from algotom_amd.rec import reconstruction as rec
+    from algotom_amd.util import phantom
+    nrow, ncol, center = 1801, 2560, 1279.5
+    angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+    prm = phantom.random_ellipses(ncol, 12, seed=5)
+    p = phantom.ellipse_sinogram(prm, torch.from_numpy(angles), ncol,
+                                 center).cuda().double()
+    rng = np.random.default_rng(5)
+    q = torch.from_numpy(rng.random((nrow, ncol))).cuda()
+    rp = rec.dfi_reconstruction(p, center, apply_log=False)
+    rq = rec.dfi_reconstruction(q, center, apply_log=False)
+    rs = rec.dfi_reconstruction(2.0 * p - 3.0 * q, center, apply_log=False)
+    lin = (rs - (2.0 * rp - 3.0 * rq)).abs().max() / rs.abs().max()
+    print("linearity %.2e" % float(lin))
+    assert float(lin) <= 1e-10
+    img = phantom.ellipse_image(prm, ncol).astype(np.float64)
+    y, x = np.mgrid[:ncol, :ncol] - 0.5 * (ncol - 1)
+    inner = np.hypot(x, y) <= 0.45 * ncol
+    d = (rp.cpu().numpy() - img)[inner]
+    l2 = np.linalg.norm(d) / np.linalg.norm(img[inner])
+    med = np.median(np.abs(d)) / img.max()
+    # sharp ellipse edges ring (Gibbs): bound the L2 and median errors
+    # (the oracle gives 0.15 / 0.0014 at 512 columns)
+    print("phantom rel L2 %.3f, median %.4f" % (l2, med))
+    assert l2 <= 0.2 and med <= 0.005
+
+
+def test_dfi_limits_and_warnings():
+    from algotom_amd.rec import reconstruction as rec
+    with pytest.raises(NotImplementedError):
+        rec.dfi_reconstruction(_sino(64, 3300, np.float32, 1), 1650.0,
+                               apply_log=False)
+    with pytest.warns(Warning, match="undersampled"):
+        rec.dfi_reconstruction(_sino(12, 65, np.float32, 2), 32.0,
+                               apply_log=False)
+    with pytest.raises(ValueError):
+        rec.dfi_reconstruction(_sino(64, 65, np.float32, 3), 65.0)
