@@ -439,6 +439,53 @@ def run_bpf(args, dev):
     print(json.dumps(line))
 
 
+def run_dfi(args, dev):
+    """DFI of 16 sinograms 1801 x 2560 (float32 transmission, log on,
+    angles=None, hann, pad_rate 0.25: padded width 3841, Bluestein length
+    8192); value = slices reconstructed per second."""
+    import torch
+    from algotom_amd.rec import reconstruction as rec
+    nsl = 16
+    sino, _ = make_stack(dev, nslice=nsl)
+    ms, out = _timed(lambda: rec.dfi_reconstruction(sino, CENTER),
+                     args.steps, args.warmup)
+    one = sino[:, 0, :].contiguous()
+    ms1, out1 = _timed(lambda: rec.dfi_reconstruction(one, CENTER),
+                       args.steps, args.warmup)
+    cpu = None
+    detail = {"ms_per_slice": ms / nsl, "single_sinogram_ms": ms1,
+              "finite": bool(torch.isfinite(out).all().item())}
+    if not args.no_cpu_baseline:
+        from oracle import dfi_np
+        s_np = one.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = dfi_np.scipy_dfi_slice(s_np, CENTER)
+        dt = time.perf_counter() - t0
+        got = out1.cpu().numpy()
+        m = dfi_np.fbp_np.circle_mask(NCOL, 1.0)
+        detail["cpu_parity_max_rel"] = float(
+            np.abs(got - ref * m).max() / np.abs(ref * m).max())
+        cpu = {"value": 1.0 / dt, "unit": "slices/s", "cores": 1,
+               "kind": "port",
+               "sample": "1 of %d sinograms (%dx%d) through the reference's "
+                         "library calls (scipy.ndimage shift and "
+                         "map_coordinates, numpy FFTs; reconstruction.py"
+                         ":849-895), %.1f s" % (nsl, NROW, NCOL, dt)}
+    line = {
+        "metric": METRIC, "value": nsl / (ms * 1e-3),
+        "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (f32 input)",
+        "data": "synthetic",
+        "config": {"workload": "dfi_stack_16x1801x2560", "nslice": nsl,
+                   "nrow": NROW, "ncol": NCOL, "center": CENTER,
+                   "padded_width": 3841, "bluestein_length": 8192,
+                   "parallelism": "single-gpu"},
+        "cpu_baseline": cpu, "detail": detail,
+    }
+    print(json.dumps(line))
+
+
 def run_vertrec(args, dev):
     import torch
     from algotom_amd.rec import vertrec
@@ -485,7 +532,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="stack",
                     choices=["stack", "uneven", "refine", "volume",
-                             "center", "bpf", "vertrec"])
+                             "center", "bpf", "vertrec", "dfi"])
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -511,6 +558,8 @@ def main():
     if args.workload == "center":
         run_center(args, dev)
         return
+    if args.workload == "dfi":
+        return run_dfi(args, dev)
     if args.workload == "bpf":
         run_bpf(args, dev)
         return

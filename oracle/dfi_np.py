@@ -305,3 +305,48 @@ def dfi(sinogram, center, angles=None, ratio=1.0, filter_name="hann",
     if input_3d and num_sino == 1:
         recon = recon[:, None, :]
     return recon
+
+
+# --------------------------------------------------------------------------
+# The same slice through scipy.ndimage itself (the library calls the
+# reference makes): the CPU baseline of the DFI bench and a cross-check of
+# the restatement above.
+# --------------------------------------------------------------------------
+def scipy_single_slice(sino, window, mask, r_mat, theta_mat):
+    """__dfi_single_slice (reconstruction.py:784-806) with
+    scipy.ndimage.map_coordinates."""
+    import scipy.ndimage as ndi
+    spec = np.fft.fftshift(np.fft.fft(np.fft.ifftshift(
+        np.asarray(sino, dtype=np.float64), axes=1)), axes=1)
+    if window is not None:
+        spec = spec * window
+    coords = np.vstack((theta_mat.ravel(), r_mat.ravel()))
+    reg = ndi.map_coordinates(spec, coords, order=5,
+                              mode="reflect").reshape(r_mat.shape) * mask
+    return np.real(np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(reg))))
+
+
+def scipy_dfi_slice(sino, center, filter_name="hann", pad_rate=0.25,
+                    pad_mode="edge", apply_log=True):
+    """One 2D sinogram through reconstruction.py:849-895 (angles=None,
+    ratio=None) with scipy.ndimage.shift for the centre shift."""
+    import scipy.ndimage as ndi
+    sino = np.asarray(sino)
+    ncol = sino.shape[1]
+    if ncol % 2 == 0:
+        sino = np.pad(sino, ((0, 0), (0, 1)), mode="edge")
+    ncol1 = sino.shape[1]
+    pad = int(pad_rate * ncol1)
+    sino = ndi.shift(sino, (0, (ncol1 - 1) / 2.0 - center), mode="nearest")
+    sino = np.pad(sino, ((0, 0), (pad, pad)), mode=pad_mode)
+    if apply_log:
+        sino[sino <= 0.0] = np.float32(1.0)
+        sino = -np.log(sino)
+    nrow, w = sino.shape
+    r_mat, theta_mat = mapping_coordinates(w, nrow, w, w)
+    window = None
+    if filter_name is not None:
+        window = np.tile(fbp_np.smoothing_window(filter_name, w), (nrow, 1))
+    rec = scipy_single_slice(sino, window, fbp_np.circle_mask(w, 1.0), r_mat,
+                             theta_mat)
+    return rec[pad:ncol + pad, pad:ncol + pad]

@@ -177,15 +177,6 @@ def test_reference_thresholds():
         D.dfi(sino_180, -1, apply_log=False)
 
 
-def _scipy_single_slice(sino, window, mask, r_mat, theta_mat):
-    spec = np.fft.fftshift(np.fft.fft(np.fft.ifftshift(sino, axes=1)),
-                           axes=1) * window
-    coords = np.vstack((theta_mat.ravel(), r_mat.ravel()))
-    reg = ndi.map_coordinates(spec, coords, order=5,
-                              mode="reflect").reshape(r_mat.shape) * mask
-    return np.real(np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(reg))))
-
-
 @pytest.mark.parametrize("ncol,nrow,center", [(96, 121, 46.3), (81, 60, 41.0)])
 def test_pipeline_against_scipy_composition(ncol, nrow, center):
     """handle_sinogram + single_slice against the same steps composed from
@@ -204,6 +195,9 @@ def test_pipeline_against_scipy_composition(ncol, nrow, center):
     r_mat, theta_mat = D.mapping_coordinates(w, nrow, w, w)
     window = np.tile(fbp_np.smoothing_window("hann", w), (nrow, 1))
     got = D.single_slice(ref_s, window, mask, r_mat, theta_mat)
-    ref = _scipy_single_slice(ref_s.astype(np.float64), window, mask, r_mat,
-                              theta_mat)
+    ref = D.scipy_single_slice(ref_s.astype(np.float64), window, mask, r_mat,
+                               theta_mat)
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    whole = D.scipy_dfi_slice(sino, center, apply_log=False)
+    mine = D.dfi(sino, center, ratio=None, apply_log=False)
+    assert np.abs(whole - mine).max() <= 1e-12 * np.abs(whole).max()
