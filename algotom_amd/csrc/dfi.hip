@@ -13,19 +13,25 @@
 //                           along the angle axis (order 3, mode "wrap");
 //   3. dfi_rows_fwd_kernel  pad (numpy.pad modes), -log, ifftshift, the
 //                           length-N DFT by Bluestein's chirp-z method over
-//                           a power-of-two LDS FFT, fftshift, smoothing
-//                           window, then the quintic B-spline prefilter
-//                           along the row (:787-790 and the first axis of
+//                           a power-of-two LDS FFT (two real rows per
+//                           complex transform), fftshift, smoothing window,
+//                           then the quintic B-spline prefilter along the
+//                           row (:787-790 and the first axis of
 //                           map_coordinates' prefilter);
 //   4. dfi_cols_kernel      the quintic prefilter along the angle axis;
 //   5. dfi_grid_kernel      generate_mapping_coordinate (:687-728) per
 //                           Cartesian frequency, order-5 evaluation on the
 //                           (theta, r) grid (mode "reflect"), the circle
-//                           mask, written ifftshifted and transposed;
-//   6. dfi_rows_inv_kernel  inverse DFT along the first axis (Bluestein),
-//                           keeping the rows the crop keeps; transpose;
-//      dfi_rows_inv_kernel  inverse DFT along the second axis, fftshift,
-//                           real part, crop [pad, pad + ncol), ratio mask.
+//                           mask, written ifftshifted and transposed as the
+//                           Hermitian part (the real part of the inverse
+//                           depends on nothing else), half of it;
+//   6. dfi_cols_inv_kernel  inverse DFT along the first axis (Bluestein) of
+//                           that half, keeping the rows the crop keeps;
+//      dfi_transpose_kernel;
+//      dfi_rows_out_kernel  inverse DFT along the second axis of the
+//                           Hermitian rows (two real rows per transform),
+//                           fftshift, crop [pad, pad + ncol), ratio mask --
+//                           the real part the reference takes (:800-801).
 //
 // Spline conventions (scipy.ndimage 1.15): prefilter = the exact inverse
 // of the sampled B-spline, applied as its impulse response h (closed form
@@ -206,10 +212,12 @@ __global__ __launch_bounds__(kThreads) void dfi_shift_kernel(
     int ncol1, double xshift, const double* __restrict__ h3,
     double* __restrict__ S) {
     extern __shared__ double sh[];
+    __shared__ double hs[kK3 + 1];
     const int L1 = ncol1 + 2 * kNearestPad;
     double* line = sh;
     double* coef = sh + L1;
     const int tid = threadIdx.x;
+    if (tid <= kK3) hs[tid] = h3[tid];
     for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
         const Tin* src = in + (long long)row * row_stride;
         for (int p = tid; p < L1; p += kThreads) {
@@ -219,9 +227,9 @@ __global__ __launch_bounds__(kThreads) void dfi_shift_kernel(
         }
         __syncthreads();
         for (int p = tid; p < L1; p += kThreads) {
-            double c = h3[0] * line[p];
+            double c = hs[0] * line[p];
             for (int t = 1; t <= kK3; ++t)
-                c += h3[t] * (line[fold_half(p - t, L1)] +
+                c += hs[t] * (line[fold_half(p - t, L1)] +
                               line[fold_half(p + t, L1)]);
             coef[p] = c;
         }
@@ -299,26 +307,50 @@ __global__ __launch_bounds__(kThreads) void dfi_angles_kernel(
 // convolution of length M >= 2N - 1:  DFT(a)[k] = chirp[k] * conv[k],
 // conv = IFFT_M(FFT_M(chirp * a) * bhat).  On entry buf[n] = chirp[n] a[n]
 // (n < N, zero up to M); on exit buf[k] = conv[k] (bhat carries the 1/M).
-template <int MAXB, bool QT>
+template <int MAXB, bool QT, int NT>
 __device__ __forceinline__ void bluestein_core(double2* buf, int M,
                                                const double2* __restrict__ tw,
                                                const double2* tq,
                                                const double2* __restrict__ bhat) {
-    lds_fft<-1, MAXB, QT>(buf, M, tw, tq);
-    for (int k = threadIdx.x; k < M; k += kFftThreads)
+    lds_fft<-1, MAXB, QT, NT>(buf, M, tw, tq);
+    for (int k = threadIdx.x; k < M; k += NT)
         buf[k] = cmul(buf[k], bhat[k]);
     __syncthreads();
-    lds_fft<+1, MAXB, QT>(buf, M, tw, tq);
+    lds_fft<+1, MAXB, QT, NT>(buf, M, tw, tq);
 }
 
+// The quarter twiddle table sits in LDS beside the buffer for M <= 4096.
+template <int MAXB, int NT>
+constexpr bool quarter_table() { return MAXB * NT <= 1024; }
+
 // ---- 3. row spectra + row prefilter ----------------------------------------
-template <int MAXB, typename Tr>
-__global__ __launch_bounds__(kFftThreads) void dfi_rows_fwd_kernel(
+// -log of one padded sample in the input's precision; values <= 0 become 1
+// and raise the flag (reconstruction.py:862-867)
+template <typename Tr>
+__device__ __forceinline__ double minus_log(double v, int* flag) {
+    if (v <= 0.0) {
+        v = 1.0;
+        *flag = 1;
+    }
+    return round_to<Tr>(-log(v));
+}
+
+// Consecutive row-prefilter outputs per thread.  Odd, so that the lanes'
+// 16-byte LDS reads, kFirRun * 16 bytes apart, fall on distinct banks.
+constexpr int kFirRun = 5;
+
+// Two real rows per Bluestein transform: z = x1 + i x2, Z = DFT(z), then
+// X1[k] = (Z[k] + conj Z[N-k]) / 2 and X2[k] = -i (Z[k] - conj Z[N-k]) / 2.
+// Each spectrum times the window at its fftshifted index kp = (k - m - 1)
+// mod N stays in natural order (X1 in buf[0, N), X2 in buf[N, 2N); M >= 2N);
+// the prefilter reads spec[kp] = buf[(kp + m + 1) mod N].
+template <int MAXB, int NT, typename Tr>
+__global__ __launch_bounds__(NT) void dfi_rows_fwd_kernel(
     const double* __restrict__ S, int rows, int ncol1, int pad, int pad_mode,
     int apply_log, const void* __restrict__ plan, int N, int M, size_t o_tw,
     size_t o_chirp, size_t o_bhat, size_t o_win, size_t o_h5,
     double2* __restrict__ X, int* __restrict__ flag) {
-    constexpr bool QT = MAXB <= 4;
+    constexpr bool QT = quarter_table<MAXB, NT>();
     extern __shared__ double2 lds[];
     double2* buf = lds;
     double2* tq = lds + M;
@@ -329,46 +361,88 @@ __global__ __launch_bounds__(kFftThreads) void dfi_rows_fwd_kernel(
     const double* h5 = at<double>(plan, o_h5);
     const int tid = threadIdx.x;
     const int m = (N - 1) / 2;
-    if (QT) stage_quarter(tq, tw, M);
-    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
-        const double* src = S + (size_t)row * ncol1;
-        for (int n = tid; n < M; n += kFftThreads) {
+    const int npairs = (rows + 1) / 2;
+    if (QT) stage_quarter<NT>(tq, tw, M);
+    for (int pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+        const int r1 = 2 * pr;
+        const bool two = r1 + 1 < rows;
+        const double* s1 = S + (size_t)r1 * ncol1;
+        const double* s2 = s1 + ncol1;
+        for (int n = tid; n < M; n += NT) {
             double2 a = make_double2(0.0, 0.0);
             if (n < N) {
                 int u = n + m;  // ifftshift
                 if (u >= N) u -= N;
                 const int c = pad_source(u - pad, ncol1, pad_mode);
-                double v = c >= 0 ? src[c] : 0.0;
+                double v1 = c >= 0 ? s1[c] : 0.0;
+                double v2 = (two && c >= 0) ? s2[c] : 0.0;
                 if (apply_log) {
-                    if (v <= 0.0) {
-                        v = 1.0;
-                        *flag = 1;
-                    }
-                    v = round_to<Tr>(-log(v));
+                    v1 = minus_log<Tr>(v1, flag);
+                    if (two) v2 = minus_log<Tr>(v2, flag);
                 }
-                a = cscale(chirp[n], v);
+                const double2 ch = chirp[n];
+                a = make_double2(ch.x * v1 - ch.y * v2, ch.x * v2 + ch.y * v1);
             }
             buf[n] = a;
         }
         __syncthreads();
-        bluestein_core<MAXB, QT>(buf, M, tw, tq, bhat);
-        // spectrum in fftshift order times the window, kept at buf[N + k']
-        // (M >= 2N, so it does not overlap buf[0, N))
-        for (int kp = tid; kp < N; kp += kFftThreads) {
-            int k = kp + m + 1;
-            if (k >= N) k -= N;
-            buf[N + kp] = cscale(cmul(chirp[k], buf[k]), win[kp]);
+        bluestein_core<MAXB, QT, NT>(buf, M, tw, tq, bhat);
+        for (int k = tid; k <= m; k += NT) {
+            const int kn = k == 0 ? 0 : N - k;
+            const double2 zk = cmul(chirp[k], buf[k]);
+            const double2 zn = cmul(chirp[kn], buf[kn]);
+            const double2 x1 =
+                make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+            const double2 x2 =
+                make_double2(0.5 * (zk.y + zn.y), 0.5 * (zn.x - zk.x));
+            const int pk = k > m ? k - m - 1 : k + m;  // (k - m - 1) mod N
+            const int pn = kn > m ? kn - m - 1 : kn + m;
+            buf[k] = cscale(x1, win[pk]);
+            buf[N + k] = cscale(x2, win[pk]);
+            if (k != 0) {
+                buf[kn] = cscale(conjg(x1), win[pn]);
+                buf[N + kn] = cscale(conjg(x2), win[pn]);
+            }
         }
         __syncthreads();
-        const double2* sp = buf + N;
-        double2* dst = X + (size_t)row * N;
-        for (int kp = tid; kp < N; kp += kFftThreads) {
-            double2 acc = cscale(sp[kp], h5[0]);
-            for (int t = 1; t <= kK5; ++t)
-                acc = cadd(acc, cscale(cadd(sp[fold_half(kp - t, N)],
-                                            sp[fold_half(kp + t, N)]),
-                                       h5[t]));
-            dst[kp] = acc;
+        // quintic prefilter along each row (half-sample symmetric in kp),
+        // kFirRun outputs per thread sharing their loads
+        double2* d1 = X + (size_t)r1 * N;
+        double2* d2 = d1 + N;
+        const int ngroups = (N + kFirRun - 1) / kFirRun;
+        for (int g = tid; g < ngroups; g += NT) {
+            const int k0 = g * kFirRun;
+            double2 a1[kFirRun], a2[kFirRun];
+#pragma unroll
+            for (int r = 0; r < kFirRun; ++r) {
+                a1[r] = make_double2(0.0, 0.0);
+                a2[r] = make_double2(0.0, 0.0);
+            }
+            for (int i = 0; i < kFirRun + 2 * kK5; ++i) {
+                int src = fold_half(k0 - kK5 + i, N) + m + 1;
+                if (src >= N) src -= N;
+                const double2 v1 = buf[src];
+                const double2 v2 = buf[N + src];
+#pragma unroll
+                for (int r = 0; r < kFirRun; ++r) {
+                    int d = i - kK5 - r;
+                    d = d < 0 ? -d : d;
+                    if (d <= kK5) {
+                        const double h = h5[d];
+                        a1[r].x += h * v1.x;
+                        a1[r].y += h * v1.y;
+                        a2[r].x += h * v2.x;
+                        a2[r].y += h * v2.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kFirRun; ++r) {
+                if (k0 + r < N) {
+                    d1[k0 + r] = a1[r];
+                    if (two) d2[k0 + r] = a2[r];
+                }
+            }
         }
         __syncthreads();
     }
@@ -383,6 +457,9 @@ constexpr int kColRun = 16;
 __global__ __launch_bounds__(kThreads) void dfi_cols_kernel(
     const double2* __restrict__ X, int rows, int N,
     const double* __restrict__ h5, double2* __restrict__ out) {
+    __shared__ double hs[kK5 + 1];
+    if (threadIdx.x <= kK5) hs[threadIdx.x] = h5[threadIdx.x];
+    __syncthreads();
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kColRun;
     if (c >= N || r0 >= rows) return;
@@ -397,7 +474,7 @@ __global__ __launch_bounds__(kThreads) void dfi_cols_kernel(
             int d = i - kK5 - q;
             d = d < 0 ? -d : d;
             if (d <= kK5) {
-                const double h = h5[d];
+                const double h = hs[d];
                 acc[q].x += h * v.x;
                 acc[q].y += h * v.y;
             }
@@ -409,16 +486,60 @@ __global__ __launch_bounds__(kThreads) void dfi_cols_kernel(
 }
 
 // ---- 5. gridding --------------------------------------------------------------
-// T[v][u] = G[(u + m) % N][(v + m) % N] (ifftshift, transposed), G[y][x] =
-// mask ? sum beta5 beta5 coef : 0 at the float32 (theta, r) of
-// generate_mapping_coordinate: X = m - x, Y = y - m.
+// G[y][x] = mask ? sum beta5 beta5 coef : 0 at the float32 (theta, r) of
+// generate_mapping_coordinate, X = m - x, Y = y - m (float64 sqrt / atan2 on
+// the integer grid, rounded as the reference rounds them).
+__device__ __forceinline__ double2 grid_value(const double2* __restrict__ coef,
+                                              int rows, int N, int m, int X,
+                                              int Y) {
+    double2 val = make_double2(0.0, 0.0);
+    if (X * X + Y * Y > m * m) return val;
+    const double r_max = (double)m;
+    float r = (float)fmin(fmax(sqrt((double)(X * X + Y * Y)), 0.0), r_max);
+    double theta = kPi + atan2((double)Y, (double)X);
+    if (theta > kPi) {
+        r = -r;
+        theta -= kPi;
+    }
+    const double rr =
+        (double)(float)fmin(fmax((double)r + r_max, 0.0), (double)(N - 1));
+    const double tt = (double)(float)(theta * ((double)rows - 1.0) / kPi);
+    const int s0 = (int)floor(tt) - 2, s1 = (int)floor(rr) - 2;
+    double wx[6];
+    int cx[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        wx[j] = beta5(rr - (double)(s1 + j));
+        cx[j] = fold_half(s1 + j, N);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double wy = beta5(tt - (double)(s0 + i));
+        const double2* crow = coef + (size_t)fold_half(s0 + i, rows) * N;
+        double2 rs = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const double2 c = crow[cx[j]];
+            rs.x += wx[j] * c.x;
+            rs.y += wx[j] * c.y;
+        }
+        val.x += wy * rs.x;
+        val.y += wy * rs.y;
+    }
+    return val;
+}
+
+// T[v][u] = H[u][v] for the columns v <= m, H the Hermitian part
+// (Gs[u][v] + conj Gs[-u][-v]) / 2 of Gs = ifftshift(G) (Gs[u][v] =
+// G[(u + m) % N][(v + m) % N]), transposed.  The reference keeps
+// Re(ifft2(Gs)) = ifft2(H); the columns v > m of H are the conjugates of
+// these and are never formed.  (G itself is Hermitian only to the float32
+// rounding of its coordinates, r' = m -+ r, hence the explicit average.)
 __global__ __launch_bounds__(kThreads) void dfi_grid_kernel(
     const double2* __restrict__ coef, int rows, int N,
     double2* __restrict__ T) {
     const int m = (N - 1) / 2;
-    const double r_max = (double)m;
-    const double hs1 = (double)rows - 1.0;
-    for (int v = blockIdx.y; v < N; v += gridDim.y) {
+    for (int v = blockIdx.y; v <= m; v += gridDim.y) {
         int x = v + m;
         if (x >= N) x -= N;
         const int X = m - x;
@@ -428,59 +549,29 @@ __global__ __launch_bounds__(kThreads) void dfi_grid_kernel(
             int y = u + m;
             if (y >= N) y -= N;
             const int Y = y - m;
-            double2 val = make_double2(0.0, 0.0);
-            if (X * X + Y * Y <= m * m) {
-                float r = (float)fmin(fmax(sqrt((double)(X * X + Y * Y)), 0.0),
-                                      r_max);
-                double theta = kPi + atan2((double)Y, (double)X);
-                if (theta > kPi) {
-                    r = -r;
-                    theta -= kPi;
-                }
-                const double rr = (double)(float)fmin(
-                    fmax((double)r + r_max, 0.0), (double)(N - 1));
-                const double tt = (double)(float)(theta * hs1 / kPi);
-                const int s0 = (int)floor(tt) - 2, s1 = (int)floor(rr) - 2;
-                double wx[6];
-#pragma unroll
-                for (int j = 0; j < 6; ++j) wx[j] = beta5(rr - (double)(s1 + j));
-                int cx[6];
-#pragma unroll
-                for (int j = 0; j < 6; ++j) cx[j] = fold_half(s1 + j, N);
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    const double wy = beta5(tt - (double)(s0 + i));
-                    const double2* crow =
-                        coef + (size_t)fold_half(s0 + i, rows) * N;
-                    double2 rs = make_double2(0.0, 0.0);
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        const double2 c = crow[cx[j]];
-                        rs.x += wx[j] * c.x;
-                        rs.y += wx[j] * c.y;
-                    }
-                    val.x += wy * rs.x;
-                    val.y += wy * rs.y;
-                }
-            }
-            dst[u] = val;
+            const double2 a = grid_value(coef, rows, N, m, X, Y);
+            const double2 b = grid_value(coef, rows, N, m, -X, -Y);
+            dst[u] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
         }
     }
 }
 
 // ---- 6. inverse DFTs ----------------------------------------------------------
-// Row-wise inverse DFT (unnormalised transform / N) of `in` (nrows x N) by
-// Bluestein on the conjugate; each row keeps ncol outputs, output index
-// kc <- DFT index (kc + pad + m + 1) % N (fftshift, then the crop).
-// FINAL = false: out (nrows x ncol) complex.  FINAL = true: real part times
-// the ratio mask into the caller's output (row `row` of the slice).
-template <int MAXB, bool FINAL, typename To>
-__global__ __launch_bounds__(kFftThreads) void dfi_rows_inv_kernel(
+// Output index kc (crop) <- DFT index (kc + pad + m + 1) mod N (fftshift).
+__device__ __forceinline__ int crop_index(int kc, int pad, int m, int N) {
+    int k = kc + pad + m + 1;  // < 2N
+    return k >= N ? k - N : k;
+}
+
+// Stage A: inverse DFT (transform / N) along the first axis for the columns
+// v <= m: rows of `in` (nrows x N), by Bluestein on the conjugate; ncol
+// outputs kept per row -> out (nrows x ncol).
+template <int MAXB, int NT>
+__global__ __launch_bounds__(NT) void dfi_cols_inv_kernel(
     const double2* __restrict__ in, int nrows, int ncol, int pad,
     const void* __restrict__ plan, int N, int M, size_t o_tw, size_t o_chirp,
-    size_t o_bhat, double2* __restrict__ cout, To* __restrict__ rout,
-    long long out_row_stride, double mask_r2) {
-    constexpr bool QT = MAXB <= 4;
+    size_t o_bhat, double2* __restrict__ out) {
+    constexpr bool QT = quarter_table<MAXB, NT>();
     extern __shared__ double2 lds[];
     double2* buf = lds;
     double2* tq = lds + M;
@@ -490,30 +581,78 @@ __global__ __launch_bounds__(kFftThreads) void dfi_rows_inv_kernel(
     const int tid = threadIdx.x;
     const int m = (N - 1) / 2;
     const double invN = 1.0 / (double)N;
-    const int mc = ncol / 2;
-    if (QT) stage_quarter(tq, tw, M);
+    if (QT) stage_quarter<NT>(tq, tw, M);
     for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
         const double2* src = in + (size_t)row * N;
-        for (int n = tid; n < M; n += kFftThreads)
+        for (int n = tid; n < M; n += NT)
             buf[n] = n < N ? cmul(chirp[n], conjg(src[n]))
                            : make_double2(0.0, 0.0);
         __syncthreads();
-        bluestein_core<MAXB, QT>(buf, M, tw, tq, bhat);
-        for (int kc = tid; kc < ncol; kc += kFftThreads) {
-            int k = kc + pad + m + 1;  // < 2N
-            if (k >= N) k -= N;
-            const double2 y = conjg(cmul(chirp[k], buf[k]));
-            if constexpr (FINAL) {
-                const double re = y.x * invN;
-                double v = re;
-                if (mask_r2 >= 0.0) {
-                    const int dy = row - mc, dx = kc - mc;
-                    v = (double)(dy * dy + dx * dx) <= mask_r2 ? re : re * 0.0;
-                }
-                rout[(long long)row * out_row_stride + kc] = (To)v;
-            } else {
-                cout[(size_t)row * ncol + kc] = cscale(y, invN);
+        bluestein_core<MAXB, QT, NT>(buf, M, tw, tq, bhat);
+        for (int kc = tid; kc < ncol; kc += NT) {
+            const int k = crop_index(kc, pad, m, N);
+            out[(size_t)row * ncol + kc] =
+                cscale(conjg(cmul(chirp[k], buf[k])), invN);
+        }
+        __syncthreads();
+    }
+}
+
+// Stage B: each output row Y has the first-pass values a[v] for v <= m
+// (row Y of `in`, m + 1 values) and a[N - v] = conj(a[v]) (Hermitian), so
+// its inverse DFT is real.  Two rows per transform: c = a1 + i a2 ->
+// IDFT(c) = y1 + i y2.  Real parts cropped, the ratio mask (mask_r2 < 0:
+// none; zero times the value keeps its sign, as the reference's float
+// mask product), stored as To at out[Y * out_row_stride + kc].
+template <int MAXB, int NT, typename To>
+__global__ __launch_bounds__(NT) void dfi_rows_out_kernel(
+    const double2* __restrict__ in, int ncol, int pad,
+    const void* __restrict__ plan, int N, int M, size_t o_tw, size_t o_chirp,
+    size_t o_bhat, To* __restrict__ out, long long out_row_stride,
+    double mask_r2) {
+    constexpr bool QT = quarter_table<MAXB, NT>();
+    extern __shared__ double2 lds[];
+    double2* buf = lds;
+    double2* tq = lds + M;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const int tid = threadIdx.x;
+    const int m = (N - 1) / 2;
+    const int h = m + 1;
+    const double invN = 1.0 / (double)N;
+    const int mc = ncol / 2;
+    const int npairs = (ncol + 1) / 2;
+    if (QT) stage_quarter<NT>(tq, tw, M);
+    for (int pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+        const int y1 = 2 * pr;
+        const bool two = y1 + 1 < ncol;
+        const double2* a1 = in + (size_t)y1 * h;
+        const double2* a2 = a1 + h;
+        for (int n = tid; n < M; n += NT) {
+            double2 c = make_double2(0.0, 0.0);
+            if (n < N) {
+                const double2 p = n <= m ? a1[n] : conjg(a1[N - n]);
+                double2 q = make_double2(0.0, 0.0);
+                if (two) q = n <= m ? a2[n] : conjg(a2[N - n]);
+                // conj(p + i q), times the chirp
+                c = cmul(chirp[n], make_double2(p.x - q.y, -(p.y + q.x)));
             }
+            buf[n] = c;
+        }
+        __syncthreads();
+        bluestein_core<MAXB, QT, NT>(buf, M, tw, tq, bhat);
+        for (int kc = tid; kc < ncol; kc += NT) {
+            const int k = crop_index(kc, pad, m, N);
+            const double2 y = conjg(cmul(chirp[k], buf[k]));
+            double v1 = y.x * invN, v2 = y.y * invN;
+            if (mask_r2 >= 0.0) {
+                const int dx = kc - mc, d1 = y1 - mc, d2 = y1 + 1 - mc;
+                if ((double)(d1 * d1 + dx * dx) > mask_r2) v1 = v1 * 0.0;
+                if ((double)(d2 * d2 + dx * dx) > mask_r2) v2 = v2 * 0.0;
+            }
+            out[(long long)y1 * out_row_stride + kc] = (To)v1;
+            if (two) out[(long long)(y1 + 1) * out_row_stride + kc] = (To)v2;
         }
         __syncthreads();
     }
@@ -543,39 +682,54 @@ __global__ __launch_bounds__(kThreads) void dfi_transpose_kernel(
 }
 
 // ---- dispatch ---------------------------------------------------------------------
-template <int MAXB>
-size_t fft_lds_bytes(int M) {
+// Transform lengths up to 4096 run with 256 threads (MAXB butterflies each,
+// the quarter twiddle table in LDS); 8192 runs with 1024 threads of 2
+// butterflies: four waves per SIMD for the single workgroup that its 128 KB
+// of LDS leaves room for (measured 1.6x faster than 256 threads of 8).
+inline size_t fft_lds_bytes(int M) {
     return (size_t)M * sizeof(double2) +
-           (MAXB <= 4 ? (size_t)(M / 4) * sizeof(double2) : 0);
+           (M <= 4096 ? (size_t)(M / 4) * sizeof(double2) : 0);
 }
 
-template <int MAXB, typename Tr>
+template <int MAXB, int NT, typename Tr>
 int launch_rows_fwd(const double* S, int rows, int ncol1, int pad,
                     int pad_mode, int apply_log, const void* plan,
                     const DfiPlan& p, double2* X, int* flag, hipStream_t s) {
-    auto k = dfi_rows_fwd_kernel<MAXB, Tr>;
-    const size_t lds = fft_lds_bytes<MAXB>(p.M);
+    auto k = dfi_rows_fwd_kernel<MAXB, NT, Tr>;
+    const size_t lds = fft_lds_bytes(p.M);
     prepare_lds(k, lds);
-    const int grid = std::min(rows, resident_blocks(k, lds));
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kFftThreads), lds, s, S, rows,
-                       ncol1, pad, pad_mode, apply_log, plan, p.N, p.M, p.tw,
-                       p.chirp, p.bhat, p.window, p.h5, X, flag);
+    const int grid = std::min((rows + 1) / 2, resident_blocks(k, lds, NT));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, S, rows, ncol1, pad,
+                       pad_mode, apply_log, plan, p.N, p.M, p.tw, p.chirp,
+                       p.bhat, p.window, p.h5, X, flag);
     return check_hip(hipGetLastError(), "dfi_rows_fwd_kernel launch");
 }
 
-template <int MAXB, bool FINAL, typename To>
-int launch_rows_inv(const double2* in, int nrows, int ncol, int pad,
-                    const void* plan, const DfiPlan& p, double2* cout,
-                    To* rout, long long out_row_stride, double mask_r2,
+template <int MAXB, int NT>
+int launch_cols_inv(const double2* in, int nrows, int ncol, int pad,
+                    const void* plan, const DfiPlan& p, double2* out,
                     hipStream_t s) {
-    auto k = dfi_rows_inv_kernel<MAXB, FINAL, To>;
-    const size_t lds = fft_lds_bytes<MAXB>(p.M);
+    auto k = dfi_cols_inv_kernel<MAXB, NT>;
+    const size_t lds = fft_lds_bytes(p.M);
     prepare_lds(k, lds);
-    const int grid = std::min(nrows, resident_blocks(k, lds));
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kFftThreads), lds, s, in, nrows,
-                       ncol, pad, plan, p.N, p.M, p.tw, p.chirp, p.bhat, cout,
-                       rout, out_row_stride, mask_r2);
-    return check_hip(hipGetLastError(), "dfi_rows_inv_kernel launch");
+    const int grid = std::min(nrows, resident_blocks(k, lds, NT));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, in, nrows, ncol, pad,
+                       plan, p.N, p.M, p.tw, p.chirp, p.bhat, out);
+    return check_hip(hipGetLastError(), "dfi_cols_inv_kernel launch");
+}
+
+template <int MAXB, int NT, typename To>
+int launch_rows_out(const double2* in, int ncol, int pad, const void* plan,
+                    const DfiPlan& p, To* out, long long out_row_stride,
+                    double mask_r2, hipStream_t s) {
+    auto k = dfi_rows_out_kernel<MAXB, NT, To>;
+    const size_t lds = fft_lds_bytes(p.M);
+    prepare_lds(k, lds);
+    const int grid = std::min((ncol + 1) / 2, resident_blocks(k, lds, NT));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, in, ncol, pad, plan,
+                       p.N, p.M, p.tw, p.chirp, p.bhat, out, out_row_stride,
+                       mask_r2);
+    return check_hip(hipGetLastError(), "dfi_rows_out_kernel launch");
 }
 
 template <typename Tr>
@@ -583,38 +737,52 @@ int rows_fwd(int maxb, const double* S, int rows, int ncol1, int pad,
              int pad_mode, int apply_log, const void* plan, const DfiPlan& p,
              double2* X, int* flag, hipStream_t s) {
     switch (maxb) {
-    case 1: return launch_rows_fwd<1, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
-    case 2: return launch_rows_fwd<2, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
-    case 4: return launch_rows_fwd<4, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
-    default: return launch_rows_fwd<8, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    case 1: return launch_rows_fwd<1, 256, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    case 2: return launch_rows_fwd<2, 256, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    case 4: return launch_rows_fwd<4, 256, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
+    default: return launch_rows_fwd<2, 1024, Tr>(S, rows, ncol1, pad, pad_mode, apply_log, plan, p, X, flag, s);
     }
 }
 
-template <bool FINAL, typename To>
-int rows_inv(int maxb, const double2* in, int nrows, int ncol, int pad,
-             const void* plan, const DfiPlan& p, double2* cout, To* rout,
-             long long out_row_stride, double mask_r2, hipStream_t s) {
+int cols_inv(int maxb, const double2* in, int nrows, int ncol, int pad,
+             const void* plan, const DfiPlan& p, double2* out, hipStream_t s) {
     switch (maxb) {
-    case 1: return launch_rows_inv<1, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
-    case 2: return launch_rows_inv<2, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
-    case 4: return launch_rows_inv<4, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
-    default: return launch_rows_inv<8, FINAL, To>(in, nrows, ncol, pad, plan, p, cout, rout, out_row_stride, mask_r2, s);
+    case 1: return launch_cols_inv<1, 256>(in, nrows, ncol, pad, plan, p, out, s);
+    case 2: return launch_cols_inv<2, 256>(in, nrows, ncol, pad, plan, p, out, s);
+    case 4: return launch_cols_inv<4, 256>(in, nrows, ncol, pad, plan, p, out, s);
+    default: return launch_cols_inv<2, 1024>(in, nrows, ncol, pad, plan, p, out, s);
     }
 }
 
+template <typename To>
+int rows_out(int maxb, const double2* in, int ncol, int pad, const void* plan,
+             const DfiPlan& p, To* out, long long out_row_stride,
+             double mask_r2, hipStream_t s) {
+    switch (maxb) {
+    case 1: return launch_rows_out<1, 256, To>(in, ncol, pad, plan, p, out, out_row_stride, mask_r2, s);
+    case 2: return launch_rows_out<2, 256, To>(in, ncol, pad, plan, p, out, out_row_stride, mask_r2, s);
+    case 4: return launch_rows_out<4, 256, To>(in, ncol, pad, plan, p, out, out_row_stride, mask_r2, s);
+    default: return launch_rows_out<2, 1024, To>(in, ncol, pad, plan, p, out, out_row_stride, mask_r2, s);
+    }
+}
+
+// Workspace: S1 (shifted sinogram), S2 (angle-handled), and two ping-pong
+// regions: R1 holds the row coefficients (rows x N), then the half grid
+// ((m + 1) x N), then the transposed first pass (ncol x (m + 1)); R2 holds
+// the column coefficients (rows x N), then the first pass ((m + 1) x ncol).
 struct DfiWork {
     size_t s1, s2, r1, r2, total;
 };
 
 DfiWork work_layout(int nrow_in, int ncol, int nrow, int N) {
     const int ncol1 = ncol + (ncol % 2 == 0 ? 1 : 0);
+    const size_t h = (size_t)(N - 1) / 2 + 1;
     DfiWork w{};
     size_t o = 0;
     w.s1 = o; o = align256(o + (size_t)nrow_in * ncol1 * sizeof(double));
     w.s2 = o; o = align256(o + (size_t)nrow * ncol1 * sizeof(double));
-    const size_t r1 = std::max({(size_t)nrow * N, (size_t)N * N,
-                                (size_t)ncol * N});
-    const size_t r2 = std::max((size_t)nrow * N, (size_t)N * ncol);
+    const size_t r1 = std::max({(size_t)nrow * N, h * N, (size_t)ncol * h});
+    const size_t r2 = std::max((size_t)nrow * N, h * ncol);
     w.r1 = o; o = align256(o + r1 * sizeof(double2));
     w.r2 = o; o = align256(o + r2 * sizeof(double2));
     w.total = o;
@@ -736,18 +904,18 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
             st = check_hip(hipGetLastError(), "dfi_cols_kernel launch");
             if (st != ATH_OK) return st;
         }
-        hipLaunchKernelGGL(dfi_grid_kernel, row_grid(p.N, p.N, kThreads),
+        const int h = (p.N - 1) / 2 + 1;  // columns v <= m
+        hipLaunchKernelGGL(dfi_grid_kernel, row_grid(h, p.N, kThreads),
                            dim3(kThreads), 0, s, R2, nrow, p.N, R1);
         st = check_hip(hipGetLastError(), "dfi_grid_kernel launch");
         if (st != ATH_OK) return st;
-        st = rows_inv<false, double>(maxb, R1, p.N, ncol, pad, plan, p, R2,
-                                     nullptr, 0, -1.0, s);
+        st = cols_inv(maxb, R1, h, ncol, pad, plan, p, R2, s);
         if (st != ATH_OK) return st;
         {
             const dim3 grid(std::min((ncol + 31) / 32, 65535),
-                            std::min((p.N + 31) / 32, 65535));
+                            std::min((h + 31) / 32, 65535));
             hipLaunchKernelGGL(dfi_transpose_kernel, grid, dim3(kThreads), 0,
-                               s, R2, p.N, ncol, R1);
+                               s, R2, h, ncol, R1);
             st = check_hip(hipGetLastError(), "dfi_transpose_kernel launch");
             if (st != ATH_OK) return st;
         }
@@ -755,12 +923,12 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
                   (size_t)sl * (size_t)out_slice_stride *
                       (out_dtype == ATH_F64 ? sizeof(double) : sizeof(float));
         st = out_dtype == ATH_F64
-                 ? rows_inv<true, double>(maxb, R1, ncol, ncol, pad, plan, p,
-                                          nullptr, reinterpret_cast<double*>(o),
-                                          out_row_stride, mask_r2, s)
-                 : rows_inv<true, float>(maxb, R1, ncol, ncol, pad, plan, p,
-                                         nullptr, reinterpret_cast<float*>(o),
-                                         out_row_stride, mask_r2, s);
+                 ? rows_out<double>(maxb, R1, ncol, pad, plan, p,
+                                    reinterpret_cast<double*>(o),
+                                    out_row_stride, mask_r2, s)
+                 : rows_out<float>(maxb, R1, ncol, pad, plan, p,
+                                   reinterpret_cast<float*>(o),
+                                   out_row_stride, mask_r2, s);
         if (st != ATH_OK) return st;
     }
     return ATH_OK;

@@ -35,10 +35,11 @@ __device__ __forceinline__ double2 rot(double2 a) {
 
 // Quarter twiddle table for a length-M transform: tq[m] = exp(-2 pi i m/M)
 // for m < M/4, taken from the plan's length-2M table (tw[2m]).  All threads.
+template <int NT = kFftThreads>
 __device__ __forceinline__ void stage_quarter(double2* tq,
                                               const double2* __restrict__ tw,
                                               int M) {
-    for (int m = threadIdx.x; m < (M >> 2); m += kFftThreads) tq[m] = tw[2 * m];
+    for (int m = threadIdx.x; m < (M >> 2); m += NT) tq[m] = tw[2 * m];
 }
 
 // exp(-2 pi i j/M) for j < M/2 from the quarter table: the second quarter
@@ -55,11 +56,11 @@ __device__ __forceinline__ double2 twiddle_half(const double2* tq, int j,
 // autosort: each pass reads all operands into registers, barriers, then
 // writes them to their sorted place in the same buffer).  DIR = -1 is the
 // forward transform, +1 the unscaled inverse.  tw[n] = exp(-2 pi i n / L),
-// L = 2M.  Each of the 256 threads handles up to MAXB butterflies a pass.
+// L = 2M.  Each of the NT threads handles up to MAXB butterflies a pass.
 // A radix-4 butterfly fetches one twiddle w and forms w^2, w^3 by complex
 // products (one table access instead of three).  With QT the twiddles come
 // from the LDS quarter table tq (stage_quarter) instead of global memory.
-template <int DIR, int MAXB, bool QT = false>
+template <int DIR, int MAXB, bool QT = false, int NT = kFftThreads>
 __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                         const double2* tq = nullptr) {
     const int tid = threadIdx.x;
@@ -71,7 +72,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
         double2 v[MAXB][4];
 #pragma unroll
         for (int b = 0; b < MAXB; ++b) {
-            const int j = tid + b * kFftThreads;
+            const int j = tid + b * NT;
             if (j < nb) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * nb];
@@ -98,7 +99,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < MAXB; ++b) {
-            const int j = tid + b * kFftThreads;
+            const int j = tid + b * NT;
             if (j < nb) {
                 const int k = j & (ns - 1);
                 const int base = (j - k) * 4 + k;
@@ -114,7 +115,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
         double2 v[2 * MAXB][2];
 #pragma unroll
         for (int b = 0; b < 2 * MAXB; ++b) {
-            const int j = tid + b * kFftThreads;
+            const int j = tid + b * NT;
             if (j < nb) {
                 v[b][0] = buf[j];
                 double2 x1 = buf[j + nb];
@@ -130,7 +131,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < 2 * MAXB; ++b) {
-            const int j = tid + b * kFftThreads;
+            const int j = tid + b * NT;
             if (j < nb) {
                 buf[j] = v[b][0];
                 buf[j + ns] = v[b][1];
@@ -175,12 +176,12 @@ inline int maxb_for(int M) {
 // Workgroups of `kernel` resident on the device at once with `lds` bytes of
 // dynamic LDS each (cached per kernel): the grid of a persistent launch.
 template <typename K>
-int resident_blocks(K kernel, size_t lds) {
+int resident_blocks(K kernel, size_t lds, int threads = kFftThreads) {
     static int cached = 0;  // one value per kernel instantiation
     if (cached > 0) return cached;
     int per_cu = 0, dev = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(kernel), kFftThreads,
+            &per_cu, reinterpret_cast<const void*>(kernel), threads,
             lds) != hipSuccess ||
         hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
