@@ -20,6 +20,16 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Row kernels by size class CLS = maxb_for(M) (the butterflies a 256-thread
+// workgroup would give each thread): the workgroup's thread count and the
+// butterflies per thread actually used.  Larger transforms get more threads
+// (more waves per SIMD to hide LDS latency) at two butterflies each:
+// M = 4096 as 512 threads (two workgroups of 80 KB per CU), M = 8192 as 1024
+// threads (one workgroup of 128 KB per CU).
+template <int CLS> struct RowCfg { static constexpr int maxb = CLS, nt = 256; };
+template <> struct RowCfg<4> { static constexpr int maxb = 2, nt = 512; };
+template <> struct RowCfg<8> { static constexpr int maxb = 2, nt = 1024; };
+
 template <typename Tin>
 __device__ __forceinline__ double load_sample(const Tin* row, int src,
                                               int apply_log);
@@ -50,10 +60,11 @@ __device__ __forceinline__ void filter_row(
     Tout* __restrict__ dst, int ncol, int pad, int pad_mode, int apply_log,
     const double2* __restrict__ tw, const double2* __restrict__ G, int M) {
     constexpr bool QT = MAXB <= 4;  // quarter table in LDS (M <= 4096)
+    constexpr int B = RowCfg<MAXB>::maxb, NT = RowCfg<MAXB>::nt;
     const int tid = threadIdx.x;
     const int width = ncol + 2 * pad;
     // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
-    for (int n = tid; n < M; n += kThreads) {
+    for (int n = tid; n < M; n += NT) {
         double v[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -68,10 +79,10 @@ __device__ __forceinline__ void filter_row(
         buf[n] = make_double2(v[0], v[1]);
     }
     __syncthreads();
-    lds_fft<-1, MAXB, QT>(buf, M, tw, tq);
+    lds_fft<-1, B, QT, NT>(buf, M, tw, tq);
 
     // rfft split, multiply by the kernel spectrum, merge for the inverse
-    for (int k = tid; k <= M / 2; k += kThreads) {
+    for (int k = tid; k <= M / 2; k += NT) {
         const int km = (M - k) & (M - 1);
         const double2 wk = tw[k];
         double2 xk, xm;
@@ -94,18 +105,18 @@ __device__ __forceinline__ void filter_row(
         }
     }
     __syncthreads();
-    lds_fft<1, MAXB, QT>(buf, M, tw, tq);
+    lds_fft<1, B, QT, NT>(buf, M, tw, tq);
 
     const double* y = reinterpret_cast<const double*>(buf);
     if (!ADJ) {
         const int off = width + pad;
-        for (int c = tid; c < ncol; c += kThreads)
+        for (int c = tid; c < ncol; c += NT)
             store_elem(dst + c, y[off + c]);
     } else {
         // z[n] = y[W + n], n in [0, W); out[c] = sum of z[n] over the n whose
         // pad source is c (the interior n = pad + c, plus folded pad samples)
         const double* z = y + width;
-        for (int c = tid; c < ncol; c += kThreads) {
+        for (int c = tid; c < ncol; c += NT) {
             double v = z[pad + c];
             if (pad_mode == ATH_PAD_EDGE) {
                 if (c == 0)
@@ -134,14 +145,15 @@ constexpr size_t row_lds_bytes(int M) {
 // Persistent: each workgroup stages the twiddles once, then filters rows
 // blockIdx.x, blockIdx.x + gridDim.x, ... of the (nrow, nslice, ncol) stack.
 template <int MAXB, typename Tin, typename Tout>
-__global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
+__global__ __launch_bounds__(RowCfg<MAXB>::nt) void ramp_filter_kernel(
     const Tin* __restrict__ sino, Tout* __restrict__ out, size_t rows,
     int nslice, int ncol, int pad, int pad_mode, int apply_log,
     const double2* __restrict__ tw, const double2* __restrict__ spec,
     int nwin, int M) {
     extern __shared__ double2 buf[];
     double2* tq = buf + M;
-    if (MAXB <= 4) stage_quarter(tq, tw, M);  // read after the row's barrier
+    if (MAXB <= 4)  // read after the row's barrier
+        stage_quarter<RowCfg<MAXB>::nt>(tq, tw, M);
     for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const double2* G =
             spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
@@ -153,14 +165,14 @@ __global__ __launch_bounds__(kThreads) void ramp_filter_kernel(
 
 // Transpose of ramp_filter_kernel (float32 in and out), persistent.
 template <int MAXB>
-__global__ __launch_bounds__(kThreads) void ramp_filter_adjoint_kernel(
+__global__ __launch_bounds__(RowCfg<MAXB>::nt) void ramp_filter_adjoint_kernel(
     const float* __restrict__ in, float* __restrict__ out, size_t rows,
     int nslice, int ncol, int pad, int pad_mode,
     const double2* __restrict__ tw, const double2* __restrict__ spec,
     int nwin, int M) {
     extern __shared__ double2 buf[];
     double2* tq = buf + M;
-    if (MAXB <= 4) stage_quarter(tq, tw, M);
+    if (MAXB <= 4) stage_quarter<RowCfg<MAXB>::nt>(tq, tw, M);
     for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const double2* G =
             spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
@@ -173,13 +185,14 @@ __global__ __launch_bounds__(kThreads) void ramp_filter_adjoint_kernel(
 // Rows of many stacks sharing the FFT length L = 2M (hence the twiddles),
 // persistent; a row's stack is found by binary search over row_begin.
 template <int MAXB, typename Tin, typename Tout>
-__global__ __launch_bounds__(kThreads) void ramp_filter_batched_kernel(
+__global__ __launch_bounds__(RowCfg<MAXB>::nt) void ramp_filter_batched_kernel(
     const ath_filter_job* __restrict__ jobs, int njobs, long long total_rows,
     int pad_mode, int apply_log, int M) {
     extern __shared__ double2 buf[];
     double2* tq = buf + M;
     if (MAXB <= 4)
-        stage_quarter(tq, static_cast<const double2*>(jobs[0].plan), M);
+        stage_quarter<RowCfg<MAXB>::nt>(
+            tq, static_cast<const double2*>(jobs[0].plan), M);
     for (long long grow = blockIdx.x; grow < total_rows; grow += gridDim.x) {
         int lo = 0, hi = njobs - 1;  // last job with row_begin <= grow
         while (lo < hi) {
@@ -241,10 +254,11 @@ int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
     auto kern = ramp_filter_kernel<MAXB, Tin, Tout>;
     prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
     const size_t rows = (size_t)nrow * nslice;
-    const size_t grid =
-        std::min(rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(
-                                                         1024 * MAXB)));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds, stream,
+    constexpr int NT = RowCfg<MAXB>::nt;
+    const size_t grid = std::min(
+        rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(1024 * MAXB),
+                                      NT));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, stream,
                        static_cast<const Tin*>(sino), static_cast<Tout*>(out),
                        rows, nslice, ncol, pad, pad_mode, apply_log, tw, spec,
                        nwin, M);
@@ -284,11 +298,12 @@ int launch_filter_batched(const ath_filter_job* jobs, int njobs,
     const size_t lds = row_lds_bytes<MAXB>(M);
     auto kern = ramp_filter_batched_kernel<MAXB, Tin, Tout>;
     prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
+    constexpr int NT = RowCfg<MAXB>::nt;
     const long long grid = std::min<long long>(
-        total_rows, resident_blocks(kern, row_lds_bytes<MAXB>(1024 * MAXB)));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds,
-                       stream, jobs, njobs, total_rows, pad_mode, apply_log,
-                       M);
+        total_rows,
+        resident_blocks(kern, row_lds_bytes<MAXB>(1024 * MAXB), NT));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, stream,
+                       jobs, njobs, total_rows, pad_mode, apply_log, M);
     return check_hip(hipGetLastError(), "ramp_filter_batched_kernel launch");
 }
 
@@ -420,12 +435,13 @@ int launch_adjoint(const float* in, int nrow, int nslice, int ncol, int pad,
     auto kern = ramp_filter_adjoint_kernel<MAXB>;
     prepare_lds(kern, row_lds_bytes<MAXB>(1024 * MAXB));
     const size_t rows = (size_t)nrow * nslice;
-    const size_t grid =
-        std::min(rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(
-                                                         1024 * MAXB)));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), lds, stream,
-                       in, out, rows, nslice, ncol, pad, pad_mode, tw, spec,
-                       nwin, M);
+    constexpr int NT = RowCfg<MAXB>::nt;
+    const size_t grid = std::min(
+        rows, (size_t)resident_blocks(kern, row_lds_bytes<MAXB>(1024 * MAXB),
+                                      NT));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, stream, in,
+                       out, rows, nslice, ncol, pad, pad_mode, tw, spec, nwin,
+                       M);
     return check_hip(hipGetLastError(), "ramp_filter_adjoint_kernel launch");
 }
 }  // namespace
