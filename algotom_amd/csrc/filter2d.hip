@@ -33,7 +33,14 @@
 namespace ath {
 namespace {
 
-constexpr int kThreads = 256;
+// Kernels by size class CLS = maxb_for(length) (butterflies per thread of a
+// 256-thread workgroup): the thread count and butterflies actually used.
+// Lengths 4096 and 8192 run 512 and 1024 threads of two butterflies, for
+// more waves per SIMD behind the LDS latency.
+template <int CLS> struct F2Cfg { static constexpr int maxb = CLS, nt = 256; };
+template <> struct F2Cfg<4> { static constexpr int maxb = 2, nt = 512; };
+template <> struct F2Cfg<8> { static constexpr int maxb = 2, nt = 1024; };
+
 constexpr int kSrcImage = 0;   // rows of the padded image
 constexpr int kSrcKernel = 1;  // rows of the tiled kernel K
 
@@ -51,11 +58,12 @@ __host__ __device__ inline Plan2d plan_view(const void* plan, int L) {
 // Forward real FFT of one row: image rows with numpy.pad on both axes, or
 // rows of the tiled kernel.  T row r of image k: T + (k * nrows + r) * (M+1).
 template <int MAXB, int SRC>
-__global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
+__global__ __launch_bounds__(F2Cfg<MAXB>::nt) void rows_fwd_kernel(
     const float* __restrict__ images, long long row_stride,
     long long img_stride, const double* __restrict__ g2, int ncol, int pad,
     int pad_mode, int width, const double2* __restrict__ tw,
     double2* __restrict__ T, int M) {
+    constexpr int NT_ = F2Cfg<MAXB>::nt;
     extern __shared__ double2 buf[];
     const int tid = threadIdx.x;
     const int u = blockIdx.x;
@@ -70,7 +78,7 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
     } else {
         g_row = g2 + (size_t)(u % width) * width;
     }
-    for (int n = tid; n < M; n += kThreads) {
+    for (int n = tid; n < M; n += NT_) {
         double v[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -88,9 +96,9 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
         buf[n] = make_double2(v[0], v[1]);
     }
     __syncthreads();
-    lds_fft<-1, MAXB>(buf, M, tw);
+    lds_fft<-1, F2Cfg<MAXB>::maxb, false, NT_>(buf, M, tw);
     double2* out = T + ((size_t)img * nrows + u) * (M + 1);
-    for (int k = tid; k <= M / 2; k += kThreads) {
+    for (int k = tid; k <= M / 2; k += NT_) {
         const int km = (M - k) & (M - 1);
         double2 xk, xm;
         split_real(buf[k], buf[km], tw[k], xk, xm);
@@ -103,31 +111,32 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
 // as Khat[k][.].  Otherwise multiply by Khat[k][.], transform back and keep
 // rows r0 .. r0+nout-1 into U[y][k].
 template <int MAXB, bool SPECTRUM>
-__global__ __launch_bounds__(kThreads) void cols_kernel(
+__global__ __launch_bounds__(F2Cfg<MAXB>::nt) void cols_kernel(
     const double2* __restrict__ T, int nrows_in, const double2* __restrict__ tw2,
     double2* __restrict__ khat_out, const double2* __restrict__ khat,
     double2* __restrict__ U, int r0, int nout, int M, double scale) {
+    constexpr int NT_ = F2Cfg<MAXB>::nt;
     extern __shared__ double2 buf[];
     const int tid = threadIdx.x;
     const int k = blockIdx.x;
     const int img = blockIdx.y;
     const int L = 2 * M;
     const double2* col = T + (size_t)img * nrows_in * (M + 1) + k;
-    for (int i = tid; i < L; i += kThreads)
+    for (int i = tid; i < L; i += NT_)
         buf[i] = i < nrows_in ? col[(size_t)i * (M + 1)] : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft<-1, MAXB>(buf, L, tw2);
+    lds_fft<-1, F2Cfg<MAXB>::maxb, false, NT_>(buf, L, tw2);
     if (SPECTRUM) {
         double2* dst = khat_out + (size_t)k * L;
-        for (int i = tid; i < L; i += kThreads) dst[i] = cscale(buf[i], scale);
+        for (int i = tid; i < L; i += NT_) dst[i] = cscale(buf[i], scale);
         return;
     }
     const double2* kc = khat + (size_t)k * L;
-    for (int i = tid; i < L; i += kThreads) buf[i] = cmul(buf[i], kc[i]);
+    for (int i = tid; i < L; i += NT_) buf[i] = cmul(buf[i], kc[i]);
     __syncthreads();
-    lds_fft<1, MAXB>(buf, L, tw2);
+    lds_fft<1, F2Cfg<MAXB>::maxb, false, NT_>(buf, L, tw2);
     double2* dst = U + (size_t)img * nout * (M + 1) + k;
-    for (int y = tid; y < nout; y += kThreads)
+    for (int y = tid; y < nout; y += NT_)
         dst[(size_t)y * (M + 1)] = buf[r0 + y];
 }
 
@@ -143,15 +152,16 @@ struct Finish2d {
 
 // Row inverse: Hermitian merge of U[y][0..M], inverse FFT, crop, finish.
 template <int MAXB, typename Tout>
-__global__ __launch_bounds__(kThreads) void rows_inv_kernel(
+__global__ __launch_bounds__(F2Cfg<MAXB>::nt) void rows_inv_kernel(
     const double2* __restrict__ U, const double2* __restrict__ tw,
     Tout* __restrict__ out, Finish2d f, int M) {
+    constexpr int NT_ = F2Cfg<MAXB>::nt;
     extern __shared__ double2 buf[];
     const int tid = threadIdx.x;
     const int y = blockIdx.x;
     const int img = blockIdx.y;
     const double2* Y = U + ((size_t)img * gridDim.x + y) * (M + 1);
-    for (int k = tid; k <= M / 2; k += kThreads) {
+    for (int k = tid; k <= M / 2; k += NT_) {
         const int km = (M - k) & (M - 1);
         const double2 wk = tw[k];
         const double2 yk = Y[k];
@@ -170,10 +180,10 @@ __global__ __launch_bounds__(kThreads) void rows_inv_kernel(
         }
     }
     __syncthreads();
-    lds_fft<1, MAXB>(buf, M, tw);
+    lds_fft<1, F2Cfg<MAXB>::maxb, false, NT_>(buf, M, tw);
     const double* v = reinterpret_cast<const double*>(buf) + f.c0;
     Tout* row = out + img * f.out_img_stride + (long long)y * f.out_row_stride;
-    for (int x = tid; x < f.ncol; x += kThreads) {
+    for (int x = tid; x < f.ncol; x += NT_) {
         bool keep = true;
         if (f.use_mask) {
             const long long dx = x - f.mask_c, dy = y - f.mask_c;
@@ -216,7 +226,7 @@ int build_plan(const double* g2, int width, int L, void* plan, void* work,
     auto rk = rows_fwd_kernel<MAXB_R, kSrcKernel>;
     const size_t lds_r = (size_t)M * sizeof(double2);
     prepare_lds(rk, 1024 * (size_t)MAXB_R * sizeof(double2));
-    hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(kThreads), lds_r, s,
+    hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(F2Cfg<MAXB_R>::nt), lds_r, s,
                        nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE, width,
                        tw, T, M);
     st = check_hip(hipGetLastError(), "rows_fwd_kernel (kernel) launch");
@@ -224,7 +234,7 @@ int build_plan(const double* g2, int width, int L, void* plan, void* work,
     auto ck = cols_kernel<MAXB_C, true>;
     const size_t lds_c = (size_t)L * sizeof(double2);
     prepare_lds(ck, 1024 * (size_t)MAXB_C * sizeof(double2));
-    hipLaunchKernelGGL(ck, dim3(M + 1, 1), dim3(kThreads), lds_c, s, T,
+    hipLaunchKernelGGL(ck, dim3(M + 1, 1), dim3(F2Cfg<MAXB_C>::nt), lds_c, s, T,
                        krows, tw2, khat, nullptr, nullptr, 0, 0, M,
                        1.0 / ((double)M * (double)L));
     return check_hip(hipGetLastError(), "cols_kernel (spectrum) launch");
@@ -243,7 +253,8 @@ int run_filter(const float* images, int nimg, int ncol, long long row_stride,
     auto rk = rows_fwd_kernel<MAXB_R, kSrcImage>;
     const size_t lds_r = (size_t)M * sizeof(double2);
     prepare_lds(rk, 1024 * (size_t)MAXB_R * sizeof(double2));
-    hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(kThreads), lds_r, s, images,
+    hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(F2Cfg<MAXB_R>::nt), lds_r, s,
+                       images,
                        row_stride, img_stride, nullptr, ncol, pad, pad_mode,
                        width, p.tw, T, M);
     int st = check_hip(hipGetLastError(), "rows_fwd_kernel launch");
@@ -251,14 +262,16 @@ int run_filter(const float* images, int nimg, int ncol, long long row_stride,
     auto ck = cols_kernel<MAXB_C, false>;
     const size_t lds_c = (size_t)L * sizeof(double2);
     prepare_lds(ck, 1024 * (size_t)MAXB_C * sizeof(double2));
-    hipLaunchKernelGGL(ck, dim3(M + 1, nimg), dim3(kThreads), lds_c, s, T,
+    hipLaunchKernelGGL(ck, dim3(M + 1, nimg), dim3(F2Cfg<MAXB_C>::nt), lds_c, s,
+                       T,
                        width, p.tw2, nullptr, p.khat, U, width + pad, ncol, M,
                        1.0);
     st = check_hip(hipGetLastError(), "cols_kernel launch");
     if (st != ATH_OK) return st;
     auto ik = rows_inv_kernel<MAXB_R, Tout>;
     prepare_lds(ik, 1024 * (size_t)MAXB_R * sizeof(double2));
-    hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(kThreads), lds_r, s, U,
+    hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(F2Cfg<MAXB_R>::nt), lds_r, s,
+                       U,
                        p.tw, out, f, M);
     return check_hip(hipGetLastError(), "rows_inv_kernel launch");
 }
