@@ -172,13 +172,34 @@ __global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a,
 
 // (img - dark) / flat_dark in float32 (the caller has replaced zeros of
 // flat_dark by 1), then -log with values <= 0 set to 1 and flagged.
-template <typename Tin>
+__device__ __forceinline__ float flat_log_value(float v, const float* dk,
+                                                const float* fd, int j,
+                                                int apply_log, int* flag) {
+    if (dk) v = (v - dk[j]) / fd[j];
+    if (apply_log) {
+        if (v <= 0.0f) {
+            v = 1.0f;
+            *flag = 1;
+        }
+        v = (float)(-log((double)v));
+    }
+    return v;
+}
+
+template <typename Tin, int VEC> struct VecOf;
+template <> struct VecOf<float, 4> { using type = float4; };
+template <> struct VecOf<unsigned short, 4> { using type = ushort4; };
+
+// VEC = 4: groups of four columns with 16-byte (8 for uint16) loads and
+// stores; the caller checks widths, strides and alignment.
+template <typename Tin, int VEC>
 __global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
     const Tin* __restrict__ proj, long long nproj, int height, int width,
     long long proj_stride, long long row_stride,
     const float* __restrict__ dark, const float* __restrict__ flat_dark,
     int apply_log, float* __restrict__ out, int* __restrict__ flag) {
     const long long rows = nproj * height;
+    const int groups = width / VEC;
     for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
         const long long n = r / height;  // block-uniform
         const int iy = (int)(r - n * height);
@@ -186,18 +207,25 @@ __global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
         float* dst = out + r * width;
         const float* dk = dark ? dark + (size_t)iy * width : nullptr;
         const float* fd = dark ? flat_dark + (size_t)iy * width : nullptr;
-        for (int jx = blockIdx.x * kThreads + threadIdx.x; jx < width;
-             jx += gridDim.x * kThreads) {
-            float v = (float)src[jx];
-            if (dk) v = (v - dk[jx]) / fd[jx];
-            if (apply_log) {
-                if (v <= 0.0f) {
-                    v = 1.0f;
-                    *flag = 1;
-                }
-                v = (float)(-log((double)v));
+        for (int g = blockIdx.x * kThreads + threadIdx.x; g < groups;
+             g += gridDim.x * kThreads) {
+            if constexpr (VEC == 1) {
+                dst[g] = flat_log_value((float)src[g], dk, fd, g, apply_log,
+                                        flag);
+            } else {
+                using V = typename VecOf<Tin, 4>::type;
+                const V x = reinterpret_cast<const V*>(src)[g];
+                const int j = 4 * g;
+                float4 y;
+                y.x = flat_log_value((float)x.x, dk, fd, j, apply_log, flag);
+                y.y = flat_log_value((float)x.y, dk, fd, j + 1, apply_log,
+                                     flag);
+                y.z = flat_log_value((float)x.z, dk, fd, j + 2, apply_log,
+                                     flag);
+                y.w = flat_log_value((float)x.w, dk, fd, j + 3, apply_log,
+                                     flag);
+                reinterpret_cast<float4*>(dst)[g] = y;
             }
-            dst[jx] = v;
         }
     }
 }
@@ -254,22 +282,52 @@ int vertical_back_projection(const float* proj, int nproj, int height,
     return check_hip(hipGetLastError(), "vertical_bp_kernel launch");
 }
 
+template <typename Tin, int VEC>
+static void launch_flat_log(const void* proj, int nproj, int height,
+                            int width, long long proj_stride,
+                            long long row_stride, const float* dark,
+                            const float* flat_dark, int apply_log, float* out,
+                            int* flag, hipStream_t s) {
+    const dim3 grid =
+        row_grid((long long)nproj * height, width / VEC, kThreads);
+    hipLaunchKernelGGL((flat_field_log_kernel<Tin, VEC>), grid,
+                       dim3(kThreads), 0, s, static_cast<const Tin*>(proj),
+                       (long long)nproj, height, width, proj_stride,
+                       row_stride, dark, flat_dark, apply_log, out, flag);
+}
+
 int flat_field_log(const void* proj, int in_dtype, int nproj, int height,
                    int width, long long proj_stride, long long row_stride,
                    const float* dark, const float* flat_dark, int apply_log,
                    float* out, int* flag, hipStream_t s) {
-    const dim3 grid = row_grid((long long)nproj * height, width, kThreads);
-    if (in_dtype == ATH_U16)
-        hipLaunchKernelGGL(flat_field_log_kernel<unsigned short>, grid,
-                           dim3(kThreads), 0, s,
-                           static_cast<const unsigned short*>(proj),
-                           (long long)nproj, height, width, proj_stride,
-                           row_stride, dark, flat_dark, apply_log, out, flag);
-    else
-        hipLaunchKernelGGL(flat_field_log_kernel<float>, grid, dim3(kThreads),
-                           0, s, static_cast<const float*>(proj),
-                           (long long)nproj, height, width, proj_stride,
-                           row_stride, dark, flat_dark, apply_log, out, flag);
+    const bool u16 = in_dtype == ATH_U16;
+    const size_t in_align = u16 ? 8 : 16;
+    const bool vec =
+        width % 4 == 0 && row_stride % 4 == 0 && proj_stride % 4 == 0 &&
+        reinterpret_cast<uintptr_t>(proj) % in_align == 0 &&
+        reinterpret_cast<uintptr_t>(out) % 16 == 0 &&
+        (!dark || (reinterpret_cast<uintptr_t>(dark) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(flat_dark) % 16 == 0));
+    if (u16) {
+        if (vec)
+            launch_flat_log<unsigned short, 4>(proj, nproj, height, width,
+                                               proj_stride, row_stride, dark,
+                                               flat_dark, apply_log, out, flag,
+                                               s);
+        else
+            launch_flat_log<unsigned short, 1>(proj, nproj, height, width,
+                                               proj_stride, row_stride, dark,
+                                               flat_dark, apply_log, out, flag,
+                                               s);
+    } else if (vec) {
+        launch_flat_log<float, 4>(proj, nproj, height, width, proj_stride,
+                                  row_stride, dark, flat_dark, apply_log, out,
+                                  flag, s);
+    } else {
+        launch_flat_log<float, 1>(proj, nproj, height, width, proj_stride,
+                                  row_stride, dark, flat_dark, apply_log, out,
+                                  flag, s);
+    }
     return check_hip(hipGetLastError(), "flat_field_log_kernel launch");
 }
 
