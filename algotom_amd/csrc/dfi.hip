@@ -486,14 +486,11 @@ __global__ __launch_bounds__(kThreads) void dfi_cols_kernel(
 }
 
 // ---- 5. gridding --------------------------------------------------------------
-// G[y][x] = mask ? sum beta5 beta5 coef : 0 at the float32 (theta, r) of
-// generate_mapping_coordinate, X = m - x, Y = y - m (float64 sqrt / atan2 on
-// the integer grid, rounded as the reference rounds them).
-__device__ __forceinline__ double2 grid_value(const double2* __restrict__ coef,
-                                              int rows, int N, int m, int X,
-                                              int Y) {
-    double2 val = make_double2(0.0, 0.0);
-    if (X * X + Y * Y > m * m) return val;
+// The float32 (theta, r) of generate_mapping_coordinate for X = m - x,
+// Y = y - m (float64 sqrt / atan2 on the integer grid, rounded as the
+// reference rounds them), as the (row, column) position in coef.
+__device__ __forceinline__ void grid_coords(int rows, int N, int m, int X,
+                                            int Y, double& tt, double& rr) {
     const double r_max = (double)m;
     float r = (float)fmin(fmax(sqrt((double)(X * X + Y * Y)), 0.0), r_max);
     double theta = kPi + atan2((double)Y, (double)X);
@@ -501,9 +498,15 @@ __device__ __forceinline__ double2 grid_value(const double2* __restrict__ coef,
         r = -r;
         theta -= kPi;
     }
-    const double rr =
-        (double)(float)fmin(fmax((double)r + r_max, 0.0), (double)(N - 1));
-    const double tt = (double)(float)(theta * ((double)rows - 1.0) / kPi);
+    rr = (double)(float)fmin(fmax((double)r + r_max, 0.0), (double)(N - 1));
+    tt = (double)(float)(theta * ((double)rows - 1.0) / kPi);
+}
+
+// sum_ij beta5(tt - k_i) beta5(rr - l_j) coef[k_i][l_j] over the 6 x 6
+// nearest knots, folded half-sample.
+__device__ __forceinline__ double2 grid_eval(const double2* __restrict__ coef,
+                                             int rows, int N, double tt,
+                                             double rr) {
     const int s0 = (int)floor(tt) - 2, s1 = (int)floor(rr) - 2;
     double wx[6];
     int cx[6];
@@ -512,7 +515,8 @@ __device__ __forceinline__ double2 grid_value(const double2* __restrict__ coef,
         wx[j] = beta5(rr - (double)(s1 + j));
         cx[j] = fold_half(s1 + j, N);
     }
-#pragma unroll
+    double2 val = make_double2(0.0, 0.0);
+#pragma unroll 1
     for (int i = 0; i < 6; ++i) {
         const double wy = beta5(tt - (double)(s0 + i));
         const double2* crow = coef + (size_t)fold_half(s0 + i, rows) * N;
@@ -531,10 +535,17 @@ __device__ __forceinline__ double2 grid_value(const double2* __restrict__ coef,
 
 // T[v][u] = H[u][v] for the columns v <= m, H the Hermitian part
 // (Gs[u][v] + conj Gs[-u][-v]) / 2 of Gs = ifftshift(G) (Gs[u][v] =
-// G[(u + m) % N][(v + m) % N]), transposed.  The reference keeps
-// Re(ifft2(Gs)) = ifft2(H); the columns v > m of H are the conjugates of
-// these and are never formed.  (G itself is Hermitian only to the float32
-// rounding of its coordinates, r' = m -+ r, hence the explicit average.)
+// G[(u + m) % N][(v + m) % N]), transposed; G[y][x] = mask ? sum beta5
+// beta5 coef : 0 at (theta, r) of (X, Y) = (m - x, y - m).  The reference
+// keeps Re(ifft2(Gs)) = ifft2(H); the columns v > m of H are the conjugates
+// of these and are never formed.  (G itself is Hermitian only to the
+// float32 rounding of its coordinates, r' = m -+ r, hence the explicit
+// average.)  The mirrored frequency (-X, -Y) has the same theta and r' near
+// 2m - r; every coef row is conjugate-symmetric about column m (spectra of
+// real rows, symmetric window, symmetric prefilter and folds), so when its
+// rounded theta is the same and its knot window the mirror image, conj of
+// its value reuses the first point's 36 coefficients in reversed column
+// order; otherwise it is gathered on its own.
 __global__ __launch_bounds__(kThreads) void dfi_grid_kernel(
     const double2* __restrict__ coef, int rows, int N,
     double2* __restrict__ T) {
@@ -549,9 +560,54 @@ __global__ __launch_bounds__(kThreads) void dfi_grid_kernel(
             int y = u + m;
             if (y >= N) y -= N;
             const int Y = y - m;
-            const double2 a = grid_value(coef, rows, N, m, X, Y);
-            const double2 b = grid_value(coef, rows, N, m, -X, -Y);
-            dst[u] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+            double2 h = make_double2(0.0, 0.0);
+            if (X * X + Y * Y <= m * m) {
+                double tta, rra, ttb, rrb;
+                grid_coords(rows, N, m, X, Y, tta, rra);
+                grid_coords(rows, N, m, -X, -Y, ttb, rrb);
+                const int s1 = (int)floor(rra) - 2;
+                const int s1b = (int)floor(rrb) - 2;
+                if (tta == ttb && s1b == 2 * m - s1 - 5) {
+                    const int s0 = (int)floor(tta) - 2;
+                    double wa[6], wb[6];
+                    int cx[6];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        wa[j] = beta5(rra - (double)(s1 + j));
+                        wb[j] = beta5(rrb - (double)(s1b + 5 - j));
+                        cx[j] = fold_half(s1 + j, N);
+                    }
+                    double2 sa = make_double2(0.0, 0.0);
+                    double2 sb = make_double2(0.0, 0.0);
+#pragma unroll 1
+                    for (int i = 0; i < 6; ++i) {
+                        const double wy = beta5(tta - (double)(s0 + i));
+                        const double2* crow =
+                            coef + (size_t)fold_half(s0 + i, rows) * N;
+                        double2 ra = make_double2(0.0, 0.0);
+                        double2 rb = make_double2(0.0, 0.0);
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) {
+                            const double2 c = crow[cx[j]];
+                            ra.x += wa[j] * c.x;
+                            ra.y += wa[j] * c.y;
+                            rb.x += wb[j] * c.x;
+                            rb.y += wb[j] * c.y;
+                        }
+                        sa.x += wy * ra.x;
+                        sa.y += wy * ra.y;
+                        sb.x += wy * rb.x;
+                        sb.y += wy * rb.y;
+                    }
+                    // sb = conj of the mirrored frequency's value
+                    h = make_double2(0.5 * (sa.x + sb.x), 0.5 * (sa.y + sb.y));
+                } else {
+                    const double2 a = grid_eval(coef, rows, N, tta, rra);
+                    const double2 b = grid_eval(coef, rows, N, ttb, rrb);
+                    h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+                }
+            }
+            dst[u] = h;
         }
     }
 }
