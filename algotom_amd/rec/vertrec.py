@@ -21,6 +21,9 @@ run on the GPU; nothing falls back to the CPU.
 * Projections: numpy arrays (float32 or uint16; other types are converted
   to float32) or torch tensors on the GPU.  Flat and dark fields are used as
   float32.
+* Centre search: ``find_center_vertical_slice`` (metrics on the GPU) and
+  ``find_center_visual_vertical_slices`` (one TIFF per centre, written with
+  ``algotom_amd.io.loadersaver``).
 """
 
 import sys
@@ -597,3 +600,74 @@ def find_center_vertical_slice(projections, slice_index, start, stop,
     if return_metric is True:
         return list_center, list_metric
     return center
+
+
+def find_center_visual_vertical_slices(projections, output_folder,
+                                       slice_index, start, stop, step=1.0,
+                                       alpha=0.0, angles=None, chunk_size=30,
+                                       ramp_filter="after", apply_log=True,
+                                       gpu=True, block=(16, 16), ncore=None,
+                                       prefer="processes", display=True,
+                                       masking=True):
+    """For finding the centre of rotation visually: a vertical slice
+    reconstructed at every centre of arange(start, stop + step, step),
+    each saved as ``center_<c:.2f>.tif`` in a new ``Find_center_NNN``
+    folder under ``output_folder`` (vertrec.py:1431-1536).  Returns that
+    folder's path (with a trailing "/").
+
+    The projections are logged once (values <= 0 are set to 1 in place,
+    with the reference's warning); all centres of a pass are then
+    reconstructed as separate lines of one vertical back-projection launch.
+    The mask keeps the columns of radius int(min(c_min, width - c_min))
+    about the middle, as the reference does.
+    """
+    import torch
+    from algotom_amd.io import loadersaver as losa
+    output_name = losa.make_folder_name(output_folder,
+                                        name_prefix="Find_center",
+                                        zero_prefix=3)
+    output_base = output_folder + "/" + output_name + "/"
+    list_center = np.arange(start, stop + step, step)
+    (num_proj, height, width) = projections.shape
+    if angles is None:
+        angles = np.deg2rad(np.linspace(0.0, 180.0, num_proj))
+    if np.any(list_center < 0) or np.any(list_center > width - 1):
+        raise ValueError("Center (relative to the cropped image) is out of "
+                         "range {})".format((0, width - 1)))
+    dev = nat.device()
+    p = rec._to_device(projections, dev, allow_f64=False)
+    if apply_log:
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        p = _prepare(p, None, True, flag, dev)
+        if int(flag.item()) != 0:
+            # vertrec.py:1493-1502: warn and set the values <= 0 of the
+            # caller's array to 1 before the log
+            warnings.warn(_LOG_WARNING)
+            projections[projections <= 0.0] = 1.0
+    if ramp_filter is None:
+        ramp_filter = "after"
+    lohi = None
+    if masking:
+        cmin = np.min(list_center)
+        rad = int(min(cmin, width - cmin))
+        lo = (width - 2 * rad) // 2
+        lohi = (lo, lo + 2 * rad)
+    pad = min(int(0.15 * width), 150)
+    xs, ys = _get_points_single_line(slice_index, alpha, width)
+    esize = 8 if ramp_filter == "after" else 4
+    per_batch = max(1, _SEARCH_BYTES // (esize * height * width))
+    t0 = time.time()
+    for b0 in range(0, len(list_center), per_batch):
+        cs = list_center[b0:b0 + per_batch]
+        imgs = _vertical_pipeline(
+            p, xs, ys, True, float(cs[0]), None, None, angles, (0, 0, 0, 0),
+            0, -1, chunk_size, ramp_filter, "hann", pad, "edge", False, gpu,
+            False, False, centers=cs, mask_lohi=lohi, keep_device=True)
+        imgs = imgs.cpu().numpy()
+        for k, center in enumerate(cs):
+            file_name = "center_{0:.2f}".format(center) + ".tif"
+            losa.save_image(output_base + file_name, imgs[k])
+            if display:
+                print("Done: {0}. Time elapsed {1:.2f}".format(
+                    output_base + file_name, time.time() - t0))
+    return output_base
