@@ -737,6 +737,264 @@ __global__ __launch_bounds__(kThreads) void dfi_transpose_kernel(
         }
 }
 
+// ---- 7. Bluestein length 16384 (padded widths 4097 .. 8191) ---------------
+// An 8192-point float64 FFT is the largest one workgroup holds in LDS, so the
+// length-M = 2H convolution runs as H-point transforms: the even and odd
+// samples forward (decimation in time, A[k] = E[k] + W^k O[k] and A[k + H] =
+// E[k] - W^k O[k], W = exp(-2 pi i / M)), the product with the kernel
+// spectrum, then the inverse transforms of U = Y[k] + Y[k + H] (even
+// outputs) and V = (Y[k] - Y[k + H]) W^-k (odd outputs), decimation in
+// frequency.  E, U and the odd outputs pass through a per-workgroup global
+// scratch of 2H values.
+constexpr int kBigH = 8192;
+constexpr int kBigNT = 512;
+constexpr int kBigB = 4;  // butterflies per thread: kBigH / 4 / kBigNT
+
+// On entry LOAD(n) gives chirp[n] a[n] for n < N.  On exit conv[n] is
+// buf[n / 2] for even n and scratch[H + n / 2] for odd n (big_conv).
+// tw: the H-point FFT's table, tw[k] = exp(-2 pi i k / 2H) = W^k, k < 2H;
+// bhat: FFT_M of the chirp kernel over M.
+template <typename Load>
+__device__ void bluestein_big(double2* buf, double2* scratch, int N,
+                              const double2* __restrict__ tw,
+                              const double2* __restrict__ bhat, Load load) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
+    __syncthreads();
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) {
+        const double2 w = tw[k];
+        const double2 e = scratch[k];
+        const double2 o = cmul(buf[k], w);
+        const double2 ylo = cmul(cadd(e, o), bhat[k]);
+        const double2 yhi = cmul(csub(e, o), bhat[k + H]);
+        scratch[k] = cadd(ylo, yhi);
+        buf[k] = cmul(csub(ylo, yhi), conjg(w));
+    }
+    __syncthreads();
+    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+    for (int q = tid; q < H; q += NT) scratch[H + q] = buf[q];
+    __syncthreads();
+    for (int k = tid; k < H; k += NT) buf[k] = scratch[k];
+    __syncthreads();
+    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+}
+
+__device__ __forceinline__ double2 big_conv(const double2* buf,
+                                            const double2* scratch, int n) {
+    return (n & 1) ? scratch[kBigH + (n >> 1)] : buf[n >> 1];
+}
+
+// bhat = FFT_M(b) / M for M = 2H (one workgroup; b as in bhat_kernel)
+__global__ __launch_bounds__(kBigNT) void bhat_big_kernel(
+    const double2* __restrict__ chirp, const double2* __restrict__ tw, int N,
+    double2* __restrict__ bhat) {
+    extern __shared__ double2 lds[];
+    constexpr int H = kBigH, NT = kBigNT, M = 2 * kBigH;
+    const int tid = threadIdx.x;
+    auto b = [&](int i) -> double2 {
+        if (i < N) return conjg(chirp[i]);
+        if (i > M - N) return conjg(chirp[M - i]);
+        return make_double2(0.0, 0.0);
+    };
+    for (int j = tid; j < H; j += NT) lds[j] = b(2 * j);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(lds, H, tw);
+    for (int k = tid; k < H; k += NT) bhat[k] = lds[k];
+    __syncthreads();
+    for (int j = tid; j < H; j += NT) lds[j] = b(2 * j + 1);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(lds, H, tw);
+    const double inv = 1.0 / (double)M;
+    for (int k = tid; k < H; k += NT) {
+        const double2 e = bhat[k];
+        const double2 o = cmul(lds[k], tw[k]);
+        bhat[k] = cscale(cadd(e, o), inv);
+        bhat[k + H] = cscale(csub(e, o), inv);
+    }
+}
+
+// Row pairs: pad, -log, the two rows' spectra split and windowed into rows
+// r1 and r1 + 1 of Xn in natural order (the row prefilter follows in
+// dfi_rows_fir_kernel).
+template <typename Tr>
+__global__ __launch_bounds__(kBigNT) void dfi_rows_fwd_big_kernel(
+    const double* __restrict__ S, int rows, int ncol1, int pad, int pad_mode,
+    int apply_log, const void* __restrict__ plan, int N, size_t o_tw,
+    size_t o_chirp, size_t o_bhat, size_t o_win, double2* __restrict__ Xn,
+    double2* __restrict__ scratch_all, int* __restrict__ flag) {
+    extern __shared__ double2 buf[];
+    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const double* win = at<double>(plan, o_win);
+    const int tid = threadIdx.x;
+    const int m = (N - 1) / 2;
+    const int npairs = (rows + 1) / 2;
+    for (int pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+        const int r1 = 2 * pr;
+        const bool two = r1 + 1 < rows;
+        const double* s1 = S + (size_t)r1 * ncol1;
+        const double* s2 = s1 + ncol1;
+        auto load = [&](int n) -> double2 {
+            int u = n + m;  // ifftshift
+            if (u >= N) u -= N;
+            const int c = pad_source(u - pad, ncol1, pad_mode);
+            double v1 = c >= 0 ? s1[c] : 0.0;
+            double v2 = (two && c >= 0) ? s2[c] : 0.0;
+            if (apply_log) {
+                v1 = minus_log<Tr>(v1, flag);
+                if (two) v2 = minus_log<Tr>(v2, flag);
+            }
+            const double2 ch = chirp[n];
+            return make_double2(ch.x * v1 - ch.y * v2, ch.x * v2 + ch.y * v1);
+        };
+        bluestein_big(buf, scratch, N, tw, bhat, load);
+        double2* d1 = Xn + (size_t)r1 * N;
+        double2* d2 = d1 + N;
+        for (int k = tid; k <= m; k += kBigNT) {
+            const int kn = k == 0 ? 0 : N - k;
+            const double2 zk = cmul(chirp[k], big_conv(buf, scratch, k));
+            const double2 zn = cmul(chirp[kn], big_conv(buf, scratch, kn));
+            const double2 x1 =
+                make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+            const double2 x2 =
+                make_double2(0.5 * (zk.y + zn.y), 0.5 * (zn.x - zk.x));
+            const int pk = k > m ? k - m - 1 : k + m;  // (k - m - 1) mod N
+            const int pn = kn > m ? kn - m - 1 : kn + m;
+            d1[k] = cscale(x1, win[pk]);
+            if (two) d2[k] = cscale(x2, win[pk]);
+            if (k != 0) {
+                d1[kn] = cscale(conjg(x1), win[pn]);
+                if (two) d2[kn] = cscale(conjg(x2), win[pn]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The quintic prefilter along rows of Xn (natural order; spec[kp] =
+// Xn[(kp + m + 1) mod N]) into X (fftshifted order), kFirRun outputs per
+// thread.
+__global__ __launch_bounds__(kThreads) void dfi_rows_fir_kernel(
+    const double2* __restrict__ Xn, int rows, int N,
+    const double* __restrict__ h5, double2* __restrict__ X) {
+    const int m = (N - 1) / 2;
+    const int ngroups = (N + kFirRun - 1) / kFirRun;
+    for (int row = blockIdx.y; row < rows; row += gridDim.y) {
+        const double2* src = Xn + (size_t)row * N;
+        double2* dst = X + (size_t)row * N;
+        for (int g = blockIdx.x * kThreads + threadIdx.x; g < ngroups;
+             g += gridDim.x * kThreads) {
+            const int k0 = g * kFirRun;
+            double2 a[kFirRun];
+#pragma unroll
+            for (int r = 0; r < kFirRun; ++r) a[r] = make_double2(0.0, 0.0);
+            for (int i = 0; i < kFirRun + 2 * kK5; ++i) {
+                int q = fold_half(k0 - kK5 + i, N) + m + 1;
+                if (q >= N) q -= N;
+                const double2 v = src[q];
+#pragma unroll
+                for (int r = 0; r < kFirRun; ++r) {
+                    int d = i - kK5 - r;
+                    d = d < 0 ? -d : d;
+                    if (d <= kK5) {
+                        const double h = h5[d];
+                        a[r].x += h * v.x;
+                        a[r].y += h * v.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kFirRun; ++r)
+                if (k0 + r < N) dst[k0 + r] = a[r];
+        }
+    }
+}
+
+// Stage A at M = 2H (as dfi_cols_inv_kernel).
+__global__ __launch_bounds__(kBigNT) void dfi_cols_inv_big_kernel(
+    const double2* __restrict__ in, int nrows, int ncol, int pad,
+    const void* __restrict__ plan, int N, size_t o_tw, size_t o_chirp,
+    size_t o_bhat, double2* __restrict__ out,
+    double2* __restrict__ scratch_all) {
+    extern __shared__ double2 buf[];
+    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const int m = (N - 1) / 2;
+    const double invN = 1.0 / (double)N;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const double2* src = in + (size_t)row * N;
+        auto load = [&](int n) -> double2 {
+            return cmul(chirp[n], conjg(src[n]));
+        };
+        bluestein_big(buf, scratch, N, tw, bhat, load);
+        for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
+            const int k = crop_index(kc, pad, m, N);
+            out[(size_t)row * ncol + kc] = cscale(
+                conjg(cmul(chirp[k], big_conv(buf, scratch, k))), invN);
+        }
+        __syncthreads();
+    }
+}
+
+// Stage B at M = 2H (as dfi_rows_out_kernel).
+template <typename To>
+__global__ __launch_bounds__(kBigNT) void dfi_rows_out_big_kernel(
+    const double2* __restrict__ in, int ncol, int pad,
+    const void* __restrict__ plan, int N, size_t o_tw, size_t o_chirp,
+    size_t o_bhat, To* __restrict__ out, long long out_row_stride,
+    double mask_r2, double2* __restrict__ scratch_all) {
+    extern __shared__ double2 buf[];
+    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    const double2* tw = at<double2>(plan, o_tw);
+    const double2* chirp = at<double2>(plan, o_chirp);
+    const double2* bhat = at<double2>(plan, o_bhat);
+    const int m = (N - 1) / 2;
+    const int h = m + 1;
+    const double invN = 1.0 / (double)N;
+    const int mc = ncol / 2;
+    const int npairs = (ncol + 1) / 2;
+    for (int pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+        const int y1 = 2 * pr;
+        const bool two = y1 + 1 < ncol;
+        const double2* a1 = in + (size_t)y1 * h;
+        const double2* a2 = a1 + h;
+        auto load = [&](int n) -> double2 {
+            const double2 p = n <= m ? a1[n] : conjg(a1[N - n]);
+            double2 q = make_double2(0.0, 0.0);
+            if (two) q = n <= m ? a2[n] : conjg(a2[N - n]);
+            return cmul(chirp[n], make_double2(p.x - q.y, -(p.y + q.x)));
+        };
+        bluestein_big(buf, scratch, N, tw, bhat, load);
+        for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
+            const int k = crop_index(kc, pad, m, N);
+            const double2 y =
+                conjg(cmul(chirp[k], big_conv(buf, scratch, k)));
+            double v1 = y.x * invN, v2 = y.y * invN;
+            if (mask_r2 >= 0.0) {
+                const int dx = kc - mc, d1 = y1 - mc, d2 = y1 + 1 - mc;
+                if ((double)(d1 * d1 + dx * dx) > mask_r2) v1 = v1 * 0.0;
+                if ((double)(d2 * d2 + dx * dx) > mask_r2) v2 = v2 * 0.0;
+            }
+            out[(long long)y1 * out_row_stride + kc] = (To)v1;
+            if (two) out[(long long)(y1 + 1) * out_row_stride + kc] = (To)v2;
+        }
+        __syncthreads();
+    }
+}
+
 // ---- dispatch ---------------------------------------------------------------------
 // Transform lengths up to 4096 run with 256 threads (MAXB butterflies each,
 // the quarter twiddle table in LDS); 8192 runs with 1024 threads of 2
@@ -827,8 +1085,22 @@ int rows_out(int maxb, const double2* in, int ncol, int pad, const void* plan,
 // ((m + 1) x N), then the transposed first pass (ncol x (m + 1)); R2 holds
 // the column coefficients (rows x N), then the first pass ((m + 1) x ncol).
 struct DfiWork {
-    size_t s1, s2, r1, r2, total;
+    size_t s1, s2, r1, r2, scr, total;
+    int nscr;  // scratch slots (workgroups) for the length-16384 path
 };
+
+// One 2H-value scratch per compute unit (the big kernels' 128 KB of LDS
+// allow one workgroup per CU).
+inline int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return cus;
+}
 
 DfiWork work_layout(int nrow_in, int ncol, int nrow, int N) {
     const int ncol1 = ncol + (ncol % 2 == 0 ? 1 : 0);
@@ -841,6 +1113,8 @@ DfiWork work_layout(int nrow_in, int ncol, int nrow, int N) {
     const size_t r2 = std::max((size_t)nrow * N, h * ncol);
     w.r1 = o; o = align256(o + r1 * sizeof(double2));
     w.r2 = o; o = align256(o + r2 * sizeof(double2));
+    w.nscr = bluestein_length(N) > kBigH ? device_cus() : 0;
+    w.scr = o; o = align256(o + (size_t)w.nscr * 2 * kBigH * sizeof(double2));
     w.total = o;
     return w;
 }
@@ -861,7 +1135,7 @@ int shift_rows(const Tin* in, long long row_stride, int nrow, int ncol,
 }  // namespace
 
 size_t dfi_plan_bytes(int width) {
-    if (width < 3 || width % 2 == 0 || bluestein_length(width) > 8192)
+    if (width < 3 || width % 2 == 0 || bluestein_length(width) > 2 * kBigH)
         return 0;
     return plan_layout(width).total;
 }
@@ -869,7 +1143,10 @@ size_t dfi_plan_bytes(int width) {
 int dfi_plan_build(const double* window, int width, void* plan,
                    hipStream_t s) {
     const DfiPlan p = plan_layout(width);
-    const int L = 2 * p.M;
+    const bool big = p.M > kBigH;
+    // the FFT's table: length-M transforms take exp(-2 pi i n / 2M); the
+    // length-2H path runs H-point transforms, exp(-2 pi i n / 2H)
+    const int L = big ? p.M : 2 * p.M;
     hipLaunchKernelGGL(twiddle_kernel, dim3((L + 255) / 256), dim3(256), 0, s,
                        at<double2>(plan, p.tw), L);
     hipLaunchKernelGGL(chirp_kernel, dim3((p.N + 255) / 256), dim3(256), 0, s,
@@ -878,6 +1155,15 @@ int dfi_plan_build(const double* window, int width, void* plan,
     hipLaunchKernelGGL(dfi_tables_kernel, dim3((tab + 255) / 256), dim3(256),
                        0, s, window, p.N, at<double>(plan, p.window),
                        at<double>(plan, p.h3), at<double>(plan, p.h5));
+    if (big) {
+        const size_t lds_big = (size_t)kBigH * sizeof(double2);
+        prepare_lds(bhat_big_kernel, lds_big);
+        hipLaunchKernelGGL(bhat_big_kernel, dim3(1), dim3(kBigNT), lds_big, s,
+                           at<double2>(plan, p.chirp),
+                           at<double2>(plan, p.tw), p.N,
+                           at<double2>(plan, p.bhat));
+        return check_hip(hipGetLastError(), "dfi plan kernels");
+    }
     const size_t lds = (size_t)p.M * sizeof(double2);
 #define ATH_BHAT(B)                                                           \
     {                                                                         \
@@ -918,6 +1204,9 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
     double* S2 = at<double>(workspace, w.s2);
     double2* R1 = at<double2>(workspace, w.r1);
     double2* R2 = at<double2>(workspace, w.r2);
+    double2* scr = at<double2>(workspace, w.scr);
+    const bool big = p.M > kBigH;
+    const size_t lds_big = (size_t)kBigH * sizeof(double2);
     const double* h3 = at<double>(plan, p.h3);
     const double* h5 = at<double>(plan, p.h5);
     const size_t esz = f64 ? sizeof(double) : sizeof(float);
@@ -947,10 +1236,37 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
             if (st != ATH_OK) return st;
             S = S2;
         }
-        st = f64 ? rows_fwd<double>(maxb, S, nrow, ncol1, pad, pad_mode,
-                                    apply_log, plan, p, R1, flag, s)
-                 : rows_fwd<float>(maxb, S, nrow, ncol1, pad, pad_mode,
-                                   apply_log, plan, p, R1, flag, s);
+        if (big) {
+            // spectra into R2 (natural order), then the row prefilter into R1
+            const int grid = std::min((nrow + 1) / 2, w.nscr);
+            if (f64) {
+                prepare_lds(dfi_rows_fwd_big_kernel<double>, lds_big);
+                hipLaunchKernelGGL(dfi_rows_fwd_big_kernel<double>,
+                                   dim3(grid), dim3(kBigNT), lds_big, s, S,
+                                   nrow, ncol1, pad, pad_mode, apply_log,
+                                   plan, p.N, p.tw, p.chirp, p.bhat, p.window,
+                                   R2, scr, flag);
+            } else {
+                prepare_lds(dfi_rows_fwd_big_kernel<float>, lds_big);
+                hipLaunchKernelGGL(dfi_rows_fwd_big_kernel<float>, dim3(grid),
+                                   dim3(kBigNT), lds_big, s, S, nrow, ncol1,
+                                   pad, pad_mode, apply_log, plan, p.N, p.tw,
+                                   p.chirp, p.bhat, p.window, R2, scr, flag);
+            }
+            st = check_hip(hipGetLastError(), "dfi_rows_fwd_big_kernel launch");
+            if (st != ATH_OK) return st;
+            const int ngroups = (p.N + kFirRun - 1) / kFirRun;
+            const dim3 fgrid((ngroups + kThreads - 1) / kThreads,
+                             std::min(nrow, 65535));
+            hipLaunchKernelGGL(dfi_rows_fir_kernel, fgrid, dim3(kThreads), 0,
+                               s, R2, nrow, p.N, h5, R1);
+            st = check_hip(hipGetLastError(), "dfi_rows_fir_kernel launch");
+        } else {
+            st = f64 ? rows_fwd<double>(maxb, S, nrow, ncol1, pad, pad_mode,
+                                        apply_log, plan, p, R1, flag, s)
+                     : rows_fwd<float>(maxb, S, nrow, ncol1, pad, pad_mode,
+                                       apply_log, plan, p, R1, flag, s);
+        }
         if (st != ATH_OK) return st;
         {
             const dim3 grid((p.N + 63) / 64,
@@ -965,7 +1281,16 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
                            dim3(kThreads), 0, s, R2, nrow, p.N, R1);
         st = check_hip(hipGetLastError(), "dfi_grid_kernel launch");
         if (st != ATH_OK) return st;
-        st = cols_inv(maxb, R1, h, ncol, pad, plan, p, R2, s);
+        if (big) {
+            prepare_lds(dfi_cols_inv_big_kernel, lds_big);
+            hipLaunchKernelGGL(dfi_cols_inv_big_kernel,
+                               dim3(std::min(h, w.nscr)), dim3(kBigNT),
+                               lds_big, s, R1, h, ncol, pad, plan, p.N, p.tw,
+                               p.chirp, p.bhat, R2, scr);
+            st = check_hip(hipGetLastError(), "dfi_cols_inv_big_kernel launch");
+        } else {
+            st = cols_inv(maxb, R1, h, ncol, pad, plan, p, R2, s);
+        }
         if (st != ATH_OK) return st;
         {
             const dim3 grid(std::min((ncol + 31) / 32, 65535),
@@ -978,13 +1303,33 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
         char* o = static_cast<char*>(out) +
                   (size_t)sl * (size_t)out_slice_stride *
                       (out_dtype == ATH_F64 ? sizeof(double) : sizeof(float));
-        st = out_dtype == ATH_F64
-                 ? rows_out<double>(maxb, R1, ncol, pad, plan, p,
-                                    reinterpret_cast<double*>(o),
-                                    out_row_stride, mask_r2, s)
-                 : rows_out<float>(maxb, R1, ncol, pad, plan, p,
+        if (big) {
+            const int grid = std::min((ncol + 1) / 2, w.nscr);
+            if (out_dtype == ATH_F64) {
+                prepare_lds(dfi_rows_out_big_kernel<double>, lds_big);
+                hipLaunchKernelGGL(dfi_rows_out_big_kernel<double>, dim3(grid),
+                                   dim3(kBigNT), lds_big, s, R1, ncol, pad,
+                                   plan, p.N, p.tw, p.chirp, p.bhat,
+                                   reinterpret_cast<double*>(o),
+                                   out_row_stride, mask_r2, scr);
+            } else {
+                prepare_lds(dfi_rows_out_big_kernel<float>, lds_big);
+                hipLaunchKernelGGL(dfi_rows_out_big_kernel<float>, dim3(grid),
+                                   dim3(kBigNT), lds_big, s, R1, ncol, pad,
+                                   plan, p.N, p.tw, p.chirp, p.bhat,
                                    reinterpret_cast<float*>(o),
-                                   out_row_stride, mask_r2, s);
+                                   out_row_stride, mask_r2, scr);
+            }
+            st = check_hip(hipGetLastError(), "dfi_rows_out_big_kernel launch");
+        } else {
+            st = out_dtype == ATH_F64
+                     ? rows_out<double>(maxb, R1, ncol, pad, plan, p,
+                                        reinterpret_cast<double*>(o),
+                                        out_row_stride, mask_r2, s)
+                     : rows_out<float>(maxb, R1, ncol, pad, plan, p,
+                                       reinterpret_cast<float*>(o),
+                                       out_row_stride, mask_r2, s);
+        }
         if (st != ATH_OK) return st;
     }
     return ATH_OK;

@@ -6,9 +6,9 @@ rounding (Bluestein vs numpy's FFT, order of the spline sums).  For float32
 sinograms the shifted sinogram and its log are rounded to float32 on both
 sides; on these seeded inputs every such rounding agrees (a disagreement
 would move one sample by a float32 ulp, ~1e-8 of the result, and fail).
-At the full detector width (2560 columns, Bluestein length 8192) where the
-oracle would take minutes: linearity in the sinogram (float64, apply_log
-off) and the analytic phantom's reconstruction error.
+At the full detector widths (2560 and 4096 columns, Bluestein lengths 8192
+and 16384) where the oracle would take minutes: linearity in the sinogram
+(float64, apply_log off) and the analytic phantom's reconstruction error.
 """
 
 import warnings
@@ -168,30 +168,37 @@ def test_dfi_stack_and_tensors():
     np.testing.assert_array_equal(one[:, 0, :], got[:, 1, :])
 
 
-@pytest.mark.parametrize("nrow,ncol,center,pad_rate", [
-    (361, 1024, 511.3, 0.25),   # width 1537, Bluestein length 4096
-    (48, 2049, 1024.3, 0.0),    # width 2049, Bluestein length 8192
+@pytest.mark.parametrize("nrow,ncol,center,pad_rate,dtype,log", [
+    # width 1537, Bluestein length 4096
+    (361, 1024, 511.3, 0.25, np.float64, False),
+    # width 2049, Bluestein length 8192
+    (48, 2049, 1024.3, 0.0, np.float64, False),
+    # width 4097, Bluestein length 16384 (two-level transform), an odd
+    # row count (the last row pair has one row), float32 with the log
+    (47, 4097, 2048.3, 0.0, np.float32, True),
 ])
-def test_dfi_large_vs_oracle(nrow, ncol, center, pad_rate):
+def test_dfi_large_vs_oracle(nrow, ncol, center, pad_rate, dtype, log):
     from oracle import dfi_np as D
     from algotom_amd.rec import reconstruction as rec
-    sino = _sino(nrow, ncol, np.float64, ncol)
+    sino = _sino(nrow, ncol, dtype, ncol)
     got = rec.dfi_reconstruction(sino, center, pad_rate=pad_rate,
-                                 apply_log=False)
-    ref = D.dfi(sino, center, pad_rate=pad_rate, apply_log=False)
+                                 apply_log=log)
+    ref = D.dfi(sino, center, pad_rate=pad_rate, apply_log=log)
     err = _rel(got, ref)
     print("%dx%d: %.2e" % (nrow, ncol, err))
     assert err <= 1e-12
 
 
-def test_dfi_full_width_linearity_and_phantom():
-    """1801 x 2560 (width 3841, Bluestein length 8192): the float64 path is
-    linear in the sinogram (apply_log off), and an analytic ellipse
-    phantom's sinogram reconstructs to the phantom."""
+@pytest.mark.parametrize("ncol", [2560, 4096])
+def test_dfi_full_width_linearity_and_phantom(ncol):
+    """1801 x 2560 (width 3841, Bluestein length 8192) and 1801 x 4096
+    (width 6145, length 16384): the float64 path is linear in the sinogram
+    (apply_log off), and an analytic ellipse phantom's sinogram
+    reconstructs to the phantom."""
     import torch
     from algotom_amd.rec import reconstruction as rec
     from algotom_amd.util import phantom
-    nrow, ncol, center = 1801, 2560, 1279.5
+    nrow, center = 1801, 0.5 * (ncol - 1)
     angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
     prm = phantom.random_ellipses(ncol, 12, seed=5)
     p = phantom.ellipse_sinogram(prm, torch.from_numpy(angles), ncol,
@@ -218,8 +225,8 @@ def test_dfi_full_width_linearity_and_phantom():
 
 def test_dfi_limits_and_warnings():
     from algotom_amd.rec import reconstruction as rec
-    with pytest.raises(NotImplementedError):
-        rec.dfi_reconstruction(_sino(64, 3300, np.float32, 1), 1650.0,
+    with pytest.raises(NotImplementedError):  # padded width 9001 > 8191
+        rec.dfi_reconstruction(_sino(64, 6000, np.float32, 1), 3000.0,
                                apply_log=False)
     with pytest.warns(Warning, match="undersampled"):
         rec.dfi_reconstruction(_sino(12, 65, np.float32, 2), 32.0,
