@@ -883,26 +883,34 @@ __global__ __launch_bounds__(kBigNT) void dfi_rows_fwd_big_kernel(
 }
 
 // The quintic prefilter along rows of Xn (natural order; spec[kp] =
-// Xn[(kp + m + 1) mod N]) into X (fftshifted order), kFirRun outputs per
-// thread.
+// Xn[(kp + m + 1) mod N]) into X (fftshifted order).  A workgroup stages
+// the spec segment its kFirSpan outputs need (plus kK5 either side) in LDS,
+// then each thread forms kFirRun consecutive outputs from it.
+constexpr int kFirSpan = kThreads * kFirRun;
+
 __global__ __launch_bounds__(kThreads) void dfi_rows_fir_kernel(
     const double2* __restrict__ Xn, int rows, int N,
     const double* __restrict__ h5, double2* __restrict__ X) {
+    __shared__ double2 seg[kFirSpan + 2 * kK5];
     const int m = (N - 1) / 2;
-    const int ngroups = (N + kFirRun - 1) / kFirRun;
+    const int tid = threadIdx.x;
     for (int row = blockIdx.y; row < rows; row += gridDim.y) {
         const double2* src = Xn + (size_t)row * N;
         double2* dst = X + (size_t)row * N;
-        for (int g = blockIdx.x * kThreads + threadIdx.x; g < ngroups;
-             g += gridDim.x * kThreads) {
-            const int k0 = g * kFirRun;
+        for (int c0 = blockIdx.x * kFirSpan; c0 < N;
+             c0 += gridDim.x * kFirSpan) {
+            for (int i = tid; i < kFirSpan + 2 * kK5; i += kThreads) {
+                int q = fold_half(c0 - kK5 + i, N) + m + 1;
+                if (q >= N) q -= N;
+                seg[i] = src[q];
+            }
+            __syncthreads();
+            const int l0 = tid * kFirRun;  // local index of the first output
             double2 a[kFirRun];
 #pragma unroll
             for (int r = 0; r < kFirRun; ++r) a[r] = make_double2(0.0, 0.0);
             for (int i = 0; i < kFirRun + 2 * kK5; ++i) {
-                int q = fold_half(k0 - kK5 + i, N) + m + 1;
-                if (q >= N) q -= N;
-                const double2 v = src[q];
+                const double2 v = seg[l0 + i];
 #pragma unroll
                 for (int r = 0; r < kFirRun; ++r) {
                     int d = i - kK5 - r;
@@ -916,7 +924,8 @@ __global__ __launch_bounds__(kThreads) void dfi_rows_fir_kernel(
             }
 #pragma unroll
             for (int r = 0; r < kFirRun; ++r)
-                if (k0 + r < N) dst[k0 + r] = a[r];
+                if (c0 + l0 + r < N) dst[c0 + l0 + r] = a[r];
+            __syncthreads();
         }
     }
 }
@@ -1255,8 +1264,7 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
             }
             st = check_hip(hipGetLastError(), "dfi_rows_fwd_big_kernel launch");
             if (st != ATH_OK) return st;
-            const int ngroups = (p.N + kFirRun - 1) / kFirRun;
-            const dim3 fgrid((ngroups + kThreads - 1) / kThreads,
+            const dim3 fgrid((p.N + kFirSpan - 1) / kFirSpan,
                              std::min(nrow, 65535));
             hipLaunchKernelGGL(dfi_rows_fir_kernel, fgrid, dim3(kThreads), 0,
                                s, R2, nrow, p.N, h5, R1);

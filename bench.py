@@ -467,8 +467,19 @@ def run_dfi(args, dev):
     one = sino[:, 0, :].contiguous()
     ms1, out1 = _timed(lambda: rec.dfi_reconstruction(one, CENTER),
                        args.steps, args.warmup)
+    # one 1801 x 4096 sinogram: padded width 6145, Bluestein length 16384
+    from algotom_amd.util import phantom
+    wide = 4096
+    prm = phantom.random_ellipses(wide, 20, seed=77)
+    pw = phantom.ellipse_sinogram(prm, np.linspace(0.0, np.pi, NROW,
+                                                   endpoint=False), wide,
+                                  0.5 * (wide - 1), device=dev)
+    sw = phantom.transmission(pw, seed=77)
+    msw, _ = _timed(lambda: rec.dfi_reconstruction(sw, 0.5 * (wide - 1)),
+                    args.steps, args.warmup)
     cpu = None
     detail = {"ms_per_slice": ms / nsl, "single_sinogram_ms": ms1,
+              "single_sinogram_1801x4096_ms": msw,
               "finite": bool(torch.isfinite(out).all().item())}
     if not args.no_cpu_baseline:
         from oracle import dfi_np
