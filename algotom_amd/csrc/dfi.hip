@@ -738,61 +738,8 @@ __global__ __launch_bounds__(kThreads) void dfi_transpose_kernel(
 }
 
 // ---- 7. Bluestein length 16384 (padded widths 4097 .. 8191) ---------------
-// An 8192-point float64 FFT is the largest one workgroup holds in LDS, so the
-// length-M = 2H convolution runs as H-point transforms: the even and odd
-// samples forward (decimation in time, A[k] = E[k] + W^k O[k] and A[k + H] =
-// E[k] - W^k O[k], W = exp(-2 pi i / M)), the product with the kernel
-// spectrum, then the inverse transforms of U = Y[k] + Y[k + H] (even
-// outputs) and V = (Y[k] - Y[k + H]) W^-k (odd outputs), decimation in
-// frequency.  E, U and the odd outputs pass through a per-workgroup global
-// scratch of 2H values.
-constexpr int kBigH = 8192;
-constexpr int kBigNT = 512;
-constexpr int kBigB = 4;  // butterflies per thread: kBigH / 4 / kBigNT
-
-// On entry LOAD(n) gives chirp[n] a[n] for n < N.  On exit conv[n] is
-// buf[n / 2] for even n and scratch[H + n / 2] for odd n (big_conv).
-// tw: the H-point FFT's table, tw[k] = exp(-2 pi i k / 2H) = W^k, k < 2H;
-// bhat: FFT_M of the chirp kernel over M.
-template <typename Load>
-__device__ void bluestein_big(double2* buf, double2* scratch, int N,
-                              const double2* __restrict__ tw,
-                              const double2* __restrict__ bhat, Load load) {
-    constexpr int H = kBigH, NT = kBigNT;
-    const int tid = threadIdx.x;
-    for (int j = tid; j < H; j += NT)
-        buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
-    __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
-    for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
-    __syncthreads();
-    for (int j = tid; j < H; j += NT)
-        buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
-    __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
-    for (int k = tid; k < H; k += NT) {
-        const double2 w = tw[k];
-        const double2 e = scratch[k];
-        const double2 o = cmul(buf[k], w);
-        const double2 ylo = cmul(cadd(e, o), bhat[k]);
-        const double2 yhi = cmul(csub(e, o), bhat[k + H]);
-        scratch[k] = cadd(ylo, yhi);
-        buf[k] = cmul(csub(ylo, yhi), conjg(w));
-    }
-    __syncthreads();
-    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
-    for (int q = tid; q < H; q += NT) scratch[H + q] = buf[q];
-    __syncthreads();
-    for (int k = tid; k < H; k += NT) buf[k] = scratch[k];
-    __syncthreads();
-    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
-}
-
-__device__ __forceinline__ double2 big_conv(const double2* buf,
-                                            const double2* scratch, int n) {
-    return (n & 1) ? scratch[kBigH + (n >> 1)] : buf[n >> 1];
-}
-
+// The convolution of length M = 2H runs as H-point LDS transforms with a
+// per-workgroup global scratch (conv_big in fft_lds.h); bhat carries 1/M.
 // bhat = FFT_M(b) / M for M = 2H (one workgroup; b as in bhat_kernel)
 __global__ __launch_bounds__(kBigNT) void bhat_big_kernel(
     const double2* __restrict__ chirp, const double2* __restrict__ tw, int N,
@@ -858,7 +805,7 @@ __global__ __launch_bounds__(kBigNT) void dfi_rows_fwd_big_kernel(
             const double2 ch = chirp[n];
             return make_double2(ch.x * v1 - ch.y * v2, ch.x * v2 + ch.y * v1);
         };
-        bluestein_big(buf, scratch, N, tw, bhat, load);
+        conv_big(buf, scratch, N, tw, bhat, load);
         double2* d1 = Xn + (size_t)r1 * N;
         double2* d2 = d1 + N;
         for (int k = tid; k <= m; k += kBigNT) {
@@ -948,7 +895,7 @@ __global__ __launch_bounds__(kBigNT) void dfi_cols_inv_big_kernel(
         auto load = [&](int n) -> double2 {
             return cmul(chirp[n], conjg(src[n]));
         };
-        bluestein_big(buf, scratch, N, tw, bhat, load);
+        conv_big(buf, scratch, N, tw, bhat, load);
         for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
             const int k = crop_index(kc, pad, m, N);
             out[(size_t)row * ncol + kc] = cscale(
@@ -986,7 +933,7 @@ __global__ __launch_bounds__(kBigNT) void dfi_rows_out_big_kernel(
             if (two) q = n <= m ? a2[n] : conjg(a2[N - n]);
             return cmul(chirp[n], make_double2(p.x - q.y, -(p.y + q.x)));
         };
-        bluestein_big(buf, scratch, N, tw, bhat, load);
+        conv_big(buf, scratch, N, tw, bhat, load);
         for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
             const int k = crop_index(kc, pad, m, N);
             const double2 y =

@@ -141,6 +141,88 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
     }
 }
 
+// Transforms of length 2H = 16384, the next size past what one workgroup
+// holds in LDS in float64.  conv_big forms the circular convolution
+// IFFT_2H(FFT_2H(x) S) (inverse unscaled; S carries the scale) from H-point
+// LDS transforms: the even and odd samples forward (decimation in time,
+// X[k] = E[k] + W^k O[k], X[k + H] = E[k] - W^k O[k], W = exp(-2 pi i/2H)),
+// the product Y = X S, then the inverse transforms of U = Y[k] + Y[k + H]
+// (even outputs) and V = (Y[k] - Y[k + H]) W^-k (odd outputs) -- decimation
+// in frequency.  E, U and the odd outputs pass through a per-workgroup
+// global scratch of 2H values.  512 threads of 4 butterflies.
+constexpr int kBigH = 8192;
+constexpr int kBigNT = 512;
+constexpr int kBigB = 4;  // butterflies per thread: kBigH / 4 / kBigNT
+
+// LOAD(n): x[n] for n < N (zero beyond).  tw: the H-point FFT's table,
+// tw[k] = exp(-2 pi i k / 2H) = W^k for k < 2H.  On exit output n is
+// big_conv(buf, scratch, n).  All kBigNT threads.
+template <typename Load>
+__device__ void conv_big(double2* buf, double2* scratch, int N,
+                         const double2* __restrict__ tw,
+                         const double2* __restrict__ S, Load load) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
+    __syncthreads();
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) {
+        const double2 w = tw[k];
+        const double2 e = scratch[k];
+        const double2 o = cmul(buf[k], w);
+        const double2 ylo = cmul(cadd(e, o), S[k]);
+        const double2 yhi = cmul(csub(e, o), S[k + H]);
+        scratch[k] = cadd(ylo, yhi);
+        buf[k] = cmul(csub(ylo, yhi), conjg(w));
+    }
+    __syncthreads();
+    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+    for (int q = tid; q < H; q += NT) scratch[H + q] = buf[q];
+    __syncthreads();
+    for (int k = tid; k < H; k += NT) buf[k] = scratch[k];
+    __syncthreads();
+    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+}
+
+__device__ __forceinline__ double2 big_conv(const double2* buf,
+                                            const double2* scratch, int n) {
+    return (n & 1) ? scratch[kBigH + (n >> 1)] : buf[n >> 1];
+}
+
+// Forward FFT of length 2H of x (LOAD as in conv_big): STORE(k, X[k]) for
+// every k < 2H.  All kBigNT threads.
+template <typename Load, typename Store>
+__device__ void fft_big_forward(double2* buf, double2* scratch, int N,
+                                const double2* __restrict__ tw, Load load,
+                                Store store) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
+    __syncthreads();
+    for (int j = tid; j < H; j += NT)
+        buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
+    __syncthreads();
+    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    for (int k = tid; k < H; k += NT) {
+        const double2 e = scratch[k];
+        const double2 o = cmul(buf[k], tw[k]);
+        store(k, cadd(e, o));
+        store(k + H, csub(e, o));
+    }
+    __syncthreads();
+}
+
 // Z = FFT_M(x[2n] + i x[2n+1]) -> X[k] = rfft_L(x)[k] for k and M-k.
 __device__ __forceinline__ void split_real(double2 zk, double2 zm, double2 wk,
                                            double2& xk, double2& xm) {

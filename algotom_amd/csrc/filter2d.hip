@@ -24,7 +24,9 @@
 // The kernel spectrum Khat (L/2+1 columns of L complex values, 1/(M L)
 // folded in) is built once per (W, window) by the same two forward passes
 // over the 2W rows of K.  The column transform holds L complex doubles in
-// LDS, so L <= 8192 (ncol up to about 3800 with the default pad).
+// LDS, so L <= 8192 (ncol up to about 3800 with the default pad); L = 16384
+// (ncol up to about 7960) runs the column pass as the two-level transform
+// of fft_lds.h (cols_big_kernel) with a per-workgroup global scratch.
 // Kernel samples at index >= L are dropped rather than wrapped: a kept
 // output (W+pad+y, W+pad+x) only meets K at offsets pad+1 .. 2W-pad-1 < L
 // in each axis, so no needed term is lost and none aliases.
@@ -140,6 +142,58 @@ __global__ __launch_bounds__(F2Cfg<MAXB>::nt) void cols_kernel(
         dst[(size_t)y * (M + 1)] = buf[r0 + y];
 }
 
+// Column pass at L = 2H = 16384 (conv_big / fft_big_forward), persistent
+// over the (image, bin column) pairs; workgroup b uses scratch slot b.
+// tw: the row transforms' table, exp(-2 pi i n / L), which is the H-point
+// table the two-level transform needs.
+template <bool SPECTRUM>
+__global__ __launch_bounds__(kBigNT) void cols_big_kernel(
+    const double2* __restrict__ T, int nrows_in, const double2* __restrict__ tw,
+    double2* __restrict__ khat_out, const double2* __restrict__ khat,
+    double2* __restrict__ U, int r0, int nout, int M, double scale,
+    int nimg, double2* __restrict__ scratch_all) {
+    extern __shared__ double2 buf[];
+    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    const int L = 2 * M;
+    const long long ncols = (long long)(M + 1) * nimg;
+    for (long long c = blockIdx.x; c < ncols; c += gridDim.x) {
+        const int img = (int)(c / (M + 1));  // block-uniform
+        const int k = (int)(c - (long long)img * (M + 1));
+        const double2* col = T + (size_t)img * nrows_in * (M + 1) + k;
+        auto load = [&](int n) -> double2 { return col[(size_t)n * (M + 1)]; };
+        if (SPECTRUM) {
+            double2* dst = khat_out + (size_t)k * L;
+            fft_big_forward(buf, scratch, nrows_in, tw, load,
+                            [&](int i, double2 v) { dst[i] = cscale(v, scale); });
+        } else {
+            conv_big(buf, scratch, nrows_in, tw, khat + (size_t)k * L, load);
+            double2* dst = U + (size_t)img * nout * (M + 1) + k;
+            for (int y = threadIdx.x; y < nout; y += kBigNT)
+                dst[(size_t)y * (M + 1)] = big_conv(buf, scratch, r0 + y);
+            __syncthreads();
+        }
+    }
+}
+
+// Scratch slots of the L = 16384 column pass: one per compute unit (its
+// 128 KB of LDS leaves room for one workgroup per CU).
+inline int big_slots() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return cus;
+}
+
+inline size_t big_scratch_bytes(int L) {
+    return L == 2 * kBigH
+               ? (size_t)big_slots() * 2 * kBigH * sizeof(double2)
+               : 0;
+}
+
 struct Finish2d {
     int ncol, c0;             // output columns, first kept sample
     long long out_row_stride, out_img_stride;
@@ -240,6 +294,41 @@ int build_plan(const double* g2, int width, int L, void* plan, void* work,
     return check_hip(hipGetLastError(), "cols_kernel (spectrum) launch");
 }
 
+// L = 16384: the row passes as for L <= 8192 (class 8: 1024 threads), the
+// column pass two-level.  work: the row-spectrum rows, then the scratch.
+int build_plan_big(const double* g2, int width, int L, void* plan,
+                   void* work, hipStream_t s) {
+    const int M = L / 2;
+    double2* tw = static_cast<double2*>(plan);
+    double2* tw2 = tw + L;
+    double2* khat = tw + 3 * (size_t)L;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((L + 255) / 256), dim3(256), 0, s,
+                       tw, L);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((2 * L + 255) / 256), dim3(256),
+                       0, s, tw2, 2 * L);
+    int st = check_hip(hipGetLastError(), "twiddle_kernel launch");
+    if (st != ATH_OK) return st;
+    auto* T = static_cast<double2*>(work);
+    const int krows = std::min(2 * width, L);
+    auto* scr = T + (size_t)krows * (M + 1);
+    auto rk = rows_fwd_kernel<8, kSrcKernel>;
+    const size_t lds_r = (size_t)M * sizeof(double2);
+    prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
+    hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(F2Cfg<8>::nt), lds_r, s,
+                       nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE, width,
+                       tw, T, M);
+    st = check_hip(hipGetLastError(), "rows_fwd_kernel (kernel) launch");
+    if (st != ATH_OK) return st;
+    auto ck = cols_big_kernel<true>;
+    const size_t lds_c = (size_t)kBigH * sizeof(double2);
+    prepare_lds(ck, lds_c);
+    const int grid = std::min(M + 1, big_slots());
+    hipLaunchKernelGGL(ck, dim3(grid), dim3(kBigNT), lds_c, s, T, krows, tw,
+                       khat, nullptr, nullptr, 0, 0, M,
+                       1.0 / ((double)M * (double)L), 1, scr);
+    return check_hip(hipGetLastError(), "cols_big_kernel (spectrum) launch");
+}
+
 template <int MAXB_R, int MAXB_C, typename Tout>
 int run_filter(const float* images, int nimg, int ncol, long long row_stride,
                long long img_stride, int pad, int pad_mode, const void* plan,
@@ -276,6 +365,42 @@ int run_filter(const float* images, int nimg, int ncol, long long row_stride,
     return check_hip(hipGetLastError(), "rows_inv_kernel launch");
 }
 
+template <typename Tout>
+int run_filter_big(const float* images, int nimg, int ncol,
+                   long long row_stride, long long img_stride, int pad,
+                   int pad_mode, const void* plan, int L, void* work,
+                   Tout* out, const Finish2d& f, hipStream_t s) {
+    const int M = L / 2;
+    const int width = ncol + 2 * pad;
+    const Plan2d p = plan_view(plan, L);
+    auto* T = static_cast<double2*>(work);
+    auto* U = T + (size_t)nimg * width * (M + 1);
+    auto* scr = U + (size_t)nimg * ncol * (M + 1);
+    auto rk = rows_fwd_kernel<8, kSrcImage>;
+    const size_t lds_r = (size_t)M * sizeof(double2);
+    prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
+    hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(F2Cfg<8>::nt), lds_r, s,
+                       images, row_stride, img_stride, nullptr, ncol, pad,
+                       pad_mode, width, p.tw, T, M);
+    int st = check_hip(hipGetLastError(), "rows_fwd_kernel launch");
+    if (st != ATH_OK) return st;
+    auto ck = cols_big_kernel<false>;
+    const size_t lds_c = (size_t)kBigH * sizeof(double2);
+    prepare_lds(ck, lds_c);
+    const long long ncols = (long long)(M + 1) * nimg;
+    const int grid = (int)std::min<long long>(ncols, big_slots());
+    hipLaunchKernelGGL(ck, dim3(grid), dim3(kBigNT), lds_c, s, T, width, p.tw,
+                       nullptr, p.khat, U, width + pad, ncol, M, 1.0, nimg,
+                       scr);
+    st = check_hip(hipGetLastError(), "cols_big_kernel launch");
+    if (st != ATH_OK) return st;
+    auto ik = rows_inv_kernel<8, Tout>;
+    prepare_lds(ik, 1024 * (size_t)8 * sizeof(double2));
+    hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(F2Cfg<8>::nt), lds_r, s, U,
+                       p.tw, out, f, M);
+    return check_hip(hipGetLastError(), "rows_inv_kernel launch");
+}
+
 }  // namespace
 
 size_t circular_plan_bytes(int L) {
@@ -284,13 +409,15 @@ size_t circular_plan_bytes(int L) {
 }
 
 size_t circular_plan_work_bytes(int width, int L) {
-    return 2 * (size_t)width * ((size_t)L / 2 + 1) * sizeof(double2);
+    return 2 * (size_t)width * ((size_t)L / 2 + 1) * sizeof(double2) +
+           big_scratch_bytes(L);
 }
 
 size_t circular_filter_work_bytes(int ncol, int pad, int L, int nimg) {
     const size_t width = (size_t)ncol + 2 * (size_t)pad;
     return (size_t)nimg * (width + (size_t)ncol) * ((size_t)L / 2 + 1) *
-           sizeof(double2);
+               sizeof(double2) +
+           big_scratch_bytes(L);
 }
 
 int circular_plan_build(const double* g2, int width, int L, void* plan,
@@ -301,9 +428,10 @@ int circular_plan_build(const double* g2, int width, int L, void* plan,
     case 2048: return build_plan<1, 2>(g2, width, L, plan, work, s);
     case 4096: return build_plan<2, 4>(g2, width, L, plan, work, s);
     case 8192: return build_plan<4, 8>(g2, width, L, plan, work, s);
+    case 16384: return build_plan_big(g2, width, L, plan, work, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "circular ramp filter: FFT length above 8192");
+                     "circular ramp filter: FFT length above 16384");
 }
 
 template <typename Tout>
@@ -321,10 +449,14 @@ static int dispatch_filter(const float* images, int nimg, int ncol,
     case 2048: ATH_F2D(1, 2)
     case 4096: ATH_F2D(2, 4)
     case 8192: ATH_F2D(4, 8)
+    case 16384:
+        return run_filter_big<Tout>(images, nimg, ncol, row_stride,
+                                    img_stride, pad, pad_mode, plan, L, work,
+                                    out, f, s);
     }
 #undef ATH_F2D
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "circular ramp filter: FFT length above 8192");
+                     "circular ramp filter: FFT length above 16384");
 }
 
 int circular_ramp_filter(const float* images, int nimg, int ncol,

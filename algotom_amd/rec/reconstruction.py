@@ -479,7 +479,8 @@ def _circular_plan(ramp_win, dev, name_key=None):
     from it (one per FFT length): the filter is the circular convolution
     with g2 = Re(ifft2(ifftshift(R2))) (reconstruction.py:555-559),
     evaluated by ath_circular_ramp_filter.  One window is kept resident
-    (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192).
+    (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192,
+    2.1 GB at 16384).
     Default windows are keyed by (width, filter_name); a caller's window by
     a hash of its values."""
     import hashlib
@@ -510,7 +511,7 @@ def _circular_plan_for(entry, fft_len, dev):
     nwork = lib.ath_circular_filter_plan_work_bytes(width, fft_len)
     if nbytes == 0 or nwork == 0:
         raise NotImplementedError(
-            "the circular ramp filter supports FFT lengths up to 8192 "
+            "the circular ramp filter supports FFT lengths up to 16384 "
             "(width %d needs %d)" % (width, fft_len))
     plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     work = torch.empty(nwork, dtype=torch.uint8, device=dev)

@@ -276,7 +276,7 @@ int ath_minus_log(const float* in, float* out, long long n, void* stream);
 
 /* Bytes of a circular-filter plan for FFT length fft_len (twiddles + the
  * 2D spectrum of the tiled kernel, float64); 0 if fft_len is not a power
- * of two in [64, 8192].  fft_len comes from ath_filter_fft_length(ncol,
+ * of two in [64, 16384].  fft_len comes from ath_filter_fft_length(ncol,
  * pad), the same rule as the 1D filter. */
 size_t ath_circular_filter_plan_bytes(int fft_len);
 

@@ -1,7 +1,7 @@
 """BPF on the GPU against the oracle (oracle/bpf_np.py).
 
 * apply_circular_ramp_filter: float64 output within 1e-11 of the oracle's
-  complex128 spectral product (FFT lengths 64 .. 8192, every pad mode);
+  complex128 spectral product (FFT lengths 64 .. 16384, every pad mode);
 * bpf_reconstruction, one sinogram: float64 output within 1e-10 (the
   back-projection is bit-identical; the filters differ by float64 rounding);
 * stacks: float32 output within 2e-6 of max (float32 rounding flips);
@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
     (5, 2, "edge"), (64, None, "edge"), (65, 7, "constant"),
     (65, 7, "reflect"), (65, 7, "symmetric"), (65, 7, "wrap"),
     (257, None, "edge"), (512, None, "edge"), (1024, None, "edge"),
-    (2560, None, "edge")])
+    (2560, None, "edge"),
+    (4096, None, "edge")])  # W = 4396: FFT length 16384, two-level columns
 def test_circular_filter_vs_oracle(ncol, pad, mode):
     from oracle import bpf_np
     from algotom_amd.rec import reconstruction as rec
