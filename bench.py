@@ -437,6 +437,16 @@ def run_bpf(args, dev):
     one = sino[:, 0, :].contiguous()
     ms2d, _ = _timed(lambda: rec.bpf_reconstruction(one, CENTER, angles),
                      args.steps, args.warmup)
+    # one 1801 x 4096 sinogram: 2D filter FFT length 16384 (two-level columns)
+    from algotom_amd.util import phantom
+    wide = 4096
+    prm = phantom.random_ellipses(wide, 20, seed=77)
+    pw = phantom.ellipse_sinogram(prm, angles, wide, 0.5 * (wide - 1),
+                                  device=dev)
+    sw = phantom.transmission(pw, seed=77)
+    msw, _ = _timed(lambda: rec.bpf_reconstruction(sw, 0.5 * (wide - 1),
+                                                   angles),
+                    args.steps, args.warmup)
     line = {
         "metric": METRIC, "value": updates / (ms * 1e-3) / 1e9,
         "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
@@ -449,6 +459,7 @@ def run_bpf(args, dev):
         "cpu_baseline": None,
         "detail": {"ms_per_slice": ms / nsl,
                    "single_sinogram_ms (float64 output)": ms2d,
+                   "single_sinogram_1801x4096_ms": msw,
                    "finite": bool(torch.isfinite(out).all().item())},
     }
     print(json.dumps(line))
