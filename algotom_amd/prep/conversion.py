@@ -10,6 +10,7 @@ arrays: both raise NotImplementedError rather than falling back to the CPU.
 
 import numpy as np
 
+from algotom_amd import _host
 from algotom_amd import _native as nat
 
 
@@ -81,7 +82,7 @@ def convert_sinogram_360_to_180(sino_360, cor, wei_mat1=None, wei_mat2=None,
     if two_d:
         out = out[:, 0, :]
     if as_numpy:
-        out = out.cpu().numpy()
+        out = _host.to_host(out)
     return out, new_cor
 
 
@@ -133,5 +134,5 @@ def extend_sinogram(sino_360, cor, apply_log=True):
     if two_d:
         out = out[:, 0, :]
     if as_numpy:
-        out = out.cpu().numpy()
+        out = _host.to_host(out)
     return out, new_cor

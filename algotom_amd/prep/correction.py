@@ -12,6 +12,7 @@ modules' methods: passing one raises NotImplementedError.
 
 import numpy as np
 
+from algotom_amd import _host
 from algotom_amd import _native as nat
 
 
@@ -105,4 +106,4 @@ def flat_field_correction(proj, flat, dark, ratio=1.0, use_dark=True,
         nat.ptr(dark_t) if dark_t is not None else None, nat.ptr(den_t),
         nat.dtype_code(tdt), nat.ptr(out), nat.stream_ptr(dev)),
         "ath_flat_field_correction")
-    return out.cpu().numpy() if as_numpy else out
+    return _host.to_host(out) if as_numpy else out

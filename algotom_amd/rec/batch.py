@@ -227,15 +227,15 @@ class BatchFBP:
         for i, s in enumerate(sinograms):
             t = s if isinstance(s, torch.Tensor) else torch.from_numpy(
                 np.ascontiguousarray(s, dtype=np.float32))
-            if self.apply_log:
-                rec._clamp_nonpositive(t)
             self.inputs[i].copy_(t.reshape(self.inputs[i].shape))
+            if self.apply_log:
+                rec._clamp_nonpositive_device(s, self.inputs[i])
 
     def results(self, as_numpy=True):
         out = []
         for o, is3d in zip(self.outputs, self.input_3d):
             r = o if is3d else o[:, 0, :]
-            out.append(r.cpu().numpy() if as_numpy else r)
+            out.append(rec._to_host(r) if as_numpy else r)
         return out
 
 

@@ -30,6 +30,7 @@ import warnings
 import numpy as np
 from scipy.signal import windows as _windows
 
+from algotom_amd import _host
 from algotom_amd import _native as nat
 from algotom_amd.rec import autotune
 
@@ -191,6 +192,22 @@ def _window_taps(ramp_win, nrow, width, filter_name):
     return _taps(win)
 
 
+def _clamp_nonpositive_device(sinogram, sino_t):
+    """reconstruction.py:431-435 for an input already on the device as
+    ``sino_t``: the test runs there; if any value is <= 0, warn and set
+    those values to 1 in ``sino_t`` and, in place, in the caller's array."""
+    bad = sino_t <= 0.0
+    if bool(bad.any()):
+        warnings.warn("!!!Applying logarithm is enabled but "
+                      "there are values <= 0.0 in the data!!!")
+        sino_t[bad] = 1.0
+        if _is_tensor(sinogram):
+            if sinogram.data_ptr() != sino_t.data_ptr():
+                sinogram[sinogram <= 0.0] = 1.0
+        else:
+            sinogram[sinogram <= 0.0] = np.float32(1.0)
+
+
 def _clamp_nonpositive(sinogram):
     """reconstruction.py:431-435: warn and set values <= 0 to 1, in place."""
     if _is_tensor(sinogram):
@@ -256,9 +273,12 @@ def _backproject_stack(filt_t, angles, center, acc, edge_pad=False,
     return out
 
 
+_to_host = _host.to_host
+
+
 def _finish(out_t, as_numpy):
     if as_numpy:
-        return out_t.cpu().numpy()
+        return _to_host(out_t)
     return out_t
 
 
@@ -413,8 +433,9 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
                          "number of the sinogram!!!")
     dev = nat.device()
     as_numpy = not _is_tensor(sinogram)
+    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
     if apply_log:
-        _clamp_nonpositive(sinogram)
+        _clamp_nonpositive_device(sinogram, sino_t)
     mask_radius = -1.0
     if ratio is not None:
         if ratio == 0.0:
@@ -427,7 +448,6 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
         acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ
     else:
         acc = nat.ACC_F32
-    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
     filt = _filter_stack(sino_t, pad, pad_mode, apply_log, taps,
                          _storage_dtype(precision))
     out = _backproject_stack(filt, angles, center, acc,
@@ -651,8 +671,10 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
                          "number of the sinogram!!!")
     dev = nat.device()
     as_numpy = not _is_tensor(sinogram)
+    sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
+        nrow, num_sino, ncol)
     if apply_log:
-        _clamp_nonpositive(sinogram)
+        _clamp_nonpositive_device(sinogram, sino_t)
     mask_radius = -1.0
     if ratio is not None:
         if ratio == 0.0:
@@ -671,8 +693,6 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
         regen = filter_name if num_sino > 1 else None
         ramp_win = _circular_window(width, regen)
         name_key = ("window", regen)
-    sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
-        nrow, num_sino, ncol)
     if apply_log:
         sino_t = _minus_log(sino_t)
     acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ

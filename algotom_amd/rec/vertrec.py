@@ -663,7 +663,7 @@ def find_center_visual_vertical_slices(projections, output_folder,
             p, xs, ys, True, float(cs[0]), None, None, angles, (0, 0, 0, 0),
             0, -1, chunk_size, ramp_filter, "hann", pad, "edge", False, gpu,
             False, False, centers=cs, mask_lohi=lohi, keep_device=True)
-        imgs = imgs.cpu().numpy()
+        imgs = rec._to_host(imgs)
         for k, center in enumerate(cs):
             file_name = "center_{0:.2f}".format(center) + ".tif"
             losa.save_image(output_base + file_name, imgs[k])

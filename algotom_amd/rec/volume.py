@@ -58,6 +58,6 @@ def reconstruct_volume_360(sino_360, cor, chunk=16, angles=None, ratio=1.0,
         if rec._is_tensor(out):
             out[:, j0:j1, :] = r
         else:
-            out[:, j0:j1, :] = r.cpu().numpy()
+            out[:, j0:j1, :] = rec._to_host(r)
         del stitched, r
     return out, new_cor

@@ -217,3 +217,22 @@ def test_apply_log_mutates_caller_array():
         rec.fbp_reconstruction(sino, 32.0, apply_log=True)
     assert any("values <= 0.0" in str(x.message) for x in w)
     assert sino[3, 5] == 1.0 and sino[7, 9] == 1.0
+
+
+@pytest.mark.parametrize("dtype,shape", [
+    ("float32", (1801, 7, 5003)),     # ~252 MB: one full and one partial buffer
+    ("float64", (2049, 67, 331)),     # ~363 MB: both buffers reused
+    ("float32", (7, 5, 3))])          # below the staging threshold
+def test_staged_host_copy(dtype, shape):
+    """The staged device->host copy returns exactly what .cpu() does,
+    including for a non-contiguous view."""
+    import torch
+    from algotom_amd import _host
+    g = torch.Generator(device="cuda").manual_seed(5)
+    t = torch.randn(shape, dtype=getattr(torch, dtype), device="cuda",
+                    generator=g)
+    got = _host.to_host(t)
+    assert got.dtype == np.dtype(dtype) and got.shape == shape
+    assert np.array_equal(got, t.cpu().numpy())
+    view = t.transpose(0, 1)
+    assert np.array_equal(_host.to_host(view), view.cpu().numpy())
