@@ -275,6 +275,58 @@ def _backproject_stack(filt_t, angles, center, acc, edge_pad=False,
 
 _to_host = _host.to_host
 
+# Host (numpy) stacks of _STREAM_MIN bytes or more run in chunks of slices
+# through a pipeline that overlaps the uploads and downloads of neighbouring
+# chunks with the computation; each chunk carries about _CHUNK_BYTES of
+# input plus output.
+_STREAM_MIN = 256 << 20
+_CHUNK_BYTES = 384 << 20
+
+
+def _streams(sinogram, out_width, out_itemsize=4):
+    nrow, nslice, ncol = sinogram.shape
+    nbytes = nslice * (4 * nrow * ncol + out_itemsize * out_width * out_width)
+    return nslice > 1 and nbytes >= _STREAM_MIN
+
+
+def _chunk_ranges(nslice, slice_bytes):
+    per = max(1, min(nslice, _CHUNK_BYTES // max(1, slice_bytes)))
+    if per >= 8:
+        per -= per % 4
+    return [(j, min(nslice, j + per)) for j in range(0, nslice, per)]
+
+
+def _stream_numpy(sinogram, out_shape, run, apply_log, dev, allow_f64,
+                  out_dtype=np.float32):
+    """``run`` over slice chunks of a host (nrow, nslice, ncol) stack with
+    overlapped transfers; returns the new (a, nslice, b) array.  With
+    ``apply_log`` the values <= 0 are set to 1 on the device, and, as
+    reconstruction.py:431-435 does, a warning is given and the caller's
+    array is fixed in place when there were any."""
+    import torch
+    a = np.asarray(sinogram)
+    in_dtype = (torch.float64 if allow_f64 and a.dtype == np.float64
+                else torch.float32)
+    out = np.empty(out_shape, dtype=out_dtype)
+    nrow, nslice, ncol = a.shape
+    ranges = _chunk_ranges(nslice, 4 * nrow * ncol + out.itemsize
+                           * out_shape[0] * out_shape[2])
+    flags = []
+
+    def step(part):
+        if apply_log:
+            bad = part <= 0.0
+            flags.append(bad.any())
+            part.masked_fill_(bad, 1.0)
+        return run(part)
+
+    _host.stream_chunks(a, out, ranges, step, in_dtype, dev)
+    if flags and bool(torch.stack(flags).any()):
+        warnings.warn("!!!Applying logarithm is enabled but "
+                      "there are values <= 0.0 in the data!!!")
+        sinogram[sinogram <= 0.0] = np.float32(1.0)
+    return out
+
 
 def _finish(out_t, as_numpy):
     if as_numpy:
@@ -433,9 +485,6 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
                          "number of the sinogram!!!")
     dev = nat.device()
     as_numpy = not _is_tensor(sinogram)
-    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
-    if apply_log:
-        _clamp_nonpositive_device(sinogram, sino_t)
     mask_radius = -1.0
     if ratio is not None:
         if ratio == 0.0:
@@ -448,10 +497,21 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
         acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ
     else:
         acc = nat.ACC_F32
-    filt = _filter_stack(sino_t, pad, pad_mode, apply_log, taps,
-                         _storage_dtype(precision))
-    out = _backproject_stack(filt, angles, center, acc,
-                             mask_radius=mask_radius, apply_scale=True)
+    storage = _storage_dtype(precision)
+    ang = _angles_device(angles, dev)
+
+    def run(sino_t):
+        filt = _filter_stack(sino_t, pad, pad_mode, apply_log, taps, storage)
+        return _backproject_stack(filt, ang, center, acc,
+                                  mask_radius=mask_radius, apply_scale=True)
+
+    if as_numpy and input_3d and _streams(sinogram, ncol):
+        return _stream_numpy(sinogram, (ncol, num_sino, ncol), run,
+                             apply_log, dev, allow_f64=True)
+    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
+    if apply_log:
+        _clamp_nonpositive_device(sinogram, sino_t)
+    out = run(sino_t)
     if not input_3d:
         out = out.reshape(ncol, ncol)
     return _finish(out, as_numpy)
@@ -671,10 +731,6 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
                          "number of the sinogram!!!")
     dev = nat.device()
     as_numpy = not _is_tensor(sinogram)
-    sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
-        nrow, num_sino, ncol)
-    if apply_log:
-        _clamp_nonpositive_device(sinogram, sino_t)
     mask_radius = -1.0
     if ratio is not None:
         if ratio == 0.0:
@@ -693,24 +749,34 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
         regen = filter_name if num_sino > 1 else None
         ramp_win = _circular_window(width, regen)
         name_key = ("window", regen)
-    if apply_log:
-        sino_t = _minus_log(sino_t)
     acc = nat.ACC_F64 if (gpu and num_sino == 1) else nat.ACC_F32SEQ
-    bp = _backproject_stack(sino_t, angles, center, acc, edge_pad=True)
-    if num_sino == 1:
-        out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
-        _circular_filter(bp, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
-                         ramp_win, out, ncol, ncol * ncol, mask_radius, nrow,
+    ang = _angles_device(angles, dev)
+
+    def run(sino_t):
+        if apply_log:
+            sino_t = _minus_log(sino_t)
+        bp = _backproject_stack(sino_t, ang, center, acc, edge_pad=True)
+        if num_sino == 1:
+            out = torch.empty((ncol, ncol), dtype=torch.float64, device=dev)
+            _circular_filter(bp, ncol, 1, ncol, ncol * ncol, pad, pad_mode,
+                             ramp_win, out, ncol, ncol * ncol, mask_radius,
+                             nrow, name_key)
+            return out.reshape(ncol, 1, ncol) if input_3d else out
+        ns = sino_t.shape[1]
+        out = torch.empty((ncol, ns, ncol), dtype=torch.float32, device=dev)
+        _circular_filter(bp, ncol, ns, ns * ncol, ncol, pad, pad_mode,
+                         ramp_win, out, ns * ncol, ncol, mask_radius, nrow,
                          name_key)
-        if input_3d:
-            out = out.reshape(ncol, 1, ncol)
-    else:
-        out = torch.empty((ncol, num_sino, ncol), dtype=torch.float32,
-                          device=dev)
-        _circular_filter(bp, ncol, num_sino, num_sino * ncol, ncol, pad,
-                         pad_mode, ramp_win, out, num_sino * ncol, ncol,
-                         mask_radius, nrow, name_key)
-    return _finish(out, as_numpy)
+        return out
+
+    if as_numpy and input_3d and _streams(sinogram, ncol):
+        return _stream_numpy(sinogram, (ncol, num_sino, ncol), run,
+                             apply_log, dev, allow_f64=False)
+    sino_t = _to_device(sinogram, dev, allow_f64=False).reshape(
+        nrow, num_sino, ncol)
+    if apply_log:
+        _clamp_nonpositive_device(sinogram, sino_t)
+    return _finish(run(sino_t), as_numpy)
 
 
 # --------------------------------------------------------------------------
@@ -815,26 +881,39 @@ def dfi_reconstruction(sinogram, center, angles=None, ratio=1.0,
         mask_r2 = float(radius * radius)
     dev = nat.device()
     as_numpy = not _is_tensor(sinogram)
-    sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
     plan = _dfi_plan(width, filter_name, dev)
     lib = nat.load()
     nws = lib.ath_dfi_workspace_bytes(nrow, ncol, rows, width)
     work = torch.empty(nws, dtype=torch.uint8, device=dev)
-    out = torch.empty((ncol, num_sino, ncol), dtype=torch.float64, device=dev)
-    flag = torch.zeros(1, dtype=torch.int32, device=dev)
-    nat.check(lib.ath_dfi_reconstruction(
-        nat.ptr(sino_t), nat.dtype_code(sino_t.dtype), nrow, num_sino, ncol,
-        num_sino * ncol, ncol, float(xshift), pad, nat.PAD_MODES[pad_mode],
-        int(bool(apply_log)), mode, row_shift, flip, rows, nat.ptr(plan),
-        width, nat.ptr(work), nws, nat.ptr(out), nat.ATH_F64,
-        num_sino * ncol, ncol, mask_r2, nat.ptr(flag), nat.stream_ptr(dev)),
-        "ath_dfi_reconstruction")
-    if apply_log and int(flag.item()):
+    flags = []
+
+    def run(sino_t):
+        ns = sino_t.shape[1]
+        out = torch.empty((ncol, ns, ncol), dtype=torch.float64, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        nat.check(lib.ath_dfi_reconstruction(
+            nat.ptr(sino_t), nat.dtype_code(sino_t.dtype), nrow, ns, ncol,
+            ns * ncol, ncol, float(xshift), pad, nat.PAD_MODES[pad_mode],
+            int(bool(apply_log)), mode, row_shift, flip, rows, nat.ptr(plan),
+            width, nat.ptr(work), nws, nat.ptr(out), nat.ATH_F64, ns * ncol,
+            ncol, mask_r2, nat.ptr(flag), nat.stream_ptr(dev)),
+            "ath_dfi_reconstruction")
+        flags.append(flag)
+        return out
+
+    if as_numpy and input_3d and _streams(sinogram, ncol, 8):
+        out = _stream_numpy(sinogram, (ncol, num_sino, ncol), run, False,
+                            dev, allow_f64=True, out_dtype=np.float64)
+    else:
+        sino_t = _to_device(sinogram, dev).reshape(nrow, num_sino, ncol)
+        out = run(sino_t)
+        if not input_3d:
+            out = out.reshape(ncol, ncol)
+        out = _finish(out, as_numpy)
+    if apply_log and bool(torch.cat(flags).any()):
         warnings.warn("!!!Applying logarithm is enabled but there are "
                       "values <= 0.0 in the data!!!")
-    if not input_3d:
-        out = out.reshape(ncol, ncol)
-    return _finish(out, as_numpy)
+    return out
 
 
 # Centre search by slice metrics (reconstruction.py:1254-1458), implemented

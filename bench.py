@@ -140,9 +140,10 @@ def cpu_baseline(sino_dev, angles, gpu_out, nsample=4):
 
 
 def host_roundtrip(sino_dev, angles, reps=2):
-    """fbp_reconstruction from host numpy to host numpy (pageable memory:
-    the <=0 check, host-to-device copy, filter, back-projection,
-    device-to-host copy): the drop-in's PCIe-inclusive time, best of reps."""
+    """fbp_reconstruction from host numpy to host numpy (slice chunks whose
+    host-to-device and device-to-host copies overlap the filter and
+    back-projection of their neighbours): the drop-in's PCIe-inclusive time,
+    best of reps."""
     from algotom_amd.rec import reconstruction as rec
     s_np = sino_dev.cpu().numpy()
     best = None
@@ -291,6 +292,19 @@ def run_volume(args, dev):
         "detail": {"finite": bool(torch.isfinite(out).all().item()),
                    "max_abs": float(out.abs().max().item())},
     }
+    # host in, host out (7.55 GB in, 26.5 GB out): the chunk pipeline
+    # overlaps both transfers with the computation
+    del out
+    sino_np = sino.cpu().numpy()
+    del sino
+    torch.cuda.empty_cache()
+    out_np = np.empty((width, nsl, width), dtype=np.float32)
+    t0 = time.perf_counter()
+    volume.reconstruct_volume_360(sino_np, cor, chunk=16, angles=angles,
+                                  precision=args.precision, out=out_np)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    line["detail"]["host_resident_ms"] = host_ms
+    line["detail"]["host_resident_gups"] = updates / (host_ms * 1e-3) / 1e9
     print(json.dumps(line))
 
 
@@ -492,6 +506,12 @@ def run_dfi(args, dev):
     detail = {"ms_per_slice": ms / nsl, "single_sinogram_ms": ms1,
               "single_sinogram_1801x4096_ms": msw,
               "finite": bool(torch.isfinite(out).all().item())}
+    # host numpy in and out (float64 result): the chunk pipeline
+    sino_np = sino.cpu().numpy()
+    rec.dfi_reconstruction(sino_np, CENTER)
+    t0 = time.perf_counter()
+    rec.dfi_reconstruction(sino_np, CENTER)
+    detail["host_numpy_ms_per_slice"] = (time.perf_counter() - t0) * 1e3 / nsl
     if not args.no_cpu_baseline:
         from oracle import dfi_np
         s_np = one.cpu().numpy()
