@@ -12,6 +12,7 @@
 // kernel writes the stitched rows; every output sample is one or two
 // multiply-adds of input samples, so the pass is HBM-bound.
 #include "ath_internal.h"
+#include "log32.h"
 
 namespace ath {
 namespace {
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void extend_kernel(
             // source column of the padded array (edge padding)
             const int c = side == 1 ? min(k, ncol - 1) : max(k - pad, 0);
             float v = src[c];
-            if (apply_log) v = (float)(-log((double)v));
+            if (apply_log) v = neg_log_f32(v);
             if (side == 1 && c >= ncol - overlap)
                 v = (float)((double)v *
                             ramp_down(c - (ncol - overlap), overlap));
@@ -178,21 +179,51 @@ __global__ __launch_bounds__(kThreads) void flat_field_kernel(
 }
 
 // out = -log(in) elementwise, float32 <- float64 log rounded (the filter
-// kernel's convention for the same step)
+// kernel's convention for the same step).  VEC = 4: float4 loads and stores
+// over the first n - n % 4 elements (16-byte aligned pointers), the rest one
+// by one.
+template <int VEC>
 __global__ __launch_bounds__(kThreads) void minus_log_kernel(
     const float* __restrict__ in, float* __restrict__ out, long long n) {
-    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < n;
-         e += (long long)gridDim.x * kThreads)
-        out[e] = (float)(-log((double)in[e]));
+    __shared__ Log32Table s_tab;
+    stage_log32_table<kThreads>(&s_tab);
+    __syncthreads();
+    const long long nv = n / VEC;
+    const long long stride = (long long)gridDim.x * kThreads;
+    const long long e0 = (long long)blockIdx.x * kThreads + threadIdx.x;
+    for (long long e = e0; e < nv; e += stride) {
+        if constexpr (VEC == 4) {
+            float4 x = reinterpret_cast<const float4*>(in)[e];
+            x.x = neg_log_f32(x.x, s_tab);
+            x.y = neg_log_f32(x.y, s_tab);
+            x.z = neg_log_f32(x.z, s_tab);
+            x.w = neg_log_f32(x.w, s_tab);
+            reinterpret_cast<float4*>(out)[e] = x;
+        } else {
+            out[e] = neg_log_f32(in[e], s_tab);
+        }
+    }
+    const long long t = nv * VEC + e0;  // tail (VEC = 4 only)
+    if (t < n) out[t] = neg_log_f32(in[t], s_tab);
 }
 
 }  // namespace
 
 int minus_log(const float* in, float* out, long long n, hipStream_t s) {
-    const long long blocks =
-        std::min<long long>((n + kThreads - 1) / kThreads, 1 << 20);
-    hipLaunchKernelGGL(minus_log_kernel, dim3((unsigned)blocks),
-                       dim3(kThreads), 0, s, in, out, n);
+    if (n <= 0) return ATH_OK;
+    const bool vec = reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(out) % 16 == 0;
+    const long long items = vec ? n / 4 : n;
+    // a few workgroups per CU, each striding over many elements (every
+    // workgroup stages the 2 KB log table once)
+    const long long blocks = std::max<long long>(
+        1, std::min<long long>((items + kThreads - 1) / kThreads, 8192));
+    if (vec)
+        hipLaunchKernelGGL(minus_log_kernel<4>, dim3((unsigned)blocks),
+                           dim3(kThreads), 0, s, in, out, n);
+    else
+        hipLaunchKernelGGL(minus_log_kernel<1>, dim3((unsigned)blocks),
+                           dim3(kThreads), 0, s, in, out, n);
     return check_hip(hipGetLastError(), "minus_log_kernel launch");
 }
 

@@ -44,6 +44,7 @@
 // All arithmetic after the shift is float64, as numpy's complex128 FFTs.
 #include "ath_internal.h"
 #include "fft_lds.h"
+#include "log32.h"
 
 namespace ath {
 namespace {
@@ -332,6 +333,8 @@ __device__ __forceinline__ double minus_log(double v, int* flag) {
         v = 1.0;
         *flag = 1;
     }
+    if constexpr (sizeof(Tr) == 4)
+        return (double)neg_log_f32((float)v);  // v is a float32 value
     return round_to<Tr>(-log(v));
 }
 

@@ -14,6 +14,7 @@
 // FFT split and the inverse merge fused in one pass) and runs the inverse.
 // The pass is HBM-bound: one read and one write of every sample.
 #include "fft_lds.h"
+#include "log32.h"
 
 namespace ath {
 namespace {
@@ -38,7 +39,7 @@ __device__ __forceinline__ double load_sample<float>(const float* row, int src,
                                                      int apply_log) {
     const float x = row[src];
     // -np.log of a float32 array is float32 (reconstruction.py:436)
-    return apply_log ? (double)(float)(-log((double)x)) : (double)x;
+    return apply_log ? (double)neg_log_f32(x) : (double)x;
 }
 template <>
 __device__ __forceinline__ double load_sample<double>(const double* row,

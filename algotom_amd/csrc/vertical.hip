@@ -23,6 +23,7 @@
 // rest in float64; outside [0, width-1] the edge value is added only with
 // edge_pad.
 #include "ath_internal.h"
+#include "log32.h"
 
 namespace ath {
 namespace {
@@ -121,30 +122,32 @@ __global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a,
                 fpos = f;
                 frac = f - (double)d;
             }
-            const float* pn = a.proj + (size_t)n * plane;
+            // Both taps of every row are loaded unconditionally, so the
+            // 2 * kRows gathers of a projection are in flight together.
+            // u == d gives yu == yd and val = yd exactly; outside the range
+            // both indices clamp to the edge column, which is the edge_pad
+            // value.  Every index lies in [0, width - 1].
+            if (!in_range) d = u = (fpos < 0.0) ? 0 : width1;
+            const bool use = in_range || a.edge_pad;
+            const float* pn = a.proj + (size_t)n * plane + (size_t)i0 * a.width;
+            float yd[kRows], yu[kRows];
 #pragma unroll
             for (int k = 0; k < kRows; ++k) {
-                const int i = i0 + 4 * k;
-                if (!valid_j || i >= a.height) continue;
-                const float* row = pn + (size_t)i * a.width;
-                double val;
-                if (in_range) {
-                    const float yd = row[d];
-                    if (u != d) {
-                        const float yu = row[u];
-                        val = (double)yd + (double)(yu - yd) * frac;
-                    } else {
-                        val = (double)yd;
-                    }
-                } else if (a.edge_pad) {
-                    val = (double)(fpos < 0.0 ? row[0] : row[width1]);
-                } else {
-                    continue;
+                const bool ok = valid_j && i0 + 4 * k < a.height;
+                const float* row = pn + (size_t)(4 * k) * a.width;
+                yd[k] = ok ? row[d] : 0.0f;
+                yu[k] = ok ? row[u] : 0.0f;
+            }
+            if (use) {
+#pragma unroll
+                for (int k = 0; k < kRows; ++k) {
+                    const double val =
+                        (double)yd[k] + (double)(yu[k] - yd[k]) * frac;
+                    if (MODE == kVertGpu)
+                        chunk64[k] += val;
+                    else
+                        chunk32[k] = (float)((double)chunk32[k] + val);
                 }
-                if (MODE == kVertGpu)
-                    chunk64[k] += val;
-                else
-                    chunk32[k] = (float)((double)chunk32[k] + val);
             }
             if ((n + 1) % a.chunk == 0 || n + 1 == a.nproj) {
 #pragma unroll
@@ -172,59 +175,95 @@ __global__ __launch_bounds__(kThreads) void vertical_bp_kernel(VertArgs a,
 
 // (img - dark) / flat_dark in float32 (the caller has replaced zeros of
 // flat_dark by 1), then -log with values <= 0 set to 1 and flagged.
-__device__ __forceinline__ float flat_log_value(float v, const float* dk,
-                                                const float* fd, int j,
-                                                int apply_log, int* flag) {
-    if (dk) v = (v - dk[j]) / fd[j];
+__device__ __forceinline__ float flat_log_value(float v, bool fd, float dk,
+                                                float fdk, int apply_log,
+                                                int* flag,
+                                                const Log32Table& tab) {
+    if (fd) v = (v - dk) / fdk;
     if (apply_log) {
         if (v <= 0.0f) {
             v = 1.0f;
             *flag = 1;
         }
-        v = (float)(-log((double)v));
+        v = neg_log_f32(v, tab);
     }
     return v;
 }
 
 template <typename Tin, int VEC> struct VecOf;
+template <> struct VecOf<float, 1> { using type = float; };
+template <> struct VecOf<unsigned short, 1> { using type = unsigned short; };
 template <> struct VecOf<float, 4> { using type = float4; };
 template <> struct VecOf<unsigned short, 4> { using type = ushort4; };
 
+template <typename V>
+__device__ __forceinline__ void unpack(const V& x, float (&v)[4]) {
+    v[0] = (float)x.x;
+    v[1] = (float)x.y;
+    v[2] = (float)x.z;
+    v[3] = (float)x.w;
+}
+__device__ __forceinline__ void unpack(float x, float (&v)[1]) { v[0] = x; }
+__device__ __forceinline__ void unpack(unsigned short x, float (&v)[1]) {
+    v[0] = (float)x;
+}
+
+// Projections per pass over one detector row: the dark and flat-dark values
+// of a column group are loaded once for all of them.
+constexpr int kProjRun = 8;
+
 // VEC = 4: groups of four columns with 16-byte (8 for uint16) loads and
-// stores; the caller checks widths, strides and alignment.
+// stores; the caller checks widths, strides and alignment.  Work items are
+// (run of kProjRun projections, detector row) pairs.
 template <typename Tin, int VEC>
 __global__ __launch_bounds__(kThreads) void flat_field_log_kernel(
     const Tin* __restrict__ proj, long long nproj, int height, int width,
     long long proj_stride, long long row_stride,
     const float* __restrict__ dark, const float* __restrict__ flat_dark,
     int apply_log, float* __restrict__ out, int* __restrict__ flag) {
-    const long long rows = nproj * height;
+    __shared__ Log32Table s_tab;
+    stage_log32_table<kThreads>(&s_tab);
+    __syncthreads();
+    using V = typename VecOf<Tin, VEC>::type;
+    using FV = typename VecOf<float, VEC>::type;
     const int groups = width / VEC;
-    for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
-        const long long n = r / height;  // block-uniform
-        const int iy = (int)(r - n * height);
-        const Tin* src = proj + n * proj_stride + iy * row_stride;
-        float* dst = out + r * width;
-        const float* dk = dark ? dark + (size_t)iy * width : nullptr;
-        const float* fd = dark ? flat_dark + (size_t)iy * width : nullptr;
+    const long long items = (nproj + kProjRun - 1) / kProjRun * height;
+    const bool fd = dark != nullptr;
+    for (long long q = blockIdx.y; q < items; q += gridDim.y) {
+        const long long run = q / height;  // block-uniform
+        const int iy = (int)(q - run * height);
+        const long long n0 = run * kProjRun;
+        const int np = (int)min((long long)kProjRun, nproj - n0);
         for (int g = blockIdx.x * kThreads + threadIdx.x; g < groups;
              g += gridDim.x * kThreads) {
-            if constexpr (VEC == 1) {
-                dst[g] = flat_log_value((float)src[g], dk, fd, g, apply_log,
-                                        flag);
+            float dk[VEC], fk[VEC];
+            if (fd) {
+                unpack(reinterpret_cast<const FV*>(dark + (size_t)iy * width)[g],
+                       dk);
+                unpack(reinterpret_cast<const FV*>(flat_dark +
+                                                   (size_t)iy * width)[g],
+                       fk);
             } else {
-                using V = typename VecOf<Tin, 4>::type;
-                const V x = reinterpret_cast<const V*>(src)[g];
-                const int j = 4 * g;
-                float4 y;
-                y.x = flat_log_value((float)x.x, dk, fd, j, apply_log, flag);
-                y.y = flat_log_value((float)x.y, dk, fd, j + 1, apply_log,
-                                     flag);
-                y.z = flat_log_value((float)x.z, dk, fd, j + 2, apply_log,
-                                     flag);
-                y.w = flat_log_value((float)x.w, dk, fd, j + 3, apply_log,
-                                     flag);
-                reinterpret_cast<float4*>(dst)[g] = y;
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) dk[k] = fk[k] = 0.0f;
+            }
+            for (int p = 0; p < np; ++p) {
+                const long long n = n0 + p;
+                const V x = reinterpret_cast<const V*>(
+                    proj + n * proj_stride + iy * row_stride)[g];
+                float v[VEC];
+                unpack(x, v);
+#pragma unroll
+                for (int k = 0; k < VEC; ++k)
+                    v[k] = flat_log_value(v[k], fd, dk[k], fk[k], apply_log,
+                                          flag, s_tab);
+                FV y;
+                if constexpr (VEC == 1) {
+                    y = v[0];
+                } else {
+                    y = make_float4(v[0], v[1], v[2], v[3]);
+                }
+                reinterpret_cast<FV*>(out + (n * height + iy) * width)[g] = y;
             }
         }
     }
@@ -288,8 +327,9 @@ static void launch_flat_log(const void* proj, int nproj, int height,
                             long long row_stride, const float* dark,
                             const float* flat_dark, int apply_log, float* out,
                             int* flag, hipStream_t s) {
-    const dim3 grid =
-        row_grid((long long)nproj * height, width / VEC, kThreads);
+    const dim3 grid = row_grid(
+        (long long)(nproj + kProjRun - 1) / kProjRun * height, width / VEC,
+        kThreads);
     hipLaunchKernelGGL((flat_field_log_kernel<Tin, VEC>), grid,
                        dim3(kThreads), 0, s, static_cast<const Tin*>(proj),
                        (long long)nproj, height, width, proj_stride,
