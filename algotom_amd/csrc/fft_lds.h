@@ -33,13 +33,23 @@ __device__ __forceinline__ double2 rot(double2 a) {
     return DIR < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
+// Quarter twiddle table layout: entry m sits at swz_tw(m), which moves only
+// the low four bits of m by a function of the higher bits (a permutation of
+// every aligned block of 16 entries).  The pass at span ns reads entries
+// k * M / (4 ns), 64 .. 4096 bytes apart for the spans 4 .. 256, which in
+// natural order fall on the same banks (up to 16-way conflicts).
+__device__ __forceinline__ int swz_tw(int t) {
+    return t ^ (((t >> 4) ^ (t >> 8)) & 15);
+}
+
 // Quarter twiddle table for a length-M transform: tq[m] = exp(-2 pi i m/M)
-// for m < M/4, taken from the plan's length-2M table (tw[2m]).  All threads.
+// for m < M/4, taken from the plan's length-2M table (tw[2m]), stored at
+// swz_tw(m).  All threads.
 template <int NT = kFftThreads>
 __device__ __forceinline__ void stage_quarter(double2* tq,
                                               const double2* __restrict__ tw,
                                               int M) {
-    for (int m = threadIdx.x; m < (M >> 2); m += NT) tq[m] = tw[2 * m];
+    for (int m = threadIdx.x; m < (M >> 2); m += NT) tq[swz_tw(m)] = tw[2 * m];
 }
 
 // exp(-2 pi i j/M) for j < M/2 from the quarter table: the second quarter
@@ -47,8 +57,8 @@ __device__ __forceinline__ void stage_quarter(double2* tq,
 __device__ __forceinline__ double2 twiddle_half(const double2* tq, int j,
                                                 int M) {
     const int q = M >> 2;
-    if (j < q) return tq[j];
-    const double2 t = tq[j - q];
+    if (j < q) return tq[swz_tw(j)];
+    const double2 t = tq[swz_tw(j - q)];
     return make_double2(t.y, -t.x);
 }
 
@@ -78,7 +88,8 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                 for (int r = 0; r < 4; ++r) v[b][r] = buf[j + r * nb];
                 const int k = j & (ns - 1);
                 if (k != 0) {
-                    double2 w1 = QT ? tq[k * (tstep >> 1)] : tw[k * tstep];
+                    double2 w1 = QT ? tq[swz_tw(k * (tstep >> 1))]
+                                    : tw[k * tstep];
                     if (DIR > 0) w1 = conjg(w1);
                     const double2 w2 = cmul(w1, w1);
                     const double2 w3 = cmul(w1, w2);
