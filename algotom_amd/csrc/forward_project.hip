@@ -157,12 +157,15 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
                         u0 = u1;
                         u1 = t;
                     }
-                    // contributors have f in [d-1, d+1), i.e. u in
+                    // contributors have f in [d-1, d+1), i.e. k - ic in
                     // [u0, u1]; u carries a relative error of a few ulp
                     // (reciprocal slope), so a 1e-6 px margin keeps the
-                    // candidate list a superset (each is tested exactly)
-                    k0 = max((int)floor(u0 + ic - 1e-6), 0);
-                    k1 = min((int)ceil(u1 + ic + 1e-6), C - 1);
+                    // candidate list a superset (each is tested exactly).
+                    // Every candidate costs the whole slice loop once any
+                    // lane of the wave contributes, so the list is kept
+                    // to the integers inside the margin.
+                    k0 = max((int)ceil(u0 + ic - 1e-6), 0);
+                    k1 = min((int)floor(u1 + ic + 1e-6), C - 1);
                 }
                 if (staged) {
                     k0 = max(k0, ka);

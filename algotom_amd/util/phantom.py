@@ -44,6 +44,24 @@ def ellipse_image(params, size):
     return img.astype(np.float32)
 
 
+def ellipse_image_torch(params, size, device="cpu"):
+    """ellipse_image as a torch float32 tensor computed on ``device`` (the
+    same float64 expressions, so the same mask)."""
+    import torch
+    ic = 0.5 * (size - 1)
+    xs = torch.arange(size, dtype=torch.float64, device=device) - ic
+    X = xs[None, :]
+    Y = -xs[:, None]
+    img = torch.zeros((size, size), dtype=torch.float64, device=device)
+    for x0, y0, a, b, phi, rho in np.asarray(params):
+        p = np.deg2rad(phi)
+        u = (X - x0) * np.cos(p) + (Y - y0) * np.sin(p)
+        v = -(X - x0) * np.sin(p) + (Y - y0) * np.cos(p)
+        inside = ((u / a) ** 2 + (v / b) ** 2 <= 1.0).to(torch.float64)
+        img += inside * float(rho)
+    return img.to(torch.float32)
+
+
 def ellipse_sinogram(params, angles, ncol, center, device="cpu"):
     """Exact line integrals, (len(angles), ncol) torch float64 on device.
 
