@@ -245,3 +245,69 @@ def _stream_chunks(src, out, ranges, compute, in_dtype, device):
         down.synchronize()
         main.synchronize()
     return out
+
+
+_prefetch = {}                   # per device: copy stream and stager
+_prefetch_lock = threading.Lock()
+
+
+def prefetch_chunks(src, ranges, dtype, device):
+    """Yields (k, tensor) for k, (r0, r1) in enumerate(ranges): a contiguous
+    device tensor of ``dtype`` holding src[r0:r1] (numpy, any strides, a
+    memory map included).  Chunk k+1 uploads on a copy stream while the
+    caller works on chunk k; the caller's work must be enqueued on the
+    current stream before it asks for the next chunk, and a yielded tensor
+    is valid until then."""
+    import torch
+    if not ranges:
+        return
+    with _prefetch_lock:
+        main = torch.cuda.current_stream(device)
+        key = torch.device(device).index
+        if key not in _prefetch:
+            _prefetch[key] = (torch.cuda.Stream(device), Stager())
+        up, stage = _prefetch[key]
+        shape1 = tuple(src.shape[1:])
+        row = int(np.prod(shape1, dtype=np.int64))
+        most = max(r1 - r0 for r0, r1 in ranges)
+        bufs = [torch.empty(most * row, dtype=dtype, device=device)
+                for _ in range(2)]
+        done = [None, None]
+
+        def view(k):
+            r0, r1 = ranges[k]
+            return bufs[k % 2][:(r1 - r0) * row].view((r1 - r0,) + shape1)
+
+        def upload(k, wait):
+            r0, r1 = ranges[k]
+            if wait is not None:
+                up.wait_event(wait)        # chunk k-2 is finished with it
+            stage.upload(src[r0:r1], view(k), up)
+            ev = torch.cuda.Event()
+            ev.record(up)
+            return ev
+
+        pool = ThreadPoolExecutor(max_workers=1)
+        fut = None
+        try:
+            fut = pool.submit(upload, 0, None)
+            for k in range(len(ranges)):
+                ev = fut.result()
+                fut = None
+                if k + 1 < len(ranges):
+                    # its buffer was chunk k-1's, finished at done[...]
+                    fut = pool.submit(upload, k + 1, done[(k + 1) % 2])
+                main.wait_event(ev)
+                yield k, view(k)
+                done_ev = torch.cuda.Event()
+                done_ev.record(main)
+                done[k % 2] = done_ev
+        finally:
+            if fut is not None:
+                try:
+                    fut.result()
+                except Exception:
+                    pass
+            pool.shutdown(wait=True)
+            up.synchronize()
+            main.synchronize()

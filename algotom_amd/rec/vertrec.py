@@ -32,6 +32,7 @@ import warnings
 
 import numpy as np
 
+from algotom_amd import _host
 from algotom_amd import _native as nat
 from algotom_amd.rec import reconstruction as rec
 
@@ -267,6 +268,31 @@ def _upload(projections, n0, n1, box, dev):
     return torch.from_numpy(a).to(dev)
 
 
+def _groups(projections, proj_start, ranges, box, dev):
+    """(g0, g1, cropped projections [g0, g1) on the device) per group.  A
+    numpy array (a memory map included) streams through the prefetch
+    pipeline: group k+1 uploads on a copy stream while group k is
+    processed; other inputs go group by group through _upload."""
+    import torch
+    if isinstance(projections, np.ndarray):
+        top, bot, left, right = box
+        n0, n1 = proj_start + ranges[0][0], proj_start + ranges[-1][1]
+        src = projections[n0:n1, top:bot, left:right]
+        if src.dtype == np.uint16:
+            src, dtype = src.view(np.int16), torch.int16
+        else:
+            dtype = torch.float32
+        shifted = [(g0 - ranges[0][0], g1 - ranges[0][0]) for g0, g1 in ranges]
+        for k, t in _host.prefetch_chunks(src, shifted, dtype, dev):
+            g0, g1 = ranges[k]
+            yield g0, g1, (t.view(torch.uint16) if dtype == torch.int16
+                           else t)
+        return
+    for g0, g1 in ranges:
+        yield g0, g1, _upload(projections, proj_start + g0,
+                              proj_start + g1, box, dev)
+
+
 def _prepare(raw, fd, apply_log, flag, dev):
     """Flat-field + log into a contiguous float32 (n, height, width)."""
     import torch
@@ -334,10 +360,9 @@ def _vertical_pipeline(projections, xs, ys, single, center, flat_field,
     per_proj = height * width * 4 * (2 if ramp_filter == "before" else 1)
     group = max(1, _GROUP_BYTES // max(per_proj * chunk, 1)) * chunk
     t0 = time.time()
-    for g0 in range(0, num_proj, group):
-        g1 = min(num_proj, g0 + group)
-        raw = _upload(projections, proj_start + g0, proj_start + g1, box,
-                      dev)
+    ranges = [(g0, min(num_proj, g0 + group))
+              for g0 in range(0, num_proj, group)]
+    for g0, g1, raw in _groups(projections, proj_start, ranges, box, dev):
         img = _prepare(raw, fd, apply_log, flag, dev)
         if ramp_filter == "before":
             img = rec._filter_stack(img, pad, pad_mode, False, taps,

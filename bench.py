@@ -688,6 +688,19 @@ def run_vertrec(args, dev):
                    "multiple_16_before_filter_ms": ms16,
                    "finite": bool(torch.isfinite(out).all().item())},
     }
+    # the same 16-line call on the projections as a host numpy array
+    # (18.9 GB): groups upload through pinned buffers on a copy stream
+    # while the previous group is processed
+    del out
+    proj_np = proj.cpu().numpy()
+    del proj
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    vertrec.vertical_reconstruction_multiple(
+        proj_np, 1000, 1000 + 5 * (nsl - 1), cen, alpha=30.0, step_index=5,
+        apply_log=True, ramp_filter="before", show_progress=False)
+    line["detail"]["multiple_16_host_numpy_ms"] = \
+        (time.perf_counter() - t0) * 1e3
     print(json.dumps(line))
 
 

@@ -120,3 +120,25 @@ def test_stager_blocks(rows, stage):
     st.download(dev * 2, dst, stream)
     assert np.array_equal(back[:, 1], 2 * src.astype(np.float32))
     assert not back[:, 0].any()
+
+
+@pytest.mark.parametrize("dtype", ["float32", "uint16"])
+def test_vertrec_host_prefetch_equals_device(monkeypatch, dtype):
+    """vertical_reconstruction_multiple on a host array (prefetched groups,
+    several per call) equals the call on the same projections on the
+    device."""
+    import torch
+    from algotom_amd.rec import vertrec
+    monkeypatch.setattr(vertrec, "_GROUP_BYTES", 200_000)  # 4 groups
+    rng = np.random.default_rng(7)
+    proj = (rng.random((97, 12, 80)) * 3000 + 100).astype(dtype)
+    proj_view = proj[:, 1:11, :]                  # strided host view
+    kw = dict(alpha=20.0, step_index=3, apply_log=True,
+              ramp_filter="before", show_progress=False, chunk_size=10)
+    got = vertrec.vertical_reconstruction_multiple(proj_view, 30, 36, 39.2,
+                                                   **kw)
+    dev = torch.from_numpy(np.ascontiguousarray(proj_view).astype(
+        np.float32)).cuda()
+    want = vertrec.vertical_reconstruction_multiple(dev, 30, 36, 39.2, **kw)
+    want = want.cpu().numpy() if torch.is_tensor(want) else want
+    assert np.array_equal(np.asarray(got), np.asarray(want))
