@@ -42,7 +42,7 @@ constexpr double kMinSlope = 1.0 / 1024.0;
 template <int FSB, int kRows>
 __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     FpArgs a, int by_rows_block) {
-    __shared__ float tile[kRows * kMaxMinor * FSB];
+    __shared__ __align__(16) float tile[kRows * kMaxMinor * FSB];
     __shared__ double s_cs[4][2];
     __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
     __shared__ double s_u[16];    // window ends: (angle, line end, bin end)
@@ -110,35 +110,40 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
         if (wid > 0 && staged) {
+            // one pixel per thread and step: the mask test once, then its
+            // FSB slices (kMaxMinor = 128 = 2^7 columns per line)
             const int nrows = m1 - m0 + 1;
-            auto stage = [&](int r, int js, int k) {
+            for (int e = tid; e < kRows * kMaxMinor; e += 256) {
+                int r, k;
+                if (by_rows) {  // lanes along x within a line
+                    r = e >> 7;
+                    k = e & (kMaxMinor - 1);
+                } else {        // lanes along the lines (x) first
+                    r = e % kRows;
+                    k = e / kRows;
+                }
+                if (r >= nrows || k >= wid) continue;
                 const int x = by_rows ? ka + k : m0 + r;
                 const int y = by_rows ? m0 + r : ka + k;
-                float v = 0.0f;
-                if (js < ns) {
-                    v = a.image[((size_t)y * S + j0 + js) * C + x];
-                    if (a.use_mask) {
-                        const long long dx = x - a.mask_c, dy = y - a.mask_c;
-                        if (!((double)(dx * dx + dy * dy) <= a.mask_r2))
-                            v = 0.0f;
-                    }
+                bool keep = true;
+                if (a.use_mask) {
+                    const long long dx = x - a.mask_c, dy = y - a.mask_c;
+                    keep = (double)(dx * dx + dy * dy) <= a.mask_r2;
                 }
-                tile[(r * kMaxMinor + k) * FSB + js] = v;
-            };
-            if (by_rows) {
-                // lanes along x (contiguous in the image)
-                for (int rj = wv; rj < nrows * FSB; rj += 4) {
-                    const int r = rj / FSB, js = rj % FSB;
-                    for (int k = lane; k < wid; k += 64) stage(r, js, k);
-                }
-            } else {
-                // lanes along the line index (x contiguous); kRows and FSB
-                // are compile-time, so the index split is shifts
-                for (int e = tid; e < kRows * FSB * wid; e += 256) {
-                    const int r = e % kRows;
-                    const int t = e / kRows;
-                    const int js = t % FSB, k = t / FSB;
-                    if (r < nrows) stage(r, js, k);
+                const float* src = a.image + ((size_t)y * S + j0) * C + x;
+                float v[FSB];
+#pragma unroll
+                for (int js = 0; js < FSB; ++js)
+                    v[js] = (keep && js < ns) ? src[(size_t)js * C] : 0.0f;
+                float* dst = tile + (r * kMaxMinor + k) * FSB;
+                if constexpr (FSB % 4 == 0) {
+#pragma unroll
+                    for (int q = 0; q < FSB / 4; ++q)
+                        reinterpret_cast<float4*>(dst)[q] = make_float4(
+                            v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+                } else {
+#pragma unroll
+                    for (int js = 0; js < FSB; ++js) dst[js] = v[js];
                 }
             }
         }
@@ -227,7 +232,7 @@ int forward_projection(const FpArgs& a, hipStream_t stream) {
     // host copy of the walking direction per block needs the angles: they
     // live on the device, so the kernel decides per block instead
     // slices per workgroup and lines per chunk measured on MI355X
-    // (16 x 901 x 1024: 16/4 15.3 ms, 8/4 16.0 ms, 8/8 20.9 ms)
+    // (16 x 901 x 1024: 16/4 9.9 ms, 8/4 11.2 ms, 16/8 12.7 ms)
     const int fsb = a.nslice >= 16 ? 16
                   : a.nslice >= 8 ? 8 : (a.nslice >= 4 ? 4 : 1);
     const dim3 grid((a.ncol + 63) / 64, (a.nrow + 3) / 4,
