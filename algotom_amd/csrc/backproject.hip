@@ -651,6 +651,13 @@ void default_config(int nslice, int& sb, int& py) {
     sb = nslice <= 2 ? nslice : 4;
     py = 8;
     if (nslice > 2 && MODE == ATH_ACC_F64) py = 4;
+    // the centre derivative reads one value per record: 2 slices x 16 rows
+    // (16 x 901 x 1024: 3.47 ms against 4.84 ms for 4 x 32; 16 x 1801 x
+    // 7192: 308 against 429 ms)
+    if (MODE == ATH_ACC_DERIV && nslice >= 2) {
+        sb = 2;
+        py = 4;
+    }
 }
 
 template <int MODE, typename Tin>
@@ -667,11 +674,7 @@ int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb,
         if (sino_dtype != ATH_F32)
             return set_error(ATH_ERR_UNSUPPORTED,
                              "centre derivative needs a float32 sinogram");
-        constexpr int A = 16;
-        const int n = a.nslice;
-        if (n == 1) return launch_tiled<ATH_ACC_DERIV, 1, 8, A, float>(a, s);
-        if (n == 2) return launch_tiled<ATH_ACC_DERIV, 2, 8, A, float>(a, s);
-        return launch_tiled<ATH_ACC_DERIV, 4, 8, A, float>(a, s);
+        return dispatch<ATH_ACC_DERIV, float>(a, sb, py, s);
     }
     if (sino_dtype == ATH_F32) {
         switch (acc) {
