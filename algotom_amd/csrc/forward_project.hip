@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     __shared__ __align__(16) float tile[kRows * kMaxMinor * FSB];
     __shared__ double s_cs[4][2];
     __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
-    __shared__ double s_u[16];    // window ends: (angle, line end, bin end)
+    __shared__ int s_ab[2];       // the chunk's staged column range
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
     const int d0 = blockIdx.x * 64;
@@ -85,28 +85,39 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
         const int m1 = min(m0 + kRows, C) - 1;
         // union window of the minor coordinate over 4 angles, bins
         // [d0 - 1, d0 + 64] and lines m0, m1 (windows are linear in m):
-        // 16 threads each evaluate one end, every thread takes min / max
-        if (tid < 16) {
-            const int q = tid >> 2, e = (tid >> 1) & 1, b = tid & 1;
-            const double mc = (double)(e ? m1 : m0) - ic;
-            const double base =
-                (by_rows ? mc * s_cs[q][1] : mc * s_cs[q][0]) + cen;
-            const double bin = b ? (double)d0 + 65.0 : (double)d0 - 2.0;
-            // a near-parallel line (inverse 0) takes the whole line
-            s_u[tid] = s_inv[q] != 0.0 ? (bin - base) * s_inv[q]
-                                       : (b ? 1e300 : -1e300);
+        // lanes 0-15 of wave 0 each evaluate one end, reduce min / max
+        // across those 16 lanes and lane 0 publishes the column range
+        if (tid < 64) {
+            double lo = 1e300, hi = -1e300;
+            if (tid < 16) {
+                const int q = tid >> 2, e = (tid >> 1) & 1, b = tid & 1;
+                const double mc = (double)(e ? m1 : m0) - ic;
+                const double base =
+                    (by_rows ? mc * s_cs[q][1] : mc * s_cs[q][0]) + cen;
+                const double bin = b ? (double)d0 + 65.0 : (double)d0 - 2.0;
+                // a near-parallel line (inverse 0) takes the whole line
+                const double u = s_inv[q] != 0.0 ? (bin - base) * s_inv[q]
+                                                 : (b ? 1e300 : -1e300);
+                lo = u;
+                hi = u;
+            }
+            // lanes 0-15 are closed under these xor offsets
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) {
+                lo = fmin(lo, __shfl_xor(lo, off));
+                hi = fmax(hi, __shfl_xor(hi, off));
+            }
+            // a negative inverse swaps the two bin ends: min / max over
+            // all sixteen covers both orders
+            if (tid == 0) {
+                s_ab[0] = lo <= -1e299 ? 0
+                                       : max((int)floor(lo + ic) - 1, 0);
+                s_ab[1] = hi >= 1e299 ? C - 1
+                                      : min((int)ceil(hi + ic) + 1, C - 1);
+            }
         }
         __syncthreads();
-        double lo = 1e300, hi = -1e300;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            lo = fmin(lo, s_u[t]);
-            hi = fmax(hi, s_u[t]);
-        }
-        // a negative inverse swaps the two bin ends: min / max over all
-        // sixteen covers both orders
-        const int ka = lo <= -1e299 ? 0 : max((int)floor(lo + ic) - 1, 0);
-        const int kb = hi >= 1e299 ? C - 1 : min((int)ceil(hi + ic) + 1, C - 1);
+        const int ka = s_ab[0], kb = s_ab[1];
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
         if (wid > 0 && staged) {
