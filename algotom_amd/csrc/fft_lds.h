@@ -62,9 +62,49 @@ __device__ __forceinline__ double2 twiddle_half(const double2* tq, int j,
     return make_double2(t.y, -t.x);
 }
 
+// 8-point DFT in place, natural order in and out (two 4-point DFTs of the
+// even and odd inputs, then W8^k and the radix-2 combination).
+template <int DIR>
+__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2,
+                                     double2& x3) {
+    const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const double2 a2 = cadd(x1, x3), a3 = rot<DIR>(csub(x1, x3));
+    x0 = cadd(a0, a2);
+    x1 = cadd(a1, a3);
+    x2 = csub(a0, a2);
+    x3 = csub(a1, a3);
+}
+template <int DIR>
+__device__ __forceinline__ void dft8(double2 (&v)[8]) {
+    constexpr double h = 0.70710678118654752440;  // sqrt(2)/2
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    // o_k *= W8^k (forward W8 = exp(-i pi/4); the inverse conjugates)
+    if (DIR < 0) {
+        o1 = make_double2((o1.x + o1.y) * h, (o1.y - o1.x) * h);
+        o3 = make_double2((o3.y - o3.x) * h, -(o3.x + o3.y) * h);
+    } else {
+        o1 = make_double2((o1.x - o1.y) * h, (o1.x + o1.y) * h);
+        o3 = make_double2(-(o3.x + o3.y) * h, (o3.x - o3.y) * h);
+    }
+    o2 = rot<DIR>(o2);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
 // In-LDS complex FFT of length M = 2^m, natural order in and out (Stockham
 // autosort: each pass reads all operands into registers, barriers, then
-// writes them to their sorted place in the same buffer).  DIR = -1 is the
+// writes them to their sorted place in the same buffer).  Radix-8 passes
+// (4 for M = 4096 instead of 6 radix-4 ones: fewer barriers and LDS round
+// trips) and at most one radix-4 and one radix-2 pass.  DIR = -1 is the
 // forward transform, +1 the unscaled inverse.  tw[n] = exp(-2 pi i n / L),
 // L = 2M.  Each of the NT threads handles up to MAXB butterflies a pass.
 // A radix-4 butterfly fetches one twiddle w and forms w^2, w^3 by complex
@@ -76,6 +116,52 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
     const int tid = threadIdx.x;
     const int L = 2 * M;
     int ns = 1;
+    // radix-8 passes first (M/8 butterflies: B8 per thread), then at most one
+    // radix-4 and one radix-2 pass for the remaining factor
+    constexpr int B8 = (MAXB + 1) / 2;
+    while (ns * 8 <= M) {
+        const int nb = M >> 3;
+        const int tstep = L / (ns * 8);
+        double2 v[B8][8];
+#pragma unroll
+        for (int b = 0; b < B8; ++b) {
+            const int j = tid + b * NT;
+            if (j < nb) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v[b][r] = buf[j + r * nb];
+                const int k = j & (ns - 1);
+                if (k != 0) {
+                    double2 w1 = QT ? tq[swz_tw(k * (tstep >> 1))]
+                                    : tw[k * tstep];
+                    if (DIR > 0) w1 = conjg(w1);
+                    const double2 w2 = cmul(w1, w1);
+                    const double2 w3 = cmul(w2, w1);
+                    const double2 w4 = cmul(w2, w2);
+                    v[b][1] = cmul(v[b][1], w1);
+                    v[b][2] = cmul(v[b][2], w2);
+                    v[b][3] = cmul(v[b][3], w3);
+                    v[b][4] = cmul(v[b][4], w4);
+                    v[b][5] = cmul(v[b][5], cmul(w4, w1));
+                    v[b][6] = cmul(v[b][6], cmul(w3, w3));
+                    v[b][7] = cmul(v[b][7], cmul(w4, w3));
+                }
+                dft8<DIR>(v[b]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B8; ++b) {
+            const int j = tid + b * NT;
+            if (j < nb) {
+                const int k = j & (ns - 1);
+                const int base = (j - k) * 8 + k;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) buf[base + r * ns] = v[b][r];
+            }
+        }
+        __syncthreads();
+        ns *= 8;
+    }
     while (ns * 4 <= M) {
         const int nb = M >> 2;
         const int tstep = L / (ns * 4);
