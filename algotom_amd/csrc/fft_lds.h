@@ -119,6 +119,12 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
     // radix-8 passes first (M/8 butterflies: B8 per thread), then at most one
     // radix-4 and one radix-2 pass for the remaining factor
     constexpr int B8 = (MAXB + 1) / 2;
+    // The first radix-8 pass writes 8 consecutive elements per lane (8-way
+    // bank conflicts); rotating each lane pair's outputs removes them at the
+    // cost of 14 complex products.  Only without the LDS quarter table: with
+    // it the extra registers push the row kernels past 128 VGPRs (one
+    // workgroup per CU instead of two), which costs more than it saves.
+    constexpr bool kRotate = !QT;
     while (ns * 8 <= M) {
         const int nb = M >> 3;
         const int tstep = L / (ns * 8);
@@ -130,7 +136,21 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
 #pragma unroll
                 for (int r = 0; r < 8; ++r) v[b][r] = buf[j + r * nb];
                 const int k = j & (ns - 1);
-                if (k != 0) {
+                if (kRotate && ns == 1) {
+                    // first pass: rotate the outputs by rr = (j >> 1) & 7
+                    // (inputs times W8^(n rr), so output t is X[(t+rr)&7])
+                    // so that the 16 lanes of a group, which write 8
+                    // consecutive elements each, hit 16 distinct banks
+                    const int rr = (j >> 1) & 7;
+                    double2 wr = tw[rr * (L >> 3)];  // exp(-2 pi i rr/8)
+                    if (DIR > 0) wr = conjg(wr);
+                    double2 w = wr;
+#pragma unroll
+                    for (int r = 1; r < 8; ++r) {
+                        v[b][r] = cmul(v[b][r], w);
+                        w = cmul(w, wr);
+                    }
+                } else if (k != 0) {
                     double2 w1 = QT ? tq[swz_tw(k * (tstep >> 1))]
                                     : tw[k * tstep];
                     if (DIR > 0) w1 = conjg(w1);
@@ -155,8 +175,15 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
             if (j < nb) {
                 const int k = j & (ns - 1);
                 const int base = (j - k) * 8 + k;
+                if (kRotate && ns == 1) {
+                    const int rr = (j >> 1) & 7;
 #pragma unroll
-                for (int r = 0; r < 8; ++r) buf[base + r * ns] = v[b][r];
+                    for (int r = 0; r < 8; ++r)
+                        buf[base + ((r + rr) & 7)] = v[b][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) buf[base + r * ns] = v[b][r];
+                }
             }
         }
         __syncthreads();
