@@ -57,10 +57,12 @@ SIGNATURES = (
     ("ath_forward_projection", _I, [_P, _I, _I, _I, _P, _F, _D, _D, _P, _P]),
     ("ath_ramp_filter_adjoint", _I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _P,
                                      _P]),
-    ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _I, _I, _I, _I,
-                                             _P, _P, _P]),
+    ("ath_conversion_workspace_bytes", _S, [_I, _I, _I, _I, _D]),
+    ("ath_convert_sinogram_360_to_180", _I, [_P, _I, _I, _I, _D, _I, _I, _I,
+                                             _P, _P, _I, _P, _S, _P, _P]),
     ("ath_minus_log", _I, [_P, _P, ctypes.c_longlong, _P]),
-    ("ath_extend_sinogram", _I, [_P, _I, _I, _I, _I, _I, _I, _P, _P]),
+    ("ath_extend_sinogram", _I, [_P, _I, _I, _I, _D, _I, _I, _P, _S, _P,
+                                 _P]),
     ("ath_flat_field_correction", _I, [_P, _I, ctypes.c_longlong, _I, _I,
                                        ctypes.c_longlong, ctypes.c_longlong,
                                        _P, _P, _I, _P, _P]),

@@ -38,17 +38,41 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False, extra_flags=()):
+def _compile(src, obj, extra_flags, verbose):
+    cmd = ([hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"]
+           + list(extra_flags) + ["-c", os.path.join(SRC_DIR, src), "-o",
+                                  obj])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose=False, extra_flags=(), jobs=None):
+    """Compiles every source to an object in parallel (one hipcc per file),
+    then links the shared library; raises with hipcc's output on failure."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB_DIR, exist_ok=True)
+    obj_dir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, s.replace(".hip", ".o")) for s in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, os.cpu_count() or 1), 16)
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda so: _compile(so[0], so[1], extra_flags,
+                                                  verbose),
+                              zip(SOURCES, objs)))
+    for src, res in zip(SOURCES, results):
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed compiling %s" % src)
     tmp = LIB_PATH + ".tmp"
-    cmd = ([hipcc()] + HIPCC_FLAGS + list(extra_flags)
-           + [os.path.join(SRC_DIR, s) for s in SOURCES] + ["-o", tmp])
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + [
+        "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libalgotom_hip.so")
+        raise RuntimeError("hipcc failed linking libalgotom_hip.so")
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -19,6 +19,14 @@ int set_error(int status, const std::string& msg);
 int check_hip(hipError_t err, const char* what);
 void clear_error();
 
+// ---- launch-state caches (thread-safe; keyed by kernel and device) --------
+// Workgroups of `kernel` resident on the current device at once (occupancy
+// per CU times the CU count).
+int resident_grid(const void* kernel, int threads, size_t lds, int* out);
+// hipFuncAttributeMaxDynamicSharedMemorySize = bytes for `kernel`, once per
+// kernel and device with the largest size asked for.
+int allow_dynamic_lds(const void* kernel, size_t bytes);
+
 // ---- element conversions ---------------------------------------------------
 __device__ __forceinline__ float to_f32(float v) { return v; }
 __device__ __forceinline__ float to_f32(__half v) { return __half2float(v); }
@@ -171,9 +179,16 @@ int slice_metrics(const void* images, int dtype, int nimg, int nrows,
 int gaussian_axis(const float* in, float* out, int nouter, int length,
                   int ninner, const double* weights, int radius,
                   hipStream_t s);
-int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
-                      int overlap, int side, int norm, int width,
-                      float* factors, float* out, hipStream_t s);
+// scipy.ndimage.shift of float32 rows along the columns (order 3, mode
+// "nearest"), optionally flipped left-right and -logged first
+int shift_rows(const float* src, long long nrows, int ncol, int flip,
+               int apply_log, double sh, float* dst, hipStream_t s);
+// stitch_image of image 1 (img1) and image 2 (img2, flipped at read when
+// flip2), nslice interleaved slices of nrow_180 rows each
+int stitch_images(const float* img1, const float* img2, int flip2,
+                  int nrow_180, int nslice, int ncol, int overlap, int side,
+                  int norm, int width, const void* w1, const void* w2,
+                  int wdtype, float* factors, float* out, hipStream_t s);
 int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                         int pad, int pad_mode, const void* plan, int fft_len,
                         int nwin, float* out, hipStream_t stream);

@@ -578,23 +578,12 @@ template <int MODE, int SB, int PY, int A, typename Tin>
 int launch_batched(const ath_bp_job* jobs, int njobs, long long total,
                    int nblocks, int* counter, hipStream_t stream) {
     auto kern = bp_batched_kernel<MODE, SB, PY, A, Tin>;
-    static int resident = 0;  // workgroups resident at once, per kernel
-    if (nblocks <= 0 && resident > 0) nblocks = resident;
     if (nblocks <= 0) {
-        int per_cu = 0, dev = 0, cus = 0;
-        int st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                               &per_cu, reinterpret_cast<const void*>(kern),
-                               kThreads, 0),
-                           "occupancy query");
+        // workgroups resident at once on the current device (cached per
+        // kernel and device)
+        const int st = resident_grid(reinterpret_cast<const void*>(kern),
+                                     kThreads, 0, &nblocks);
         if (st != ATH_OK) return st;
-        st = check_hip(hipGetDevice(&dev), "hipGetDevice");
-        if (st != ATH_OK) return st;
-        st = check_hip(hipDeviceGetAttribute(
-                           &cus, hipDeviceAttributeMultiprocessorCount, dev),
-                       "CU count");
-        if (st != ATH_OK) return st;
-        nblocks = (per_cu > 0 ? per_cu : 1) * cus;
-        resident = nblocks;
     }
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream,
                        counter);

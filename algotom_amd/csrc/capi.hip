@@ -4,6 +4,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "ath_internal.h"
@@ -31,6 +35,65 @@ int check_hip(hipError_t err, const char* what) {
     return set_error(ATH_ERR_HIP, std::string(what) + ": " +
                                       hipGetErrorName(err) + " (" +
                                       hipGetErrorString(err) + ")");
+}
+
+namespace {
+struct LaunchKey {
+    const void* kernel;
+    int device, threads;
+    size_t lds;
+    bool operator<(const LaunchKey& o) const {
+        return std::tie(kernel, device, threads, lds) <
+               std::tie(o.kernel, o.device, o.threads, o.lds);
+    }
+};
+std::mutex g_launch_mutex;
+std::map<LaunchKey, int> g_resident;
+std::map<std::pair<const void*, int>, size_t> g_dyn_lds;
+}  // namespace
+
+int resident_grid(const void* kernel, int threads, size_t lds, int* out) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st != ATH_OK) return st;
+    const LaunchKey key{kernel, dev, threads, lds};
+    {
+        std::lock_guard<std::mutex> lock(g_launch_mutex);
+        auto it = g_resident.find(key);
+        if (it != g_resident.end()) {
+            *out = it->second;
+            return ATH_OK;
+        }
+    }
+    int per_cu = 0, cus = 0;
+    st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                       &per_cu, kernel, threads, lds),
+                   "occupancy query");
+    if (st != ATH_OK) return st;
+    st = check_hip(hipDeviceGetAttribute(
+                       &cus, hipDeviceAttributeMultiprocessorCount, dev),
+                   "CU count");
+    if (st != ATH_OK) return st;
+    const int n = std::max(1, per_cu) * std::max(1, cus);
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    g_resident[key] = n;
+    *out = n;
+    return ATH_OK;
+}
+
+int allow_dynamic_lds(const void* kernel, size_t bytes) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st != ATH_OK) return st;
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    size_t& done = g_dyn_lds[{kernel, dev}];
+    if (done >= bytes) return ATH_OK;
+    st = check_hip(hipFuncSetAttribute(
+                       kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                       (int)bytes),
+                   "hipFuncSetAttribute(max dynamic LDS)");
+    if (st == ATH_OK) done = bytes;
+    return st;
 }
 
 namespace {
@@ -86,6 +149,50 @@ int fill_bp_args(BpArgs& a, const void* sino, int nrow, int nslice, int ncol,
 }  // namespace ath
 
 using namespace ath;
+
+namespace ath {
+
+// The fused filter + back-projection on a workspace of workspace_bytes:
+// plan (aligned to 256 bytes), then the filtered stack.  Every byte it
+// reads is written first in the same call (the plan by plan_build, the
+// filtered stack by the filter), so no part needs clearing.
+int fbp_into_workspace(const void* sino, int in_dtype, int nrow, int nslice,
+                       int ncol, const float* angles, float center,
+                       double mask_radius, const double* taps, int nwin,
+                       int pad, int pad_mode, int apply_log, int acc,
+                       int filt_dtype, void* workspace,
+                       size_t workspace_bytes, float* recon, hipStream_t s) {
+    if (!workspace || !taps)
+        return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    const size_t need = ath_fbp_workspace_bytes(nrow, nslice, ncol, pad, nwin,
+                                                filt_dtype);
+    if (need == 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry or dtype");
+    if (workspace_bytes < need)
+        return set_error(ATH_ERR_WORKSPACE, "workspace too small");
+    if (filt_dtype != ATH_F32 && acc != ATH_ACC_F32)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "reduced-precision filtered storage needs "
+                         "acc = ATH_ACC_F32");
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center, 0,
+                          mask_radius, 1, recon);
+    if (st != ATH_OK) return st;
+    const int L = ath_filter_fft_length(ncol, pad);
+    if (L < 0) return L;
+    const size_t plan = plan_bytes(L, nwin);
+    char* ws = static_cast<char*>(workspace);
+    void* filt = ws + ((plan + 255) & ~(size_t)255);
+    st = plan_build(taps, nwin, ncol + 2 * pad, L, ws, s);
+    if (st != ATH_OK) return st;
+    st = ath_ramp_filter(sino, in_dtype, nrow, nslice, ncol, pad, pad_mode,
+                         apply_log, ws, L, nwin, filt, filt_dtype, s);
+    if (st != ATH_OK) return st;
+    a.sino = filt;
+    return back_projection(a, filt_dtype, acc, 0, 0, s);
+}
+
+}  // namespace ath
 
 extern "C" {
 
@@ -350,27 +457,75 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                                static_cast<hipStream_t>(stream));
 }
 
+size_t ath_conversion_workspace_bytes(int op, int nrow, int nslice, int ncol,
+                                      double overlap) {
+    if (nrow < 1 || nslice < 1 || ncol < 1 || (op != 0 && op != 1))
+        return 0;
+    const int ov = (int)std::nearbyint(overlap + 1.0e-3);
+    const bool sub = overlap - (double)ov != 0.0;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    if (op == 1)
+        return sub ? al((size_t)nrow * nslice * ncol * sizeof(float)) : 0;
+    const size_t img = (size_t)(nrow / 2 + 1) * nslice * ncol * sizeof(float);
+    return al(2 * (size_t)nslice * sizeof(float)) + (sub ? 2 * al(img) : 0);
+}
+
 int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
-                                    int nslice, int ncol, int overlap,
+                                    int nslice, int ncol, double overlap,
                                     int side, int norm, int width,
-                                    float* factors, float* out,
+                                    const void* wei1, const void* wei2,
+                                    int wei_dtype, void* work,
+                                    size_t work_bytes, float* out,
                                     void* stream) {
     clear_error();
-    if (!sino360 || !out || (norm && !factors))
+    if (!sino360 || !out || !work)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
     int st = check_geometry(nrow, nslice, ncol);
     if (st != ATH_OK) return st;
     if (nrow < 2) return set_error(ATH_ERR_INVALID_ARG, "nrow must be >= 2");
-    if (overlap < 1 || overlap > ncol)
+    if (!std::isfinite(overlap))
+        return set_error(ATH_ERR_INVALID_ARG, "overlap must be finite");
+    // conversion.py:107-108: the integer part rounds overlap + 1e-3
+    const int ov = (int)std::nearbyint(overlap + 1.0e-3);
+    const double sub = overlap - (double)ov;
+    if (ov < 1 || ov > ncol)
         return set_error(ATH_ERR_INVALID_ARG, "overlap must be in [1, ncol]");
     if (side != 0 && side != 1)
         return set_error(ATH_ERR_INVALID_ARG, "side must be 0 or 1");
-    if ((long long)width < 2LL * ncol - overlap)
+    if ((long long)width < 2LL * ncol - ov)
         return set_error(ATH_ERR_INVALID_ARG,
                          "width must be >= 2*ncol - overlap");
-    return stitch_360_to_180(sino360, nrow, nslice, ncol, overlap, side, norm,
-                             width, factors, out,
-                             static_cast<hipStream_t>(stream));
+    if ((wei1 == nullptr) != (wei2 == nullptr))
+        wei1 = wei2 = nullptr;  // stitch_image uses the ramp then (:119)
+    if (wei1 && wei_dtype != ATH_F32 && wei_dtype != ATH_F64)
+        return set_error(ATH_ERR_INVALID_ARG,
+                         "weight matrices must be float32 or float64");
+    if (work_bytes < ath_conversion_workspace_bytes(0, nrow, nslice, ncol,
+                                                    overlap))
+        return set_error(ATH_ERR_WORKSPACE, "workspace too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n180 = nrow / 2 + 1;
+    char* ws = static_cast<char*>(work);
+    float* factors = reinterpret_cast<float*>(ws);
+    const float* top = sino360;
+    const float* bot = sino360 + (size_t)(nrow - n180) * nslice * ncol;
+    if (sub == 0.0)
+        return stitch_images(top, bot, 1, n180, nslice, ncol, ov, side, norm,
+                             width, wei1, wei2, wei_dtype, factors, out, s);
+    // conversion.py:109-117: image 1 shifted by +sub (side 1) or -sub
+    // (side 0), the flipped image 2 by the opposite amount
+    const size_t img = (size_t)n180 * nslice * ncol;
+    const size_t off1 = (2 * (size_t)nslice * sizeof(float) + 255) & ~(size_t)255;
+    const size_t off2 = off1 + ((img * sizeof(float) + 255) & ~(size_t)255);
+    float* t1 = reinterpret_cast<float*>(ws + off1);
+    float* t2 = reinterpret_cast<float*>(ws + off2);
+    const double s1 = side == 1 ? sub : -sub;
+    st = shift_rows(top, (long long)n180 * nslice, ncol, 0, 0, s1, t1, s);
+    if (st != ATH_OK) return st;
+    st = shift_rows(bot, (long long)n180 * nslice, ncol, 1, 0, -s1, t2, s);
+    if (st != ATH_OK) return st;
+    return stitch_images(t1, t2, 0, n180, nslice, ncol, ov, side, norm, width,
+                         wei1, wei2, wei_dtype, factors, out, s);
 }
 
 int ath_flat_field_log(const void* proj, int in_dtype, int nproj, int height,
@@ -434,19 +589,35 @@ int ath_vertical_finish(void* data, int dtype, long long total, int width,
 }
 
 int ath_extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
-                        int overlap, int side, int apply_log, float* out,
-                        void* stream) {
+                        double overlap, int side, int apply_log, void* work,
+                        size_t work_bytes, float* out, void* stream) {
     clear_error();
     if (!sino || !out)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
     if (nrow < 1 || nslice < 1 || ncol < 1)
         return set_error(ATH_ERR_INVALID_ARG, "bad geometry");
-    if (overlap < 1 || overlap > ncol)
+    if (!std::isfinite(overlap))
+        return set_error(ATH_ERR_INVALID_ARG, "overlap must be finite");
+    const int ov = (int)std::nearbyint(overlap + 1.0e-3);
+    const double sub = overlap - (double)ov;
+    if (ov < 1 || ov > ncol)
         return set_error(ATH_ERR_INVALID_ARG, "overlap must be in [1, ncol]");
     if (side != 0 && side != 1)
         return set_error(ATH_ERR_INVALID_ARG, "side must be 0 or 1");
-    return extend_sinogram(sino, nrow, nslice, ncol, overlap, side, apply_log,
-                           out, static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (sub == 0.0)
+        return extend_sinogram(sino, nrow, nslice, ncol, ov, side, apply_log,
+                               out, s);
+    if (!work || work_bytes < ath_conversion_workspace_bytes(
+                                  1, nrow, nslice, ncol, overlap))
+        return set_error(ATH_ERR_WORKSPACE, "workspace too small");
+    // conversion.py:435-453: (log), then the shift by +sub (side 1) or
+    // -sub (side 0), then the weights and the padding
+    float* t = static_cast<float*>(work);
+    const int st = shift_rows(sino, (long long)nrow * nslice, ncol, 0,
+                              apply_log, side == 1 ? sub : -sub, t, s);
+    if (st != ATH_OK) return st;
+    return extend_sinogram(t, nrow, nslice, ncol, ov, side, 0, out, s);
 }
 
 int ath_flat_field_correction(const void* proj, int in_dtype,
@@ -728,35 +899,13 @@ int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
                            size_t workspace_bytes, float* recon,
                            void* stream) {
     clear_error();
-    if (!workspace || !taps)
+    if (!workspace)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
-    const size_t need = ath_fbp_workspace_bytes(nrow, nslice, ncol, pad, nwin,
-                                                filt_dtype);
-    if (need == 0)
-        return set_error(ATH_ERR_INVALID_ARG, "bad geometry or dtype");
-    if (workspace_bytes < need)
-        return set_error(ATH_ERR_WORKSPACE, "workspace too small");
-    if (filt_dtype != ATH_F32 && acc != ATH_ACC_F32)
-        return set_error(ATH_ERR_UNSUPPORTED,
-                         "reduced-precision filtered storage needs "
-                         "acc = ATH_ACC_F32");
-    BpArgs a;
-    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center, 0,
-                          mask_radius, 1, recon);
-    if (st != ATH_OK) return st;
-    const int L = ath_filter_fft_length(ncol, pad);
-    if (L < 0) return L;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t plan = plan_bytes(L, nwin);
-    char* ws = static_cast<char*>(workspace);
-    void* filt = ws + ((plan + 255) & ~(size_t)255);
-    st = plan_build(taps, nwin, ncol + 2 * pad, L, ws, s);
-    if (st != ATH_OK) return st;
-    st = ath_ramp_filter(sino, in_dtype, nrow, nslice, ncol, pad, pad_mode,
-                         apply_log, ws, L, nwin, filt, filt_dtype, stream);
-    if (st != ATH_OK) return st;
-    a.sino = filt;
-    return back_projection(a, filt_dtype, acc, 0, 0, s);
+    return ath::fbp_into_workspace(sino, in_dtype, nrow, nslice, ncol, angles,
+                                   center, mask_radius, taps, nwin, pad,
+                                   pad_mode, apply_log, acc, filt_dtype,
+                                   workspace, workspace_bytes, recon,
+                                   static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

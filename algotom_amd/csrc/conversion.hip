@@ -13,6 +13,7 @@
 // multiply-adds of input samples, so the pass is HBM-bound.
 #include "ath_internal.h"
 #include "log32.h"
+#include "spline.h"
 
 namespace ath {
 namespace {
@@ -28,27 +29,41 @@ __device__ __forceinline__ double ramp_down(int t, int ov) {
     return (double)t * step + 1.0;
 }
 
+// The two images of a stitch, slice j row r: image 1 at
+// img1 + (r * nslice + j) * ncol; image 2 at img2 + (r * nslice + j) * ncol,
+// read left-right flipped when flip2 (the raw bottom rows of a 360-degree
+// sinogram) or as stored (rows already flipped and shifted).
+struct StitchImages {
+    const float* img1;
+    const float* img2;
+    int flip2;
+    __device__ __forceinline__ float at1(size_t row, int col, int ncol) const {
+        return img1[row * ncol + col];
+    }
+    __device__ __forceinline__ float at2(size_t row, int col, int ncol) const {
+        return img2[row * ncol + (flip2 ? ncol - 1 - col : col)];
+    }
+};
+
 // factors[2*j] = mean of the first image's overlap, factors[2*j+1] = mean of
 // the second image's overlap (slice j), float64 sums rounded to float32.
 __global__ __launch_bounds__(kThreads) void stitch_means_kernel(
-    const float* __restrict__ sino, int nrow, int nslice, int ncol,
-    int nrow_180, int overlap, int side, float* __restrict__ factors) {
+    StitchImages im, int nslice, int ncol, int nrow_180, int overlap,
+    int side, float* __restrict__ factors) {
     __shared__ double red[2][kThreads];
     const int j = blockIdx.x;
     const int tid = threadIdx.x;
-    // image 1 = top rows [0, nrow_180); image 2 = rows [nrow - nrow_180,
-    // nrow) flipped left-right.  side 1: mean(img1[:, -ov:]),
-    // mean(img2[:, :ov]) = mean of the bottom rows' last ov columns; side 0:
-    // mean(img1[:, :ov]), mean(img2[:, -ov:]) = bottom rows' first ov.
+    // side 1: mean(img1[:, -ov:]), mean(img2[:, :ov]); side 0:
+    // mean(img1[:, :ov]), mean(img2[:, -ov:]) (image coordinates)
     const int c1 = side == 1 ? ncol - overlap : 0;
-    const int c2 = side == 1 ? ncol - overlap : 0;
+    const int c2 = side == 1 ? 0 : ncol - overlap;
     const long long n = (long long)nrow_180 * overlap;
     double s1 = 0.0, s2 = 0.0;
     for (long long e = tid; e < n; e += kThreads) {
         const int r = (int)(e / overlap), k = (int)(e % overlap);
-        s1 += (double)sino[((size_t)r * nslice + j) * ncol + c1 + k];
-        const int rb = nrow - nrow_180 + r;
-        s2 += (double)sino[((size_t)rb * nslice + j) * ncol + c2 + k];
+        const size_t row = (size_t)r * nslice + j;
+        s1 += (double)im.at1(row, c1 + k, ncol);
+        s2 += (double)im.at2(row, c2 + k, ncol);
     }
     red[0][tid] = s1;
     red[1][tid] = s2;
@@ -66,18 +81,25 @@ __global__ __launch_bounds__(kThreads) void stitch_means_kernel(
     }
 }
 
-// out: (nrow_180, nslice, width), width >= 2*ncol - overlap (edge-extended)
+// out: (nrow_180, nslice, width), width >= 2*ncol - overlap (edge-extended).
+// w1 / w2: the caller's weight matrices (nrow_180, ncol) of type TW shared
+// by every slice, or NULL for make_weight_matrix's linear ramp.  The
+// product of a sample with its weight is formed as numpy forms mat *
+// wei_mat: in float32 for float32 weights, in float64 for float64 ones.
+// mat_comb[:, first] = img_first * w_first (rounded to float32), then
+// mat_comb[:, second] += img_second * w_second in the product's type.
+template <typename TW>
 __global__ __launch_bounds__(kThreads) void stitch_kernel(
-    const float* __restrict__ sino, int nrow, int nslice, int ncol,
-    int nrow_180, int overlap, int side, int norm,
-    const float* __restrict__ factors, int width, float* __restrict__ out) {
+    StitchImages im, int nslice, int ncol, int nrow_180, int overlap,
+    int side, int norm, const float* __restrict__ factors,
+    const TW* __restrict__ w1, const TW* __restrict__ w2, int width,
+    float* __restrict__ out) {
+    using V = TW;
     const int rows = nrow_180 * nslice;
     const int w0 = 2 * ncol - overlap;
     for (int rj = blockIdx.y; rj < rows; rj += gridDim.y) {
       const int j = rj % nslice, r = rj / nslice;  // block-uniform
-      const float* top = sino + ((size_t)r * nslice + j) * ncol;
-      const float* bot =
-          sino + ((size_t)(nrow - nrow_180 + r) * nslice + j) * ncol;
+      const size_t row = (size_t)r * nslice + j;
       // image 2 is scaled by factor1 / factor2 ((mat2 * f1) / f2)
       const float f1 = norm ? factors[2 * j] : 1.0f;
       const float f2 = norm ? factors[2 * j + 1] : 1.0f;
@@ -86,33 +108,76 @@ __global__ __launch_bounds__(kThreads) void stitch_kernel(
            kk += gridDim.x * kThreads) {
         const int k = kk < w0 ? kk : w0 - 1;  // edge extension to width
         // first image (placed at 0) and second image (placed at ncol - ov)
-        const int a_off = 0, b_off = ncol - overlap;
         // side 1: first = img1 (top), second = img2; side 0: swapped
         float v = 0.0f;
         for (int part = 0; part < 2; ++part) {
-            const int off = part == 0 ? a_off : b_off;
+            const int off = part == 0 ? 0 : ncol - overlap;
             const int col = k - off;
             if (col < 0 || col >= ncol) continue;
             const bool is_img1 = (side == 1) == (part == 0);
-            float x, wgt = 1.0f;
+            float x;
             if (is_img1) {
-                x = top[col];
+                x = im.at1(row, col, ncol);
             } else {
-                x = bot[ncol - 1 - col];  // left-right flip
+                x = im.at2(row, col, ncol);
                 if (norm) x = (x * f1) / f2;
             }
-            // linear ramp over the overlap (make_weight_matrix): the image
-            // on the left fades out over its last ov columns, the one on the
-            // right fades in over its first ov columns
-            if (part == 0 && col >= ncol - overlap) {
-                wgt = (float)ramp_down(col - (ncol - overlap), overlap);
-            } else if (part == 1 && col < overlap) {
-                wgt = (float)(1.0 - ramp_down(col, overlap));  // list_up
+            V prod;
+            if (w1 != nullptr) {
+                const TW* w = is_img1 ? w1 : w2;
+                prod = (V)x * w[(size_t)r * ncol + col];
+            } else {
+                // linear ramp over the overlap (make_weight_matrix): the
+                // image on the left fades out over its last ov columns, the
+                // one on the right fades in over its first ov columns
+                float wgt = 1.0f;
+                if (part == 0 && col >= ncol - overlap)
+                    wgt = (float)ramp_down(col - (ncol - overlap), overlap);
+                else if (part == 1 && col < overlap)
+                    wgt = (float)(1.0 - ramp_down(col, overlap));  // list_up
+                prod = (V)(x * wgt);
             }
-            v += x * wgt;
+            // the first image is stored (float32), the second added to it
+            v = part == 0 ? (float)prod : (float)((V)v + prod);
         }
         dst[kk] = v;
       }
+    }
+}
+
+// scipy.ndimage.shift(row, (0, sh), mode="nearest") (order 3) of nrows
+// float32 rows of ncol samples (row q at src + q * ncol, read left-right
+// flipped when flip, -log applied first when apply_log), float32 out at
+// dst + q * ncol: the reference's sub-pixel shift of the two images before
+// stitching (conversion.py:109-117) and of the logged sinogram in
+// extend_sinogram (:435-453).  One workgroup per row, grid-strided.
+__global__ __launch_bounds__(kThreads) void shift_rows_kernel(
+    const float* __restrict__ src, long long nrows, int ncol, int flip,
+    int apply_log, double sh, float* __restrict__ dst) {
+    extern __shared__ double sh_line[];
+    __shared__ double hs[kK3 + 1];
+    __shared__ Log32Table s_tab;
+    const int L1 = ncol + 2 * kNearestPad;
+    double* line = sh_line;
+    double* coef = sh_line + L1;
+    const int tid = threadIdx.x;
+    if (tid <= kK3) hs[tid] = cubic_tap(tid);
+    if (apply_log) stage_log32_table<kThreads>(&s_tab);
+    __syncthreads();
+    for (long long q = blockIdx.x; q < nrows; q += gridDim.x) {
+        const float* in = src + q * ncol;
+        for (int p = tid; p < L1; p += kThreads) {
+            const int c = min(max(p - kNearestPad, 0), ncol - 1);
+            float x = in[flip ? ncol - 1 - c : c];
+            if (apply_log) x = neg_log_f32(x, s_tab);
+            line[p] = (double)x;
+        }
+        __syncthreads();
+        float* o = dst + q * ncol;
+        shift_line_nearest<kThreads>(line, coef, ncol, sh, hs,
+                                     [o](int jx, double v) {
+                                         o[jx] = (float)v;
+                                     });
     }
 }
 
@@ -271,21 +336,52 @@ int flat_field_correction(const void* proj, int in_dtype, long long nouter,
 #undef ATH_FLAT
 }
 
-int stitch_360_to_180(const float* sino, int nrow, int nslice, int ncol,
-                      int overlap, int side, int norm, int width,
-                      float* factors, float* out, hipStream_t s) {
-    const int nrow_180 = nrow / 2 + 1;
+int shift_rows(const float* src, long long nrows, int ncol, int flip,
+               int apply_log, double sh, float* dst, hipStream_t s) {
+    const size_t lds = 2 * (size_t)(ncol + 2 * kNearestPad) * sizeof(double);
+    if (lds > 152 * 1024)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "sub-pixel shift: rows above 9700 columns");
+    if (lds > 64 * 1024) {
+        const int st = allow_dynamic_lds(
+            reinterpret_cast<const void*>(shift_rows_kernel), lds);
+        if (st != ATH_OK) return st;
+    }
+    int grid = 0;
+    int st = resident_grid(reinterpret_cast<const void*>(shift_rows_kernel),
+                           kThreads, lds, &grid);
+    if (st != ATH_OK) return st;
+    const long long blocks = std::max<long long>(
+        1, std::min<long long>(nrows, (long long)grid));
+    hipLaunchKernelGGL(shift_rows_kernel, dim3((unsigned)blocks),
+                       dim3(kThreads), lds, s, src, nrows, ncol, flip,
+                       apply_log, sh, dst);
+    return check_hip(hipGetLastError(), "shift_rows_kernel launch");
+}
+
+int stitch_images(const float* img1, const float* img2, int flip2,
+                  int nrow_180, int nslice, int ncol, int overlap, int side,
+                  int norm, int width, const void* w1, const void* w2,
+                  int wdtype, float* factors, float* out, hipStream_t s) {
+    const StitchImages im{img1, img2, flip2};
     if (norm) {
         hipLaunchKernelGGL(stitch_means_kernel, dim3(nslice), dim3(kThreads),
-                           0, s, sino, nrow, nslice, ncol, nrow_180, overlap,
-                           side, factors);
+                           0, s, im, nslice, ncol, nrow_180, overlap, side,
+                           factors);
         const int st = check_hip(hipGetLastError(), "stitch_means_kernel");
         if (st != ATH_OK) return st;
     }
     const dim3 grid = row_grid((long long)nrow_180 * nslice, width, kThreads);
-    hipLaunchKernelGGL(stitch_kernel, grid, dim3(kThreads),
-                       0, s, sino, nrow, nslice, ncol, nrow_180, overlap, side,
-                       norm, factors, width, out);
+    if (wdtype == ATH_F64)
+        hipLaunchKernelGGL(stitch_kernel<double>, grid, dim3(kThreads), 0, s,
+                           im, nslice, ncol, nrow_180, overlap, side, norm,
+                           factors, static_cast<const double*>(w1),
+                           static_cast<const double*>(w2), width, out);
+    else
+        hipLaunchKernelGGL(stitch_kernel<float>, grid, dim3(kThreads), 0, s,
+                           im, nslice, ncol, nrow_180, overlap, side, norm,
+                           factors, static_cast<const float*>(w1),
+                           static_cast<const float*>(w2), width, out);
     return check_hip(hipGetLastError(), "stitch_kernel launch");
 }
 

@@ -45,13 +45,11 @@
 #include "ath_internal.h"
 #include "fft_lds.h"
 #include "log32.h"
+#include "spline.h"
 
 namespace ath {
 namespace {
 
-constexpr int kK3 = 32;         // cubic inverse-filter half length
-constexpr int kK5 = 52;         // quintic inverse-filter half length
-constexpr int kNearestPad = 12; // scipy's pre-padding for mode "nearest"
 constexpr int kThreads = 256;
 constexpr int kMaxLine = 8192 + 2 * kNearestPad;  // longest shifted line
 constexpr double kPi = 3.141592653589793;         // numpy.pi
@@ -92,41 +90,6 @@ __host__ __device__ T* at(void* base, size_t off) {
 template <typename T>
 __host__ __device__ const T* at(const void* base, size_t off) {
     return reinterpret_cast<const T*>(static_cast<const char*>(base) + off);
-}
-
-// ---- B-spline weights -----------------------------------------------------
-__device__ __forceinline__ double beta3(double t) {
-    t = fabs(t);
-    if (t < 1.0) return 2.0 / 3.0 - t * t + 0.5 * t * t * t;
-    if (t < 2.0) { const double u = 2.0 - t; return u * u * u / 6.0; }
-    return 0.0;
-}
-
-__device__ __forceinline__ double beta5(double t) {
-    t = fabs(t);
-    if (t < 1.0) {
-        const double t2 = t * t;
-        return 11.0 / 20.0 + t2 * (-0.5 + t2 * (0.25 - t / 12.0));
-    }
-    if (t < 2.0)
-        return 17.0 / 40.0 +
-               t * (5.0 / 8.0 +
-                    t * (-7.0 / 4.0 +
-                         t * (5.0 / 4.0 + t * (-3.0 / 8.0 + t / 24.0))));
-    if (t < 3.0) {
-        const double u = 3.0 - t, u2 = u * u;
-        return u2 * u2 * u / 120.0;
-    }
-    return 0.0;
-}
-
-// half-sample symmetric (scipy "reflect") and whole-sample ("mirror")
-// index folds onto [0, n)
-__device__ __forceinline__ int fold_half(int i, int n) {
-    return pad_source(i, n, ATH_PAD_SYMMETRIC);
-}
-__device__ __forceinline__ int fold_whole(int i, int n) {
-    return pad_source(i, n, ATH_PAD_REFLECT);
 }
 
 // numpy.remainder(a, b) for b > 0
@@ -185,11 +148,7 @@ __global__ void dfi_tables_kernel(const double* __restrict__ window, int N,
                                   double* __restrict__ h5) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) win[i] = window ? window[i] : 1.0;
-    if (i <= kK3) {
-        const double w = -4.0;
-        const double z = 2.0 / (w - sqrt(w * w - 4.0));
-        h3[i] = (1.0 - z) * (1.0 - z) * pow(z, (double)i) / (1.0 - z * z);
-    }
+    if (i <= kK3) h3[i] = cubic_tap(i);
     if (i <= kK5) {
         const double w1 = -13.0 + sqrt(105.0), w2 = -13.0 - sqrt(105.0);
         const double z1 = 2.0 / (w1 - sqrt(w1 * w1 - 4.0));

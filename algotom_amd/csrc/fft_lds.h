@@ -376,38 +376,31 @@ inline int maxb_for(int M) {
     return 0;
 }
 
-// Allows dynamic LDS above 64 KB for `kernel`, once per kernel (a host-side
-// attribute, so later launches, including graph-captured ones, enqueue only
-// the kernel).  `bytes` is the largest size the kernel is launched with.
-// Workgroups of `kernel` resident on the device at once with `lds` bytes of
-// dynamic LDS each (cached per kernel): the grid of a persistent launch.
+// Workgroups of `kernel` resident on the current device at once with `lds`
+// bytes of dynamic LDS each: the grid of a persistent launch (cached per
+// kernel, device, block size and LDS size; 1024 if the query fails).
 template <typename K>
 int resident_blocks(K kernel, size_t lds, int threads = kFftThreads) {
-    static int cached = 0;  // one value per kernel instantiation
-    if (cached > 0) return cached;
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(kernel), threads,
-            lds) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                              dev) != hipSuccess) {
-        (void)hipGetLastError();
+    int n = 0;
+    if (resident_grid(reinterpret_cast<const void*>(kernel), threads, lds,
+                      &n) != ATH_OK) {
+        clear_error();
         return 1024;
     }
-    cached = std::max(1, per_cu) * std::max(1, cus);
-    return cached;
+    return n;
 }
 
+// Allows dynamic LDS above 64 KB for `kernel` on the current device (a
+// host-side attribute, so later launches, including graph-captured ones,
+// enqueue only the kernel).  `bytes` is the largest size the kernel is
+// launched with.  Set once per kernel and device, and only marked done
+// when it succeeded.
 template <typename K>
 void prepare_lds(K kernel, size_t bytes) {
-    static bool done = false;  // one flag per kernel instantiation
-    if (done || bytes <= 64 * 1024) return;
-    hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(kernel),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) (void)hipGetLastError();
-    done = true;
+    if (bytes <= 64 * 1024) return;
+    if (allow_dynamic_lds(reinterpret_cast<const void*>(kernel), bytes) !=
+        ATH_OK)
+        clear_error();
 }
 
 }  // namespace

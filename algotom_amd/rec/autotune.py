@@ -77,11 +77,43 @@ def _load():
     return _table
 
 
-def lookup(key):
-    """(slice_batch, tile_rows) for key, or None."""
+def compiled_configs(kernel, wide=False):
+    """Every (slice_batch, tile_rows) the library compiles for a kernel
+    ("stack": ath_back_projection_tiled, "batch":
+    ath_back_projection_batched); wide = 16-bit storage."""
+    if kernel == "batch":
+        return {(1, 16), (1, 32), (1, 64), (4, 16), (4, 32)}
+    out = {(sb, rows) for sb in (1, 2, 4) for rows in CANDIDATE_ROWS}
+    if wide:
+        out |= {(8, 16), (8, 32)}
+    return out
+
+
+def lookup(key, valid=None):
+    """(slice_batch, tile_rows) for key, or None.  With ``valid`` (a set of
+    configurations), an entry outside it -- a stale or hand-edited cache
+    entry the library does not compile -- is dropped from the in-process
+    table and None is returned."""
     with _lock:
-        hit = _load().get(key)
-    return tuple(hit) if hit else None
+        table = _load()
+        hit = table.get(key)
+        if not hit:
+            return None
+        try:
+            cfg = tuple(int(v) for v in hit)
+        except (TypeError, ValueError):
+            cfg = None
+        if cfg is None or len(cfg) != 2 or (valid is not None
+                                            and cfg not in valid):
+            del table[key]
+            return None
+    return cfg
+
+
+def forget(key):
+    """Drop key from the in-process table (a configuration that failed)."""
+    with _lock:
+        _load().pop(key, None)
 
 
 def record(key, config, persist=True):

@@ -4,18 +4,16 @@ The reference reconstructs slices one call at a time (reconstruction.py
 :460-490) and datasets in separate SLURM jobs.  Here a batch is planned once
 (device buffers, filter plans, job descriptors) and executed as
 
-* one ramp-filter launch per FFT length (every row of every sinogram that
-  shares the length, one workgroup per row), then
-* one persistent back-projection launch over the (sinogram, tile) work list
-  of the whole batch, workgroups pulling items from a device counter,
-  heaviest sinograms (most angles) first.
+* one ramp-filter launch per FFT length and input type (every row of every
+  sinogram that shares them, one workgroup per row), then
+* one persistent back-projection launch per arithmetic over the (sinogram,
+  tile) work list of its sinograms, workgroups pulling items from a device
+  counter, heaviest sinograms (most angles) first.
 
 ``capture()`` records that launch sequence in a HIP graph (torch.cuda.CUDAGraph
 on ROCm) so a replay costs one graph launch.  Per sinogram the arithmetic is
 that of ``fbp_reconstruction`` with the same arguments.
 """
-
-import ctypes
 
 import numpy as np
 
@@ -42,11 +40,21 @@ class BatchFBP:
         None keeps the reference arithmetic.
     tile_rows : int or None
         Back-projection tile height; None takes the tuned value.
+    dtypes : list of numpy dtypes or None
+        Element type per sinogram (float32 or float64, as fbp_reconstruction
+        keeps it through the log and the filter); None: all float32.
+
+    Each sinogram gets exactly the arithmetic ``fbp_reconstruction`` gives
+    it: with ``precision=None`` a single sinogram with ``gpu=True`` is
+    back-projected with float64 sums and the rest with float32 storage
+    after every angle, so the batch runs one persistent back-projection
+    launch per arithmetic it contains.
     """
 
     def __init__(self, shapes, centers, angles=None, filter_name="hann",
                  pad=None, pad_mode="edge", apply_log=False, ratio=1.0,
-                 gpu=True, precision=None, tile_rows=None, nblocks=0):
+                 gpu=True, precision=None, tile_rows=None, nblocks=0,
+                 dtypes=None):
         import torch
         self.dev = nat.device()
         self.lib = nat.load()
@@ -58,34 +66,41 @@ class BatchFBP:
             raise ValueError("one centre per sinogram is required")
         if angles is None:
             angles = [None] * n
+        if dtypes is None:
+            dtypes = [np.float32] * n
+        in_types = [torch.float64 if np.dtype(d) == np.float64
+                    else torch.float32 for d in dtypes]
         self.shapes3 = [(s[0], 1, s[1]) if len(s) == 2 else s for s in shapes]
         self.input_3d = [len(s) == 3 for s in shapes]
         self.apply_log = bool(apply_log)
         self.pad_mode = pad_mode
         self.precision = precision
-        all_2d = all(s[1] == 1 for s in self.shapes3)
         if precision is None:
-            self.acc = nat.ACC_F64 if (gpu and all_2d) else nat.ACC_F32SEQ
+            accs = [nat.ACC_F64 if (gpu and s[1] == 1) else nat.ACC_F32SEQ
+                    for s in self.shapes3]
         else:
-            self.acc = nat.ACC_F32
+            accs = [nat.ACC_F32] * n
         store = rec._storage_dtype(precision)
         self.store_code = nat.dtype_code(store)
-        self.slice_batch = 1 if all_2d else 4
-        key = autotune.make_key("batched", self.acc, str(store).split(".")[-1],
+        main = max(set(accs), key=lambda a: sum(
+            s[0] * s[1] * s[2] * s[2]
+            for s, b in zip(self.shapes3, accs) if b == a))
+        key = autotune.make_key("batched", main, str(store).split(".")[-1],
                                 max(s[2] for s in self.shapes3),
                                 max(s[1] for s in self.shapes3))
         self.tune_key = key
+        valid = {r for _, r in autotune.compiled_configs("batch")}
         if tile_rows is None:
             hit = autotune.lookup(key)
-            tile_rows = hit[1] if hit else 32
+            tile_rows = hit[1] if hit and hit[1] in valid else 32
         self.tile_rows = int(tile_rows)
         self.nblocks = int(nblocks)
 
         self.inputs, self.filtered, self.outputs = [], [], []
         self._angles, self._keep = [], []
-        for (nrow, nslice, ncol), ang in zip(self.shapes3, angles):
-            self.inputs.append(torch.empty((nrow, nslice, ncol),
-                                           dtype=torch.float32,
+        for (nrow, nslice, ncol), ang, tin in zip(self.shapes3, angles,
+                                                  in_types):
+            self.inputs.append(torch.empty((nrow, nslice, ncol), dtype=tin,
                                            device=self.dev))
             self.filtered.append(torch.empty((nrow, nslice, ncol),
                                              dtype=store, device=self.dev))
@@ -99,15 +114,16 @@ class BatchFBP:
                                  "row number of the sinogram!!!")
             self._angles.append(rec._angles_device(ang, self.dev))
 
-        # ---- filter groups: one launch per FFT length
+        # ---- filter groups: one launch per (FFT length, input type)
         self.filter_groups = []
         groups = {}
         for i, (nrow, nslice, ncol) in enumerate(self.shapes3):
             p = min(int(0.15 * ncol), 150) if pad is None else int(pad)
             L = nat.check(self.lib.ath_filter_fft_length(ncol, p),
                           "ath_filter_fft_length")
-            groups.setdefault(L, []).append((i, p))
-        for L, members in sorted(groups.items()):
+            code = nat.dtype_code(in_types[i])
+            groups.setdefault((L, code), []).append((i, p))
+        for (L, code), members in sorted(groups.items()):
             jobs = (nat.FilterJob * len(members))()
             row = 0
             for k, (i, p) in enumerate(members):
@@ -120,47 +136,57 @@ class BatchFBP:
                                         plan.data_ptr(), nrow, nslice, ncol,
                                         p, nwin, 0, row)
                 row += nrow * nslice
-            self.filter_groups.append((L, len(members), row,
+            self.filter_groups.append((L, code, len(members), row,
                                        self._to_device(jobs)))
 
-        # ---- back-projection work list: heaviest per-tile cost first
-        order = sorted(range(n), key=lambda i: -self.shapes3[i][0])
-        self._bp_jobs = (nat.BpJob * n)()
-        tile = 0
-        for k, i in enumerate(order):
-            nrow, nslice, ncol = self.shapes3[i]
-            c = centers[i]
-            if c < 0 or c >= ncol:
-                raise ValueError("Center is out of the range [0, {}]"
-                                 .format(ncol - 1))
-            radius = -1.0
-            if ratio is not None:
-                r = ratio
-                if r == 0.0:
-                    r = min(c, ncol - c) / (0.5 * ncol)
-                radius = float(r * (ncol // 2))
-            self._bp_jobs[k] = nat.BpJob(
-                self.filtered[i].data_ptr(), self.outputs[i].data_ptr(),
-                self._angles[i].data_ptr(), nrow, nslice, ncol,
-                float(np.float32(c)), radius, 1, 0, tile)
-            tile += self.lib.ath_bp_job_tiles(ncol, nslice, self.slice_batch,
-                                              self.tile_rows)
-        self.total_tiles = tile
+        # ---- back-projection work lists, one per arithmetic: heaviest
+        # per-tile cost (most angles) first
+        self.bp_groups = []
+        for acc in sorted(set(accs)):
+            members = [i for i in range(n) if accs[i] == acc]
+            order = sorted(members, key=lambda i: -self.shapes3[i][0])
+            jobs = (nat.BpJob * len(order))()
+            for k, i in enumerate(order):
+                nrow, nslice, ncol = self.shapes3[i]
+                c = centers[i]
+                if c < 0 or c >= ncol:
+                    raise ValueError("Center is out of the range [0, {}]"
+                                     .format(ncol - 1))
+                radius = -1.0
+                if ratio is not None:
+                    r = ratio
+                    if r == 0.0:
+                        r = min(c, ncol - c) / (0.5 * ncol)
+                    radius = float(r * (ncol // 2))
+                jobs[k] = nat.BpJob(
+                    self.filtered[i].data_ptr(), self.outputs[i].data_ptr(),
+                    self._angles[i].data_ptr(), nrow, nslice, ncol,
+                    float(np.float32(c)), radius, 1, 0, 0)
+            sb = 1 if all(self.shapes3[i][1] == 1 for i in members) else 4
+            self.bp_groups.append({
+                "acc": acc, "slice_batch": sb, "jobs": jobs,
+                "counter": torch.zeros(1, dtype=torch.int32,
+                                       device=self.dev)})
+        self.acc = main
         self.updates = sum(s[0] * s[1] * s[2] * s[2] for s in self.shapes3)
         self._retile()
-        self.bp_jobs_dev = self._to_device(self._bp_jobs)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.graph = None
 
+    @property
+    def total_tiles(self):
+        return sum(g["total"] for g in self.bp_groups)
+
     def _retile(self):
-        tile = 0
-        for k in range(len(self._bp_jobs)):
-            jb = self._bp_jobs[k]
-            jb.tile_begin = tile
-            tile += self.lib.ath_bp_job_tiles(jb.ncol, jb.nslice,
-                                              self.slice_batch,
-                                              self.tile_rows)
-        self.total_tiles = tile
+        for g in self.bp_groups:
+            tile = 0
+            for k in range(len(g["jobs"])):
+                jb = g["jobs"][k]
+                jb.tile_begin = tile
+                tile += self.lib.ath_bp_job_tiles(jb.ncol, jb.nslice,
+                                                  g["slice_batch"],
+                                                  self.tile_rows)
+            g["total"] = tile
+            g["jobs_dev"] = self._to_device(g["jobs"])
 
     def _to_device(self, cstruct):
         import torch
@@ -171,22 +197,22 @@ class BatchFBP:
         """Change the back-projection tile height (drops a captured graph)."""
         self.tile_rows = int(tile_rows)
         self._retile()
-        self.bp_jobs_dev = self._to_device(self._bp_jobs)
         self.graph = None
 
     def launch(self):
         """Enqueue the whole batch on the current stream."""
         stream = nat.stream_ptr(self.dev)
-        for L, njobs, rows, jobs in self.filter_groups:
+        for L, code, njobs, rows, jobs in self.filter_groups:
             nat.check(self.lib.ath_ramp_filter_batched(
-                nat.ptr(jobs), njobs, rows, nat.ATH_F32, self.store_code, L,
+                nat.ptr(jobs), njobs, rows, code, self.store_code, L,
                 nat.PAD_MODES[self.pad_mode], int(self.apply_log), stream),
                 "ath_ramp_filter_batched")
-        nat.check(self.lib.ath_back_projection_batched(
-            nat.ptr(self.bp_jobs_dev), len(self._bp_jobs), self.total_tiles,
-            self.store_code, self.acc, self.slice_batch, self.tile_rows,
-            self.nblocks, nat.ptr(self.counter), stream),
-            "ath_back_projection_batched")
+        for g in self.bp_groups:
+            nat.check(self.lib.ath_back_projection_batched(
+                nat.ptr(g["jobs_dev"]), len(g["jobs"]), g["total"],
+                self.store_code, g["acc"], g["slice_batch"], self.tile_rows,
+                self.nblocks, nat.ptr(g["counter"]), stream),
+                "ath_back_projection_batched")
 
     def capture(self):
         """Record launch() in a HIP graph; run() then replays it."""
@@ -215,21 +241,28 @@ class BatchFBP:
             self.set_tile_rows(cfg[1])
             self.launch()
         best, timings = autotune.tune(
-            self.tune_key, [(self.slice_batch, r)
-                            for r in autotune.CANDIDATE_ROWS], run, persist)
+            self.tune_key, [(1, r) for r in autotune.CANDIDATE_ROWS], run,
+            persist)
         self.set_tile_rows(best[1])
         return best, timings
 
     def load(self, sinograms):
-        """Copy sinograms (numpy or tensors) into the input buffers; values
-        <= 0 are handled as in fbp_reconstruction when apply_log is set."""
+        """Copy sinograms (numpy or tensors) into the input buffers (float64
+        ones stay float64); values <= 0 are handled as in
+        fbp_reconstruction when apply_log is set."""
         import torch
         for i, s in enumerate(sinograms):
-            t = s if isinstance(s, torch.Tensor) else torch.from_numpy(
-                np.ascontiguousarray(s, dtype=np.float32))
-            self.inputs[i].copy_(t.reshape(self.inputs[i].shape))
+            buf = self.inputs[i]
+            if isinstance(s, torch.Tensor):
+                t = s
+            else:
+                a = np.asarray(s)
+                want = (np.float64 if buf.dtype == torch.float64
+                        else np.float32)
+                t = torch.from_numpy(np.ascontiguousarray(a, dtype=want))
+            buf.copy_(t.reshape(buf.shape))
             if self.apply_log:
-                rec._clamp_nonpositive_device(s, self.inputs[i])
+                rec._clamp_nonpositive_device(s, buf)
 
     def results(self, as_numpy=True):
         out = []
@@ -245,10 +278,12 @@ def fbp_reconstruction_batch(sinograms, centers, angles=None,
                              precision=None):
     """fbp_reconstruction over a list of sinograms of any shapes, in one
     batched execution.  Returns a list of numpy arrays."""
+    dtypes = [np.float64 if str(getattr(s, "dtype", "")).endswith("float64")
+              else np.float32 for s in sinograms]
     plan = BatchFBP([tuple(s.shape) for s in sinograms], centers, angles,
                     filter_name=filter_name, pad=pad, pad_mode=pad_mode,
                     apply_log=apply_log, ratio=ratio, gpu=gpu,
-                    precision=precision)
+                    precision=precision, dtypes=dtypes)
     plan.load(sinograms)
     plan.run()
     return plan.results()

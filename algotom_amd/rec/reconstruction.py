@@ -25,11 +25,13 @@ every compute entry raises ``RuntimeError``.
 
 import functools
 import math
+import threading
 import warnings
 
 import numpy as np
 from scipy.signal import windows as _windows
 
+from algotom_amd import _cache
 from algotom_amd import _host
 from algotom_amd import _native as nat
 from algotom_amd.rec import autotune
@@ -152,33 +154,30 @@ def _angles_device(angles, dev):
     return torch.from_numpy(a).to(dev)
 
 
-_plan_cache = {}
+_plan_cache = _cache.DeviceCache(32)
 
 
 def _filter_plan(taps, dev, fft_len):
     """Device filter plan for ``taps`` (nwin, W) on FFT length ``fft_len``;
-    cached per device."""
+    cached per device (thread-safe LRU of 32 plans)."""
     import torch
     nwin, width = taps.shape
     key = (dev.index, width, fft_len, taps.tobytes())
-    hit = _plan_cache.get(key)
-    if hit is not None:
-        return hit
-    lib = nat.load()
-    nbytes = lib.ath_filter_plan_bytes(fft_len, nwin)
-    if nbytes == 0:
-        raise ValueError("ramp filter width %d is too large" % width)
-    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    taps_d = torch.from_numpy(taps).to(dev)
-    nat.check(lib.ath_filter_plan_build(nat.ptr(taps_d), nwin, width, fft_len,
-                                        nat.ptr(plan), nbytes,
-                                        nat.stream_ptr(dev)),
-              "ath_filter_plan_build")
-    if len(_plan_cache) > 32:
-        _plan_cache.clear()
-    entry = (plan, nwin, taps_d)
-    _plan_cache[key] = entry
-    return entry
+
+    def build():
+        lib = nat.load()
+        nbytes = lib.ath_filter_plan_bytes(fft_len, nwin)
+        if nbytes == 0:
+            raise ValueError("ramp filter width %d is too large" % width)
+        plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        taps_d = torch.from_numpy(np.ascontiguousarray(taps)).to(dev)
+        nat.check(lib.ath_filter_plan_build(nat.ptr(taps_d), nwin, width,
+                                            fft_len, nat.ptr(plan), nbytes,
+                                            nat.stream_ptr(dev)),
+                  "ath_filter_plan_build")
+        return (plan, nwin, taps_d)
+
+    return _plan_cache.get(key, build, lambda e: (e[0], e[2]))
 
 
 def _window_taps(ramp_win, nrow, width, filter_name):
@@ -260,16 +259,21 @@ def _backproject_stack(filt_t, angles, center, acc, edge_pad=False,
     args = (nat.ptr(filt_t), nat.dtype_code(filt_t.dtype), nrow, nslice, ncol,
             nat.ptr(ang), float(np.float32(center)), int(bool(edge_pad)),
             int(acc), float(mask_radius), int(bool(apply_scale)), nat.ptr(out))
-    cfg = autotune.lookup(autotune.make_key(
-        "stack", acc, str(filt_t.dtype).split(".")[-1], ncol, nslice))
-    if cfg is None:
-        nat.check(lib.ath_back_projection(*args, nat.stream_ptr(dev)),
-                  "ath_back_projection")
-    else:
-        nat.check(lib.ath_back_projection_tiled(*args, int(cfg[0]),
-                                                int(cfg[1]),
-                                                nat.stream_ptr(dev)),
-                  "ath_back_projection_tiled")
+    wide = filt_t.dtype in (torch.float16, torch.bfloat16)
+    key = autotune.make_key("stack", acc, str(filt_t.dtype).split(".")[-1],
+                            ncol, nslice)
+    cfg = autotune.lookup(key, autotune.compiled_configs("stack", wide))
+    if cfg is not None:
+        st = lib.ath_back_projection_tiled(*args, int(cfg[0]), int(cfg[1]),
+                                           nat.stream_ptr(dev))
+        if st != nat.ATH_ERR_UNSUPPORTED:
+            nat.check(st, "ath_back_projection_tiled")
+            return out
+        # a tuned configuration this library does not compile: forget it
+        # and use the built-in default
+        autotune.forget(key)
+    nat.check(lib.ath_back_projection(*args, nat.stream_ptr(dev)),
+              "ath_back_projection")
     return out
 
 
@@ -551,7 +555,7 @@ def _circular_window(width, filter_name):
     return win
 
 
-_circ_cache = {}
+_circ_cache = _cache.DeviceCache(1)
 
 
 def _circular_plan(ramp_win, dev, name_key=None):
@@ -570,37 +574,45 @@ def _circular_plan(ramp_win, dev, name_key=None):
         key = (dev.index, width) + tuple(name_key)
     else:
         key = (dev.index, width, hashlib.sha1(win.tobytes()).hexdigest())
-    hit = _circ_cache.get(key)
-    if hit is not None:
-        return hit
-    g2 = np.ascontiguousarray(np.real(np.fft.ifft2(np.fft.ifftshift(win))))
-    entry = {"width": width, "g2": g2, "plans": {}}
-    _circ_cache.clear()
-    _circ_cache[key] = entry
-    return entry
+
+    def build():
+        g2 = np.ascontiguousarray(np.real(np.fft.ifft2(
+            np.fft.ifftshift(win))))
+        return {"width": width, "g2": g2, "plans": {},
+                "lock": threading.Lock()}
+
+    return _circ_cache.get(key, build)
 
 
 def _circular_plan_for(entry, fft_len, dev):
     import torch
-    plan = entry["plans"].get(fft_len)
-    if plan is not None:
+    with entry["lock"]:
+        plan = entry["plans"].get(fft_len)
+        if plan is not None:
+            home = entry["homes"][fft_len]
+            cur = torch.cuda.current_stream(dev)
+            if cur.cuda_stream != home:
+                plan.record_stream(cur)
+            return plan
+        lib = nat.load()
+        width = entry["width"]
+        nbytes = lib.ath_circular_filter_plan_bytes(fft_len)
+        nwork = lib.ath_circular_filter_plan_work_bytes(width, fft_len)
+        if nbytes == 0 or nwork == 0:
+            raise NotImplementedError(
+                "the circular ramp filter supports FFT lengths up to 16384 "
+                "(width %d needs %d)" % (width, fft_len))
+        plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        work = torch.empty(nwork, dtype=torch.uint8, device=dev)
+        g2_d = torch.from_numpy(entry["g2"]).to(dev)
+        nat.check(lib.ath_circular_filter_plan_build(
+            nat.ptr(g2_d), width, fft_len, nat.ptr(plan), nbytes,
+            nat.ptr(work), nwork, nat.stream_ptr(dev)),
+            "ath_circular_filter_plan_build")
+        entry["plans"][fft_len] = plan
+        entry.setdefault("homes", {})[fft_len] = \
+            torch.cuda.current_stream(dev).cuda_stream
         return plan
-    lib = nat.load()
-    width = entry["width"]
-    nbytes = lib.ath_circular_filter_plan_bytes(fft_len)
-    nwork = lib.ath_circular_filter_plan_work_bytes(width, fft_len)
-    if nbytes == 0 or nwork == 0:
-        raise NotImplementedError(
-            "the circular ramp filter supports FFT lengths up to 16384 "
-            "(width %d needs %d)" % (width, fft_len))
-    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    work = torch.empty(nwork, dtype=torch.uint8, device=dev)
-    g2_d = torch.from_numpy(entry["g2"]).to(dev)
-    nat.check(lib.ath_circular_filter_plan_build(
-        nat.ptr(g2_d), width, fft_len, nat.ptr(plan), nbytes, nat.ptr(work),
-        nwork, nat.stream_ptr(dev)), "ath_circular_filter_plan_build")
-    entry["plans"][fft_len] = plan
-    return plan
 
 
 _CIRC_WORK_BYTES = 4 << 30
@@ -782,37 +794,36 @@ def bpf_reconstruction(sinogram, center, angles=None, ratio=1.0,
 # --------------------------------------------------------------------------
 # DFI (direct Fourier inversion), reconstruction.py:687-907
 # --------------------------------------------------------------------------
-_dfi_plans = {}
+_dfi_plans = _cache.DeviceCache(8)
 
 
 def _dfi_plan(width, filter_name, dev):
     """Device plan of ath_dfi_reconstruction for padded width ``width``:
     Bluestein tables, the smoothing window (reconstruction.py:875-878) and
-    the B-spline prefilter taps; a few are kept per device."""
+    the B-spline prefilter taps; a few are kept per device (thread-safe
+    LRU)."""
     import torch
     key = (dev.index, width, filter_name)
-    plan = _dfi_plans.get(key)
-    if plan is not None:
+
+    def build():
+        lib = nat.load()
+        nbytes = lib.ath_dfi_plan_bytes(width)
+        if nbytes == 0:
+            raise NotImplementedError(
+                "dfi_reconstruction on the GPU supports padded widths "
+                "ncol + 2*int(pad_rate*ncol) up to 8191 (got %d)" % width)
+        plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        win = None
+        if filter_name is not None:
+            win = torch.from_numpy(np.ascontiguousarray(
+                make_smoothing_window(filter_name, width),
+                dtype=np.float64)).to(dev)
+        nat.check(lib.ath_dfi_plan_build(
+            nat.ptr(win) if win is not None else None, width, nat.ptr(plan),
+            nbytes, nat.stream_ptr(dev)), "ath_dfi_plan_build")
         return plan
-    lib = nat.load()
-    nbytes = lib.ath_dfi_plan_bytes(width)
-    if nbytes == 0:
-        raise NotImplementedError(
-            "dfi_reconstruction on the GPU supports padded widths "
-            "ncol + 2*int(pad_rate*ncol) up to 8191 (got %d)" % width)
-    plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    win = None
-    if filter_name is not None:
-        win = torch.from_numpy(np.ascontiguousarray(
-            make_smoothing_window(filter_name, width),
-            dtype=np.float64)).to(dev)
-    nat.check(lib.ath_dfi_plan_build(
-        nat.ptr(win) if win is not None else None, width, nat.ptr(plan),
-        nbytes, nat.stream_ptr(dev)), "ath_dfi_plan_build")
-    if len(_dfi_plans) >= 8:
-        _dfi_plans.clear()
-    _dfi_plans[key] = plan
-    return plan
+
+    return _dfi_plans.get(key, build, lambda p: (p,))
 
 
 def _dfi_angles(angles, nrow):

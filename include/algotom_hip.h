@@ -225,30 +225,48 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                             int pad, int pad_mode, const void* plan,
                             int fft_len, int nwin, float* out, void* stream);
 
-/* 360-degree -> 180-degree conversion of a half-acquisition (offset axis)
- * sinogram stack: rows [0, nrow/2] and the left-right flipped last
- * nrow/2 + 1 rows stitched with a linear ramp over an INTEGER `overlap`
- * (side 0: overlap on the left of the first half, 1: on the right).
- * norm != 0 scales the second half by the ratio of the overlap means.
- * sino360: (nrow, nslice, ncol); out: (nrow/2 + 1, nslice, width) with
- * width >= 2*ncol - overlap (extra columns repeat the edge); factors:
- * device scratch of 2*nslice floats.  Replaces
- * convert_sinogram_360_to_180 / stitch_image (algotom/prep/conversion.py
- * :318-364, :76-144) for integer overlaps. */
-int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
-                                    int nslice, int ncol, int overlap,
-                                    int side, int norm, int width,
-                                    float* factors, float* out, void* stream);
+/* Device workspace bytes of a conversion: op 0 = ath_convert_sinogram_
+ * 360_to_180 (the overlap means, plus two shifted images when the overlap
+ * has a sub-pixel part), op 1 = ath_extend_sinogram (one shifted copy of
+ * the input for a sub-pixel overlap, else 0).  0 for invalid arguments
+ * (op 0 never needs 0 bytes). */
+size_t ath_conversion_workspace_bytes(int op, int nrow, int nslice, int ncol,
+                                      double overlap);
 
-/* extend_sinogram (algotom/prep/conversion.py:395-454) for an integer
- * overlap: sino (nrow, nslice, ncol) float32 -> out (nrow, nslice,
- * 2*ncol - overlap) float32: optional -log, the overlap columns weighted
- * by numpy.linspace(1, 0, overlap) (side 1, last columns) or
- * linspace(0, 1, overlap) (side 0, first columns), edge padding on the
- * other side, times 2. */
+/* 360-degree -> 180-degree conversion of a half-acquisition (offset axis)
+ * sinogram stack: rows [0, nrow/2] (image 1) and the left-right flipped
+ * last nrow/2 + 1 rows (image 2) stitched over `overlap` columns (side 0:
+ * overlap on the left of image 1, 1: on the right).  overlap_int =
+ * round(overlap + 1e-3); a sub-pixel remainder shifts image 1 by it (side
+ * 1; its negative on side 0) and image 2 by the opposite amount with
+ * scipy.ndimage.shift's cubic spline (mode "nearest") before stitching.
+ * norm != 0 scales image 2 by the ratio of the overlap means.  wei1/wei2:
+ * caller weight matrices (nrow/2 + 1, ncol) of wei_dtype ATH_F32 or ATH_F64
+ * shared by every slice, or both NULL for the linear ramp of
+ * make_weight_matrix.  sino360: (nrow, nslice, ncol); out: (nrow/2 + 1,
+ * nslice, width) with width >= 2*ncol - overlap_int (extra columns repeat
+ * the edge); work: ath_conversion_workspace_bytes(0, ...) bytes.  Replaces
+ * convert_sinogram_360_to_180 / stitch_image / make_weight_matrix
+ * (algotom/prep/conversion.py:318-364, :76-144, :45-73). */
+int ath_convert_sinogram_360_to_180(const float* sino360, int nrow,
+                                    int nslice, int ncol, double overlap,
+                                    int side, int norm, int width,
+                                    const void* wei1, const void* wei2,
+                                    int wei_dtype, void* work,
+                                    size_t work_bytes, float* out,
+                                    void* stream);
+
+/* extend_sinogram (algotom/prep/conversion.py:395-454): sino (nrow,
+ * nslice, ncol) float32 -> out (nrow, nslice, 2*ncol - overlap_int)
+ * float32: optional -log, the sub-pixel shift (scipy.ndimage.shift, cubic,
+ * mode "nearest") by overlap - overlap_int (side 1; its negative on side
+ * 0), the overlap columns weighted by numpy.linspace(1, 0, overlap_int)
+ * (side 1, last columns) or linspace(0, 1, overlap_int) (side 0, first
+ * columns), edge padding on the other side, times 2.  work:
+ * ath_conversion_workspace_bytes(1, ...) bytes (NULL when that is 0). */
 int ath_extend_sinogram(const float* sino, int nrow, int nslice, int ncol,
-                        int overlap, int side, int apply_log, float* out,
-                        void* stream);
+                        double overlap, int side, int apply_log, void* work,
+                        size_t work_bytes, float* out, void* stream);
 
 /* flat_field_correction (algotom/prep/correction.py:45-98) without its
  * artifact-removal options: out = (float32(proj) - dark) / denom, or

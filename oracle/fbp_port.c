@@ -12,6 +12,10 @@
  *                       real part, crop) as the equivalent circular
  *                       convolution, evaluated with a power-of-two FFT
  *   orc_trig_tables  <- reconstruction.py:169-171 / :334-336
+ *   orc_forward_project  the transpose of orc_backproject's float64-sum
+ *                       form (no reference counterpart: the adjoint that
+ *                       differentiable FBP needs), as a scatter over
+ *                       pixels per angle
  *
  * Deterministic and race-free: OpenMP runs over image rows, each pixel sums
  * its angles in ascending order (the reference's prange over angles at
@@ -118,6 +122,59 @@ void orc_backproject(const float* sino, int nrow, int nslice, int ncol,
             }
         free(a64);
         free(a32);
+    }
+}
+
+/*
+ * Transpose of the back-projection: for every angle i, pixel (x, y) with
+ * f = (x - ic) c + (y - ic) s + center in [0, ncol-1], d = floor f,
+ * t = f - d, the masked pixel value v adds (1 - t) v to bin d and t v to
+ * bin d + 1 (when d + 1 < ncol) of row i -- the weights the
+ * back-projector gives s[d] and s[d+1] (reconstruction.py:170-181).
+ * image (ncol, nslice, ncol) float32, sino out (nrow, nslice, ncol)
+ * float32 = scale * sum (float64 sums over the pixels in row-major order).
+ * Parallel over angles: each angle owns its output row.
+ */
+void orc_forward_project(const float* image, int nrow, int nslice, int ncol,
+                         const float* cos_t, const float* sin_t, float center,
+                         double mask_radius, double scale, float* sino,
+                         int nthreads) {
+    set_threads(nthreads);
+    const int w1 = ncol - 1;
+    const double icenter = 0.5 * (double)w1;
+    const double cen = (double)center;
+    const int mc = ncol / 2;
+    const double r2 = mask_radius * mask_radius;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nrow; ++i) {
+        double* acc = (double*)calloc((size_t)nslice * ncol, sizeof(double));
+        const double c = (double)cos_t[i];
+        const double s = (double)sin_t[i];
+        for (int y = 0; y < ncol; ++y) {
+            const double ys = ((double)y - icenter) * s;
+            for (int x = 0; x < ncol; ++x) {
+                if (mask_radius >= 0.0) {
+                    const long long dx = x - mc, dy = y - mc;
+                    if (!((double)(dx * dx + dy * dy) <= r2)) continue;
+                }
+                const double f = (((double)x - icenter) * c + ys) + cen;
+                if (!(f >= 0.0 && f <= (double)w1)) continue;
+                const double fl = floor(f);
+                const int d = (int)fl;
+                const double t = f - fl;
+                for (int j = 0; j < nslice; ++j) {
+                    const double v =
+                        (double)image[((long long)y * nslice + j) * ncol + x];
+                    acc[(long long)j * ncol + d] += (1.0 - t) * v;
+                    if (d + 1 <= w1) acc[(long long)j * ncol + d + 1] += t * v;
+                }
+            }
+        }
+        for (int j = 0; j < nslice; ++j)
+            for (int k = 0; k < ncol; ++k)
+                sino[((long long)i * nslice + j) * ncol + k] =
+                    (float)(acc[(long long)j * ncol + k] * scale);
+        free(acc);
     }
 }
 

@@ -41,6 +41,9 @@ def lib():
         _lib.orc_finalize.argtypes = [p, i, i, i, ctypes.c_double, i]
         _lib.orc_backproject_f64.argtypes = [p, i, i, i, p, p,
                                              ctypes.c_double, p, i]
+        _lib.orc_forward_project.argtypes = [p, i, i, i, p, p,
+                                             ctypes.c_float, ctypes.c_double,
+                                             ctypes.c_double, p, i]
     return _lib
 
 
@@ -74,6 +77,26 @@ def backproject(sino, angles, center, edge_pad=False, acc=fbp_np.ACC_F32SEQ,
     lib().orc_backproject(_ptr(s), nrow, nslice, ncol, _ptr(c), _ptr(sn),
                           float(np.float32(center)), int(bool(edge_pad)),
                           int(acc), _ptr(out), nthreads or default_threads())
+    return out[:, 0, :] if two_d else out
+
+
+def forward_project(image, angles, center, mask_radius=-1.0, scale=1.0,
+                    nthreads=None):
+    """Transpose of backproject's float64-sum form: image (ncol, ncol) or
+    (ncol, nslice, ncol) float32 -> sinogram (nrow, ncol) or (nrow, nslice,
+    ncol) float32; mask_radius < 0: no mask."""
+    im = np.ascontiguousarray(image, dtype=np.float32)
+    two_d = im.ndim == 2
+    if two_d:
+        im = im[:, None, :]
+    ncol, nslice, _ = im.shape
+    c, sn = trig_tables(angles)
+    nrow = c.size
+    out = np.empty((nrow, nslice, ncol), dtype=np.float32)
+    lib().orc_forward_project(_ptr(im), nrow, nslice, ncol, _ptr(c), _ptr(sn),
+                              float(np.float32(center)), float(mask_radius),
+                              float(scale), _ptr(out),
+                              nthreads or default_threads())
     return out[:, 0, :] if two_d else out
 
 

@@ -8,7 +8,9 @@
 * dL/dcenter == the FP64 central finite difference of the FP64 oracle loss
   on a 128-angle x 64-column case (h = 1e-6), and the derivative image
   matches FP64 finite differences of the back-projection away from kinks;
-* both checkpoint modes give identical gradients.
+* both checkpoint modes give identical gradients;
+* every check runs on a full 180-degree scan and on the limited 120-degree
+  arc of the refinement workload (angles linspace(0, 2 pi / 3, n)).
 """
 
 import numpy as np
@@ -17,19 +19,24 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+ARCS = [np.pi, 2 * np.pi / 3]
+
+
 def _setup(nrow=128, ncol=64, nslice=1, center=None, seed=0, arc=np.pi):
     rng = np.random.default_rng(seed)
-    angles = np.linspace(0.0, arc, nrow, endpoint=False)
+    # a full scan over [0, pi); the limited arc includes its end angle
+    angles = np.linspace(0.0, arc, nrow, endpoint=arc != np.pi)
     center = (ncol - 1) / 2 + 0.37 if center is None else center
     sino = rng.standard_normal((nrow, nslice, ncol)).astype(np.float32)
     return rng, angles, center, sino
 
 
-def test_forward_projection_is_dense_transpose():
+@pytest.mark.parametrize("arc", ARCS)
+def test_forward_projection_is_dense_transpose(arc):
     import torch
     from oracle import fbp_np
     from algotom_amd.rec import autograd
-    rng, angles, center, _ = _setup(nrow=40, ncol=33)
+    rng, angles, center, _ = _setup(nrow=40, ncol=33, arc=arc)
     geom = autograd.FBPGeometry(40, 33, angles, ratio=None)
     img = rng.standard_normal((33, 2, 33)).astype(np.float32)
     got = geom.forward_project(torch.from_numpy(img).cuda(), center)
@@ -40,10 +47,11 @@ def test_forward_projection_is_dense_transpose():
         assert np.abs(got[:, j, :] - ref).max() <= 1e-6 * np.abs(ref).max()
 
 
-def test_forward_projection_adjoint_identity_with_mask():
+@pytest.mark.parametrize("arc", ARCS)
+def test_forward_projection_adjoint_identity_with_mask(arc):
     import torch
-    from algotom_amd.rec import autograd, reconstruction as rec
-    rng, angles, center, sino = _setup(nrow=90, ncol=120, nslice=3)
+    from algotom_amd.rec import autograd
+    rng, angles, center, sino = _setup(nrow=90, ncol=120, nslice=3, arc=arc)
     geom = autograd.FBPGeometry(90, 120, angles, ratio=1.0)
     s = torch.from_numpy(sino).cuda()
     x = torch.from_numpy(rng.standard_normal((120, 3, 120))
@@ -87,12 +95,13 @@ def _fp64_loss(sino, center, angles, target, ratio=1.0):
     return float(np.sum((y - target) ** 2)), y
 
 
+@pytest.mark.parametrize("arc", ARCS)
 @pytest.mark.parametrize("checkpoint", ["filtered", "input"])
-def test_gradients_vs_dense_and_finite_differences(checkpoint):
+def test_gradients_vs_dense_and_finite_differences(checkpoint, arc):
     import torch
     from oracle import fbp_np
     from algotom_amd.rec import autograd
-    rng, angles, center, sino = _setup(nrow=128, ncol=64)
+    rng, angles, center, sino = _setup(nrow=128, ncol=64, arc=arc)
     sino2 = sino[:, 0, :]
     target = rng.standard_normal((64, 64))
     geom = autograd.FBPGeometry(128, 64, angles, ratio=1.0,
@@ -127,11 +136,12 @@ def test_gradients_vs_dense_and_finite_differences(checkpoint):
     assert abs(g_c - fd) <= 1e-3 * abs(fd) + 1e-6
 
 
-def test_center_derivative_image_vs_fd():
+@pytest.mark.parametrize("arc", ARCS)
+def test_center_derivative_image_vs_fd(arc):
     import torch
     from oracle import fbp_np
     from algotom_amd.rec import autograd
-    rng, angles, center, sino = _setup(nrow=60, ncol=48, seed=5)
+    rng, angles, center, sino = _setup(nrow=60, ncol=48, seed=5, arc=arc)
     geom = autograd.FBPGeometry(60, 48, angles, ratio=None)
     filt = geom.filter(torch.from_numpy(sino).cuda())
     d_img = geom.backproject(filt, center, acc=3).cpu().numpy()[:, 0, :]

@@ -25,6 +25,11 @@ CASES = [
     (96, 40, 4, 3.0, 2 * np.pi, False),
     (96, 40, 1, 94.9, np.pi, True),
     (512, 64, 1, 256.0, np.pi, False),
+    # the widths of the uneven batch and of the stitched volume, with few
+    # angles: the largest staging spans and 64-bit offsets
+    (4096, 23, 1, 2047.5 + 13.7, np.pi, False),
+    (4096, 17, 3, 2031.25, np.pi, True),
+    (7192, 11, 2, 3595.5, 2 * np.pi, False),
 ]
 
 

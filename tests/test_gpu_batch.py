@@ -41,23 +41,25 @@ def test_batch_matches_single_calls(precision):
         assert np.array_equal(g, ref), (s.shape, c)
 
 
-def test_batch_stacks_and_2d_mixed():
+@pytest.mark.parametrize("gpu", [True, False])
+def test_batch_stacks_and_2d_mixed(gpu):
+    # each member gets fbp_reconstruction's own arithmetic: float64 sums for
+    # a single sinogram with gpu=True, float32 per angle for stacks; a
+    # float64 sinogram stays float64 through the log and the filter
     from algotom_amd.rec import batch, reconstruction as rec
     rng = np.random.default_rng(2)
     sinos = [rng.uniform(0.1, 1, (40, 3, 70)).astype(np.float32),
              rng.uniform(0.1, 1, (25, 90)).astype(np.float32),
-             rng.uniform(0.1, 1, (60, 5, 64)).astype(np.float32)]
-    centers = [34.2, 44.0, 31.5]
-    plan = batch.BatchFBP([s.shape for s in sinos], centers, apply_log=True)
-    plan.load(sinos)
-    plan.run()
-    for s, c, g in zip(sinos, centers, plan.results()):
-        ref = rec.fbp_reconstruction(s.copy(), c, apply_log=True, gpu=True)
-        if s.ndim == 2:
-            # stacks force the float32-per-angle arithmetic for the batch
-            ref = rec.fbp_reconstruction(s.copy(), c, apply_log=True,
-                                         gpu=False)
-        assert np.array_equal(g, ref)
+             rng.uniform(0.1, 1, (60, 5, 64)).astype(np.float32),
+             rng.uniform(0.1, 1, (33, 81)),
+             rng.uniform(0.1, 1, (33, 2, 81))]
+    centers = [34.2, 44.0, 31.5, 40.3, 39.6]
+    got = batch.fbp_reconstruction_batch([s.copy() for s in sinos], centers,
+                                         apply_log=True, gpu=gpu)
+    for s, c, g in zip(sinos, centers, got):
+        ref = rec.fbp_reconstruction(s.copy(), c, apply_log=True, gpu=gpu)
+        assert g.shape == ref.shape
+        assert np.array_equal(g, ref), (s.shape, s.dtype, gpu)
 
 
 def test_graph_replays_match_eager():

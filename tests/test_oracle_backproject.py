@@ -103,6 +103,34 @@ def test_adjointness_dense_matrix():
     assert np.abs(b32 - bt).max() <= 1e-5 * np.abs(bt).max()
 
 
+@pytest.mark.parametrize("arc,center,mask", [(np.pi, 6.3, -1.0),
+                                             (2 * np.pi / 3, 5.0, 6.0),
+                                             (2 * np.pi, 7.9, 4.5)])
+def test_port_forward_projection_is_dense_transpose(arc, center, mask):
+    # the port's scatter forward projection against the transpose of the
+    # back-projection matrix assembled column by column from the port's
+    # float64-sum back-projection of unit sinograms
+    rng = np.random.default_rng(12)
+    ncol, nrow = 13, 7
+    ang = np.linspace(0.0, arc, nrow)
+    cols = []
+    for i in range(nrow):
+        for d in range(ncol):
+            e = np.zeros((nrow, ncol), np.float32)
+            e[i, d] = 1.0
+            cols.append(port.backproject(e, ang, center,
+                                         acc=fbp_np.ACC_F64).ravel())
+    b = np.array(cols, dtype=np.float64).T          # (ncol^2, nrow*ncol)
+    x = rng.standard_normal((ncol, ncol)).astype(np.float32)
+    xm = x.astype(np.float64)
+    if mask >= 0:
+        yy, xx = np.mgrid[:ncol, :ncol] - ncol // 2
+        xm = xm * ((xx * xx + yy * yy) <= mask * mask)
+    exp = (b.T @ xm.ravel()).reshape(nrow, ncol) * 0.5
+    got = port.forward_project(x, ang, center, mask_radius=mask, scale=0.5)
+    assert np.abs(got - exp).max() <= 1e-6 * np.abs(exp).max()
+
+
 def test_mirror_symmetry_180():
     # projecting at theta + pi reverses the detector axis about the centre
     rng = np.random.default_rng(10)
