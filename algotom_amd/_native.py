@@ -19,6 +19,7 @@ ATH_ERR_HIP = -2
 ATH_ERR_UNSUPPORTED = -3
 ATH_ERR_NO_DEVICE = -4
 ATH_ERR_WORKSPACE = -5
+FBP_NOALLOC_MAX_BYTES = 64 << 20  # ATH_FBP_NOALLOC_MAX_BYTES
 
 ATH_F32, ATH_F64, ATH_F16, ATH_BF16, ATH_U16 = 0, 1, 2, 3, 4
 VERT_GPU, VERT_CPU = 0, 1
@@ -97,6 +98,9 @@ SIGNATURES = (
     ("ath_fbp_workspace_bytes", _S, [_I, _I, _I, _I, _I, _I]),
     ("ath_fbp_reconstruction", _I, [_P, _I, _I, _I, _I, _P, _F, _D, _P, _I,
                                     _I, _I, _I, _I, _I, _P, _S, _P, _P]),
+    ("ath_fbp_reconstruction_noalloc", _I, [_P, _I, _I, _I, _I, _P, _F, _D,
+                                            _P, _I, _I, _I, _I, _I, _I, _P,
+                                            _P]),
 )
 
 

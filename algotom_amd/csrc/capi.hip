@@ -192,6 +192,42 @@ int fbp_into_workspace(const void* sino, int in_dtype, int nrow, int nslice,
     return back_projection(a, filt_dtype, acc, 0, 0, s);
 }
 
+namespace {
+// Workspace of ath_fbp_reconstruction_noalloc: one copy per GPU, loaded with
+// the library's code object and held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host it names a stand-in,
+// not device memory): it uses the address hipGetSymbolAddress returns on
+// each device.
+__device__ __align__(256) unsigned char scratch[ATH_FBP_NOALLOC_MAX_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st != ATH_OK) return st;
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(ATH_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        st = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                       "hipGetSymbolAddress(scratch)");
+        if (st != ATH_OK) return st;
+        if (!p)
+            return set_error(ATH_ERR_HIP,
+                             "hipGetSymbolAddress(scratch) gave no address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ATH_OK;
+}
+
 }  // namespace ath
 
 extern "C" {
@@ -905,6 +941,34 @@ int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
                                    center, mask_radius, taps, nwin, pad,
                                    pad_mode, apply_log, acc, filt_dtype,
                                    workspace, workspace_bytes, recon,
+                                   static_cast<hipStream_t>(stream));
+}
+
+int ath_fbp_reconstruction_noalloc(const void* sino, int in_dtype, int nrow,
+                                   int nslice, int ncol, const float* angles,
+                                   float center, double mask_radius,
+                                   const double* taps, int nwin, int pad,
+                                   int pad_mode, int apply_log, int acc,
+                                   int filt_dtype, float* recon,
+                                   void* stream) {
+    clear_error();
+    const size_t need = ath_fbp_workspace_bytes(nrow, nslice, ncol, pad, nwin,
+                                                filt_dtype);
+    if (need == 0)
+        return set_error(ATH_ERR_INVALID_ARG, "bad geometry or dtype");
+    if (need > ATH_FBP_NOALLOC_MAX_BYTES)
+        return set_error(ATH_ERR_WORKSPACE,
+                         "ath_fbp_reconstruction_noalloc: this geometry needs " +
+                             std::to_string(need) +
+                             " workspace bytes, above the fixed " +
+                             std::to_string(ATH_FBP_NOALLOC_MAX_BYTES));
+    void* ws = nullptr;
+    const int st = ath::noalloc_scratch(&ws);
+    if (st != ATH_OK) return st;
+    return ath::fbp_into_workspace(sino, in_dtype, nrow, nslice, ncol, angles,
+                                   center, mask_radius, taps, nwin, pad,
+                                   pad_mode, apply_log, acc, filt_dtype, ws,
+                                   ATH_FBP_NOALLOC_MAX_BYTES, recon,
                                    static_cast<hipStream_t>(stream));
 }
 

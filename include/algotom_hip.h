@@ -504,6 +504,29 @@ int ath_fbp_reconstruction(const void* sino, int in_dtype, int nrow,
                            size_t workspace_bytes, float* recon,
                            void* stream);
 
+/* Largest workspace ath_fbp_reconstruction_noalloc can use (64 MiB): e.g.
+ * one 3601 x 4096 float32 sinogram with its plan, or a 1801 x 8 x 1024
+ * stack. */
+#define ATH_FBP_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+
+/* ath_fbp_reconstruction with no workspace argument; it allocates nothing.
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a fixed device array of
+ * ATH_FBP_NOALLOC_MAX_BYTES in the library, one per GPU, held until the
+ * process ends; a call whose ath_fbp_workspace_bytes(...) exceeds it
+ * returns ATH_ERR_WORKSPACE.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (other streams or threads) race; calls
+ * ordered on one stream do not.  Each call writes every workspace byte it
+ * reads, so nothing carries over from the previous call. */
+int ath_fbp_reconstruction_noalloc(const void* sino, int in_dtype, int nrow,
+                                   int nslice, int ncol, const float* angles,
+                                   float center, double mask_radius,
+                                   const double* taps, int nwin, int pad,
+                                   int pad_mode, int apply_log, int acc,
+                                   int filt_dtype, float* recon,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

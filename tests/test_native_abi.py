@@ -31,6 +31,24 @@ def test_library_exports_every_declared_symbol():
     assert bound == names
 
 
+def test_noalloc_entry_rejects_geometry_above_its_array():
+    """The allocation-free fused entry refuses, before any device work, a
+    geometry whose workspace exceeds its fixed array."""
+    lib = nat.load()
+    # 1801 x 8 x 2560 float32 filtered: 147 MB > 64 MiB
+    need = lib.ath_fbp_workspace_bytes(1801, 8, 2560, 150, 1, nat.ATH_F32)
+    assert need > nat.FBP_NOALLOC_MAX_BYTES
+    st = lib.ath_fbp_reconstruction_noalloc(
+        None, nat.ATH_F32, 1801, 8, 2560, None, 1280.0, -1.0, None, 1, 150,
+        0, 1, nat.ACC_F32SEQ, nat.ATH_F32, None, None)
+    assert st == nat.ATH_ERR_WORKSPACE
+    status, msg = nat.last_error()
+    assert status == st and "workspace bytes" in msg
+    # just inside the array: 1801 x 2 x 1024 passes the size check
+    assert lib.ath_fbp_workspace_bytes(1801, 2, 1024, 150, 1,
+                                       nat.ATH_F32) <= nat.FBP_NOALLOC_MAX_BYTES
+
+
 def test_version_and_errors_without_device():
     lib = nat.load()
     assert lib.ath_version() >= 100
