@@ -26,6 +26,10 @@ VERT_GPU, VERT_CPU = 0, 1
 DFI_NO_ANGLES, DFI_ANGLES, DFI_FOLD_360 = 0, 1, 2
 PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
              "wrap": 4}
+# every numpy.pad mode the 1D ramp filter takes: the index-map modes above
+# and the value modes (numpy's defaults: end value 0, whole-row statistics)
+FILTER_PAD_MODES = dict(PAD_MODES, linear_ramp=5, maximum=6, mean=7,
+                        median=8, minimum=9)
 ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV = 0, 1, 2, 3
 
 # (name, restype, argtypes) of every exported entry point

@@ -31,8 +31,8 @@
 //   ATH_ACC_F32SEQ  record float32 (s, ds); val in float64, sum stored float32
 //   ATH_ACC_F32     record float32 (s, ds); one FMA and one add in float32
 //   kAccH16/kAccB16 float16/bfloat16 storage: record (s, ds) as a 16-bit pair,
-//                   weight pair (1, w); one v_dot2c_f32_{f16,bf16} per update
-//                   accumulating in float32.
+//                   weight pair (1, w) with w rounded to nearest; one
+//                   v_dot2c_f32_{f16,bf16} per update accumulating in float32.
 #include "ath_internal.h"
 
 namespace ath {
@@ -45,6 +45,7 @@ constexpr int kAccB16 = 17;
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
 
 constexpr int isqrt_ceil(int v) {
     int r = 0;
@@ -63,6 +64,15 @@ constexpr int bp_span(int tx, int ty) {
 template <class M>
 constexpr int mode_span(int tx, int ty) {
     return bp_span(tx, ty) + (M::kF32Pos ? 2 : 0);
+}
+
+// floor(f) as an int in one instruction (f within int range); with
+// __builtin_amdgcn_fractf(f) == f - floorf(f) (exact for |f| < 2^23) it
+// replaces floor, convert and subtract
+__device__ __forceinline__ int floor_to_int(float f) {
+    int d;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(d) : "v"(f));
+    return d;
 }
 
 // detector position exactly as the reference evaluates it
@@ -175,9 +185,10 @@ struct Mode<kAccH16> {
         return weight32((float)fr);
     }
     static __device__ __forceinline__ W weight32(float fr) {
-        // (1, w): 1 is exact; w rounds toward zero (one v_cvt_pkrtz)
-        const auto p = __builtin_amdgcn_cvt_pkrtz(1.0f, fr);
-        return __builtin_bit_cast(W, p);
+        // (1, w) in one v_cvt_pk_f16_f32: 1 is exact; w rounds to
+        // nearest (rounding toward zero would pull every tap toward s[d]:
+        // C1 error vs float64 2.9e-5 with nearest, 5.3e-5 toward zero)
+        return __builtin_convertvector(float2_t{1.0f, fr}, W);
     }
     static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
         a = __builtin_amdgcn_fdot2(q, w, a, false);
@@ -202,7 +213,10 @@ struct Mode<kAccB16> {
         return weight32((float)fr);
     }
     static __device__ __forceinline__ W weight32(float fr) {
-        return W{(__bf16)1.0f, (__bf16)fr};
+        // (1, w) in one v_cvt_pk_bf16_f32, w rounded to nearest; the
+        // bfloat16 samples dominate the error (C1 vs float64: 2.71e-4 with
+        // an exact weight, 2.82e-4 with this one)
+        return __builtin_convertvector(float2_t{1.0f, fr}, W);
     }
     static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
         a = __builtin_amdgcn_fdot2_f32_bf16(q, w, a, false);
@@ -248,9 +262,8 @@ __device__ __forceinline__ void inner_angle(
 #pragma unroll
         for (int k = 0; k < PY; ++k) {
             const float f = fmaf((float)yc[k], s32, xcc);
-            const float fl = floorf(f);
-            const int d = (int)fl;
-            const auto w = M::weight32(f - fl);
+            const int d = floor_to_int(f);
+            const auto w = M::weight32(__builtin_amdgcn_fractf(f));
             const Rec* q = qbase + d * SB;
 #pragma unroll
             for (int js = 0; js < SB; ++js) M::add(acc[k][js], q[js], w);

@@ -325,7 +325,7 @@ int ath_ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
     int st = check_geometry(nrow, nslice, ncol);
     if (st != ATH_OK) return st;
     if (pad < 0) return set_error(ATH_ERR_INVALID_ARG, "pad must be >= 0");
-    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_MINIMUM)
         return set_error(ATH_ERR_INVALID_ARG, "unsupported pad mode");
     if (!valid_dtype(out_dtype) ||
         (in_dtype != ATH_F32 && in_dtype != ATH_F64))
@@ -411,7 +411,7 @@ int ath_ramp_filter_batched(const ath_filter_job* jobs, int njobs,
     if (njobs < 1 || total_rows < 1 || total_rows > 0x7fffffffLL)
         return set_error(ATH_ERR_INVALID_ARG,
                          "need 1 <= njobs and 1 <= total_rows < 2^31");
-    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_WRAP)
+    if (pad_mode < ATH_PAD_EDGE || pad_mode > ATH_PAD_MINIMUM)
         return set_error(ATH_ERR_INVALID_ARG, "unsupported pad mode");
     if (!valid_dtype(out_dtype) ||
         (in_dtype != ATH_F32 && in_dtype != ATH_F64))

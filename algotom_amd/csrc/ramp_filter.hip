@@ -48,6 +48,136 @@ __device__ __forceinline__ double load_sample<double>(const double* row,
     return apply_log ? -log(x) : x;
 }
 
+// ---- numpy.pad's value modes (ATH_PAD_LINEAR_RAMP .. ATH_PAD_MINIMUM) ----
+// With numpy's defaults (end_values 0, stat_length None) every row needs
+// two numbers, computed by the row's workgroup before it fills the padded
+// row: the pad value of the statistic modes (the same on both sides), or
+// the left / right steps edge / pad of linear_ramp (float64).
+struct PadVals {
+    double lo, hi;
+};
+
+template <typename T> struct KeyOf;
+template <> struct KeyOf<float> { using K = unsigned int; };
+template <> struct KeyOf<double> { using K = unsigned long long; };
+
+// order-preserving map between floating-point values and unsigned keys
+template <typename T, typename K>
+__device__ __forceinline__ K to_key(T v) {
+    constexpr K sign = (K)1 << (sizeof(K) * 8 - 1);
+    const K u = __builtin_bit_cast(K, v);
+    return (u & sign) ? (K)~u : (K)(u | sign);
+}
+template <typename T, typename K>
+__device__ __forceinline__ T from_key(K k) {
+    constexpr K sign = (K)1 << (sizeof(K) * 8 - 1);
+    return __builtin_bit_cast(T, (k & sign) ? (K)(k ^ sign) : (K)~k);
+}
+
+// The k-th smallest (0-based) of n keys in LDS: radix select, 4 bits per
+// pass from the top, `hist` 16 ints of LDS.  All NT threads call it; it
+// ends synchronised.  Invariant: more than k keys match the prefix.
+template <int NT, typename K>
+__device__ K lds_select(const K* keys, int n, int k, int* hist) {
+    const int tid = threadIdx.x;
+    K prefix = 0, mask = 0;
+    for (int shift = (int)sizeof(K) * 8 - 4; shift >= 0; shift -= 4) {
+        if (tid < 16) hist[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) {
+            const K key = keys[i];
+            if ((key & mask) == prefix)
+                atomicAdd(&hist[(int)((key >> shift) & 15)], 1);
+        }
+        __syncthreads();
+        int d = 0, below = 0;
+        while (below + hist[d] <= k) below += hist[d++];
+        k -= below;
+        prefix |= (K)d << shift;
+        mask |= (K)15 << shift;
+        __syncthreads();  // every thread has read hist before the reset
+    }
+    return prefix;
+}
+
+template <int NT>
+__device__ double block_reduce(double v, int mode, double* red) {
+    auto comb = [mode](double a, double b) {
+        return mode == ATH_PAD_MAXIMUM ? fmax(a, b)
+               : mode == ATH_PAD_MINIMUM ? fmin(a, b) : a + b;
+    };
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = comb(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < NT / 64; ++w) r = comb(r, red[w]);
+    __syncthreads();  // red free for the caller
+    return r;
+}
+
+// PadVals of one row (sample c = the row's value after the optional log).
+// buf: the row's LDS (M >= ncol + 1 double2, M >= 32) used as scratch; all
+// NT threads; ends synchronised.
+template <int NT, typename Tin>
+__device__ PadVals row_pad_values(double2* buf, const Tin* __restrict__ row,
+                                  int ncol, int pad, int mode,
+                                  int apply_log) {
+    const int tid = threadIdx.x;
+    PadVals pv{0.0, 0.0};
+    if (mode == ATH_PAD_LINEAR_RAMP) {
+        // numpy.linspace(0, edge, pad, endpoint=False, dtype=row type):
+        // value j = j * (edge / pad) evaluated in float64, then rounded to
+        // the row's type
+        pv.lo = load_sample<Tin>(row, 0, apply_log) / (double)pad;
+        pv.hi = load_sample<Tin>(row, ncol - 1, apply_log) / (double)pad;
+        return pv;
+    }
+    if (mode == ATH_PAD_MEDIAN) {
+        using K = typename KeyOf<Tin>::K;
+        K* keys = reinterpret_cast<K*>(buf);
+        int* hist = reinterpret_cast<int*>(keys + ncol + (ncol & 1));
+        for (int c = tid; c < ncol; c += NT)
+            keys[c] = to_key<Tin, K>((Tin)load_sample<Tin>(row, c, apply_log));
+        __syncthreads();
+        // numpy.median: the middle value, or the mean of the two middle
+        // values in the row's type
+        const int k2 = ncol / 2;
+        const Tin hi = from_key<Tin, K>(lds_select<NT, K>(keys, ncol, k2,
+                                                          hist));
+        Tin med = hi;
+        if ((ncol & 1) == 0) {
+            const Tin lo = from_key<Tin, K>(lds_select<NT, K>(keys, ncol,
+                                                              k2 - 1, hist));
+            med = (Tin)(lo + hi) / (Tin)2;
+        }
+        pv.lo = pv.hi = (double)med;
+        return pv;
+    }
+    double acc = mode == ATH_PAD_MAXIMUM   ? -HUGE_VAL
+                 : mode == ATH_PAD_MINIMUM ? HUGE_VAL : 0.0;
+    for (int c = tid; c < ncol; c += NT) {
+        const double v = load_sample<Tin>(row, c, apply_log);
+        acc = mode == ATH_PAD_MAXIMUM   ? fmax(acc, v)
+              : mode == ATH_PAD_MINIMUM ? fmin(acc, v) : acc + v;
+    }
+    double r = block_reduce<NT>(acc, mode, reinterpret_cast<double*>(buf));
+    if (mode == ATH_PAD_MEAN) r = (double)((Tin)r / (Tin)ncol);
+    pv.lo = pv.hi = r;
+    return pv;
+}
+
+// value of padded sample k (outside [pad, pad + ncol)) in a value mode
+template <typename Tin>
+__device__ __forceinline__ double pad_value(const PadVals& pv, int mode,
+                                            int k, int pad, int ncol) {
+    if (mode != ATH_PAD_LINEAR_RAMP) return pv.lo;
+    // left ramp j = k; right ramp reversed: j = pad - 1 - (k - pad - ncol)
+    const bool left = k < pad;
+    const int j = left ? k : 2 * pad + ncol - 1 - k;
+    return (double)(Tin)((double)j * (left ? pv.lo : pv.hi));
+}
+
 // Filters one row in LDS buffer `buf` (M double2): pad, rfft, multiply by
 // the kernel half spectrum G, irfft, crop into dst.  All 256 threads.
 // ADJ: the transpose of the filter -- the row is embedded with zeros
@@ -64,6 +194,14 @@ __device__ __forceinline__ void filter_row(
     constexpr int B = RowCfg<MAXB>::maxb, NT = RowCfg<MAXB>::nt;
     const int tid = threadIdx.x;
     const int width = ncol + 2 * pad;
+    PadVals pv{0.0, 0.0};
+    bool value_mode = false;
+    if constexpr (!ADJ) {
+        value_mode = pad_mode >= ATH_PAD_LINEAR_RAMP && pad > 0;
+        if (value_mode)
+            pv = row_pad_values<NT, Tin>(buf, src_row, ncol, pad, pad_mode,
+                                         apply_log);
+    }
     // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
     for (int n = tid; n < M; n += NT) {
         double v[2];
@@ -74,7 +212,10 @@ __device__ __forceinline__ void filter_row(
             if (k < width) {
                 const int s = ADJ ? pad_source(k - pad, ncol, ATH_PAD_CONSTANT)
                                   : pad_source(k - pad, ncol, pad_mode);
-                if (s >= 0) v[h] = load_sample<Tin>(src_row, s, apply_log);
+                if (s >= 0)
+                    v[h] = load_sample<Tin>(src_row, s, apply_log);
+                else if (value_mode)
+                    v[h] = pad_value<Tin>(pv, pad_mode, k, pad, ncol);
             }
         }
         buf[n] = make_double2(v[0], v[1]);

@@ -58,7 +58,7 @@ class BatchFBP:
         import torch
         self.dev = nat.device()
         self.lib = nat.load()
-        if pad_mode not in nat.PAD_MODES:
+        if pad_mode not in nat.FILTER_PAD_MODES:
             raise NotImplementedError("pad_mode %r" % pad_mode)
         shapes = [tuple(s) for s in shapes]
         n = len(shapes)
@@ -205,7 +205,8 @@ class BatchFBP:
         for L, code, njobs, rows, jobs in self.filter_groups:
             nat.check(self.lib.ath_ramp_filter_batched(
                 nat.ptr(jobs), njobs, rows, code, self.store_code, L,
-                nat.PAD_MODES[self.pad_mode], int(self.apply_log), stream),
+                nat.FILTER_PAD_MODES[self.pad_mode], int(self.apply_log),
+                stream),
                 "ath_ramp_filter_batched")
         for g in self.bp_groups:
             nat.check(self.lib.ath_back_projection_batched(

@@ -1,15 +1,22 @@
 """Centre-of-rotation search by slice metrics -- drop-in for algotom's
 ``_find_center_based_slice_metric`` and ``find_center_based_slice_metric``
-(algotom/rec/reconstruction.py:1254-1458), method "fbp".
+(algotom/rec/reconstruction.py:1254-1458), methods "fbp", "bpf" and "dfi".
 
 The reference reconstructs the slice once per candidate centre, re-filtering
 the sinogram every time (:1349-1355).  Here the sinogram is smoothed
-(ath_gaussian_filter_rows), logged and ramp-filtered ONCE, every candidate
-centre is back-projected in one persistent batched launch (the filtered
-sinogram shared by all jobs), and the entropy / sharpness metric of every
-slice is computed on the GPU (ath_slice_metrics).  Per centre the slice is
-exactly the fbp_reconstruction result, so the metrics and the chosen centre
-match a reference run on the same sinogram.
+(ath_gaussian_filter_rows) and logged once; then
+  * fbp: ramp-filtered once, and every candidate centre back-projected in
+    one persistent batched launch (the filtered sinogram shared by all
+    jobs);
+  * bpf: every centre back-projected with edge padding in one batched
+    launch, then the 2D circular ramp filter of all the slices, masked and
+    scaled in float64 (bpf_reconstruction's 2D branch, :645-684);
+  * dfi (and, as in the reference's dispatch :1204-1207, any other name
+    but "gridrec" and "astra"): dfi_reconstruction per centre;
+and the entropy / sharpness metric of every slice is computed on the GPU
+(ath_slice_metrics).  Per centre the slice is exactly the result of the
+method's drop-in, so the metrics and the chosen centre match a reference run
+on the same sinogram.
 """
 
 import numpy as np
@@ -46,10 +53,22 @@ def _smooth_rows(sino_t, sigma):
 _TILE_ROWS_F64 = 32
 
 
-def _backproject_centers(filt, centers, angles, ratio, acc):
-    """(ncenter, ncol, ncol) slices of one filtered sinogram (nrow, 1, ncol)
+def _mask_radius(center, ratio, ncol):
+    """Radius of make_circle_mask for fbp/bpf's ratio logic
+    (reconstruction.py:437-440); -1 for no mask."""
+    if ratio is None:
+        return -1.0
+    r = ratio
+    if r == 0.0:
+        r = min(center, ncol - center) / (0.5 * ncol)
+    return float(r * (ncol // 2))
+
+
+def _backproject_centers(filt, centers, angles, ratio, acc, edge_pad=False,
+                         finish=True):
+    """(ncenter, ncol, ncol) slices of one sinogram (nrow, 1, ncol)
     back-projected at every centre in one persistent batched launch, each
-    with fbp_reconstruction's mask and scale."""
+    with fbp_reconstruction's mask and scale (finish) or without them."""
     import torch
     dev = filt.device
     lib = nat.load()
@@ -67,15 +86,10 @@ def _backproject_centers(filt, centers, angles, ratio, acc):
         if c < 0 or c >= ncol:
             raise ValueError("Center is out of the range [0, {}]"
                              .format(ncol - 1))
-        radius = -1.0
-        if ratio is not None:
-            r = ratio
-            if r == 0.0:
-                r = min(c, ncol - c) / (0.5 * ncol)
-            radius = float(r * (ncol // 2))
+        radius = _mask_radius(c, ratio, ncol) if finish else -1.0
         jobs[k] = nat.BpJob(filt.data_ptr(), out[k].data_ptr(), ang.data_ptr(),
-                            nrow, 1, ncol, float(np.float32(c)), radius, 1, 0,
-                            tile)
+                            nrow, 1, ncol, float(np.float32(c)), radius,
+                            int(finish), int(bool(edge_pad)), tile)
         tile += lib.ath_bp_job_tiles(ncol, 1, 1, rows)
     raw = np.frombuffer(bytes(jobs), dtype=np.uint8).copy()
     jobs_d = torch.from_numpy(raw).to(dev)
@@ -117,8 +131,10 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
     (float64 sums if True); ``ncore`` and ``prefer`` are ignored.
     """
     import torch
-    if method != "fbp":
-        raise NotImplementedError("only method='fbp' runs on the GPU path")
+    if method in ("gridrec", "astra"):
+        raise NotImplementedError(
+            "method %r reconstructs through tomopy / astra, outside this "
+            "GPU path (methods: 'fbp', 'bpf', 'dfi')" % method)
     list_center = np.arange(start, stop, step)
     if len(list_center) == 0:
         raise ValueError("Invalid searching parameters: (start, stop, step)={}"
@@ -134,27 +150,23 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
     if apply_log:
         rec._clamp_nonpositive(sino_t)
     nrow, ncol = sino_t.shape
-    if angles is None:
-        angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
-    pad = min(int(0.15 * ncol), 150)
-    taps = rec._window_taps(None, nrow, ncol + 2 * pad, filter_name)
-    filt = rec._filter_stack(sino_t.reshape(nrow, 1, ncol), pad, "edge",
-                             bool(apply_log), taps, torch.float32)
-    acc = nat.ACC_F64 if gpu else nat.ACC_F32SEQ
+    make, dtype, px_bytes = _slice_maker(method, sino_t, bool(apply_log),
+                                         angles, ratio, filter_name, gpu)
     nmin, nmax = 0.0, 1.0
     if metric_function is None and metric == "entropy":
-        ref = _backproject_centers(filt, [float(np.mean(list_center))],
-                                   angles, ratio, acc)
+        ref = make([float(np.mean(list_center))])
         nmin = float(torch.min(ref).item())
         nmax = float(torch.max(ref).item())
         if nmin == nmax:
             raise ValueError("Empty image!!!")
-    per_batch = max(1, _MAX_BATCH_BYTES // (4 * ncol * ncol))
+        if dtype == nat.ATH_F32:
+            nmin, nmax = float(np.float32(nmin)), float(np.float32(nmax))
+    per_batch = max(1, _MAX_BATCH_BYTES // (px_bytes * ncol * ncol))
     values = []
     lib = nat.load()
     for b0 in range(0, len(list_center), per_batch):
         cs = list_center[b0:b0 + per_batch]
-        imgs = _backproject_centers(filt, cs, angles, ratio, acc)
+        imgs = make(cs)
         if metric_function is not None:
             for k in range(len(cs)):
                 values.append(float(metric_function(imgs[k].cpu().numpy(),
@@ -165,10 +177,9 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
         nws = lib.ath_slice_metrics_workspace_bytes(len(cs), ncol, ncol, code)
         ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=dev)
         nat.check(lib.ath_slice_metrics(
-            nat.ptr(imgs), nat.ATH_F32, len(cs), ncol, ncol, ncol * ncol,
-            float(np.float32(nmin)), float(np.float32(nmax)), code,
-            nat.ptr(out), nat.ptr(ws), nws, nat.stream_ptr(dev)),
-            "ath_slice_metrics")
+            nat.ptr(imgs), dtype, len(cs), ncol, ncol, ncol * ncol, nmin,
+            nmax, code, nat.ptr(out), nat.ptr(ws), nws,
+            nat.stream_ptr(dev)), "ath_slice_metrics")
         values.extend(out.cpu().numpy().tolist())
     list_metric = np.asarray(values, dtype=np.float64)
     if invert_metric:
@@ -176,6 +187,70 @@ def _find_center_based_slice_metric(sinogram, start, stop, step=1,
     if return_metric:
         return list_center, list_metric
     return list_center[np.argmin(list_metric)]
+
+
+def _slice_maker(method, sino_t, apply_log, angles, ratio, filter_name, gpu):
+    """(make, dtype, px_bytes): make(centers) returns the (ncenter, ncol,
+    ncol) device slices that _reconstruct_slice (reconstruction.py
+    :1183-1208) gives for those centres with the method's defaults (pad,
+    "edge" padding), of element type dtype (ATH_F32 for fbp, ATH_F64 for
+    bpf and dfi), using about px_bytes device bytes per slice pixel.
+    sino_t: the smoothed (nrow, ncol) float32 sinogram, values > 0 if
+    apply_log."""
+    import torch
+    nrow, ncol = sino_t.shape
+    dev = sino_t.device
+    if method == "fbp":
+        if angles is None:
+            angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+        pad = min(int(0.15 * ncol), 150)
+        taps = rec._window_taps(None, nrow, ncol + 2 * pad, filter_name)
+        filt = rec._filter_stack(sino_t.reshape(nrow, 1, ncol), pad, "edge",
+                                 apply_log, taps, torch.float32)
+        acc = nat.ACC_F64 if gpu else nat.ACC_F32SEQ
+        return (lambda cs: _backproject_centers(filt, cs, angles, ratio,
+                                                acc)), nat.ATH_F32, 4
+    logged = rec._minus_log(sino_t) if apply_log else sino_t
+    if method == "bpf":
+        if angles is None:
+            angles = np.deg2rad(np.linspace(0.0, 180.0, nrow))
+        pad = min(int(0.15 * ncol), 150)
+        ramp_win = rec._circular_window(ncol + 2 * pad, filter_name)
+        name_key = ("window", filter_name)
+        acc = nat.ACC_F64 if gpu else nat.ACC_F32SEQ
+        sino3 = logged.reshape(nrow, 1, ncol)
+
+        def make_bpf(cs):
+            bp = _backproject_centers(sino3, cs, angles, None, acc,
+                                      edge_pad=True, finish=False)
+            out = torch.empty((len(cs), ncol, ncol), dtype=torch.float64,
+                              device=dev)
+            radii = [_mask_radius(c, ratio, ncol) for c in cs]
+            k = 0
+            while k < len(cs):   # one filter call per run of equal radii
+                k1 = k + 1
+                while k1 < len(cs) and radii[k1] == radii[k]:
+                    k1 += 1
+                rec._circular_filter(bp[k:k1], ncol, k1 - k, ncol,
+                                     ncol * ncol, pad, "edge", ramp_win,
+                                     out[k:k1], ncol, ncol * ncol, radii[k],
+                                     nrow, name_key)
+                k = k1
+            return out
+
+        return make_bpf, nat.ATH_F64, 12
+
+    def make_dfi(cs):
+        out = torch.empty((len(cs), ncol, ncol), dtype=torch.float64,
+                          device=dev)
+        for k, c in enumerate(cs):
+            out[k] = rec.dfi_reconstruction(logged, float(c), angles=angles,
+                                            ratio=ratio,
+                                            filter_name=filter_name,
+                                            apply_log=False)
+        return out
+
+    return make_dfi, nat.ATH_F64, 8
 
 
 def find_center_based_slice_metric(sinogram, start, stop, step=0.5,

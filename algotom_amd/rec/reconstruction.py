@@ -224,10 +224,10 @@ def _clamp_nonpositive(sinogram):
 def _filter_stack(sino_t, pad, pad_mode, apply_log, taps, out_dtype):
     """Run the ramp-filter kernel on a (nrow, nslice, ncol) device tensor."""
     import torch
-    if pad_mode not in nat.PAD_MODES:
+    if pad_mode not in nat.FILTER_PAD_MODES:
         raise NotImplementedError(
             "pad_mode %r is not supported by the GPU filter (supported: %s)"
-            % (pad_mode, ", ".join(nat.PAD_MODES)))
+            % (pad_mode, ", ".join(nat.FILTER_PAD_MODES)))
     dev = sino_t.device
     nrow, nslice, ncol = sino_t.shape
     lib = nat.load()
@@ -237,7 +237,8 @@ def _filter_stack(sino_t, pad, pad_mode, apply_log, taps, out_dtype):
     out = torch.empty((nrow, nslice, ncol), dtype=out_dtype, device=dev)
     nat.check(lib.ath_ramp_filter(nat.ptr(sino_t), nat.dtype_code(sino_t.dtype),
                                   nrow, nslice, ncol, int(pad),
-                                  nat.PAD_MODES[pad_mode], int(bool(apply_log)),
+                                  nat.FILTER_PAD_MODES[pad_mode],
+                                  int(bool(apply_log)),
                                   nat.ptr(plan), fft_len, nwin, nat.ptr(out),
                                   nat.dtype_code(out_dtype),
                                   nat.stream_ptr(dev)),
@@ -368,7 +369,9 @@ def apply_ramp_filter(sinogram, ramp_win=None, filter_name=None, pad=None,
     pad : int or None
         Padding before the FFT; min(int(0.15*ncol), 150) if None.
     pad_mode : str
-        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap".
+        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap",
+        "linear_ramp", "maximum", "mean", "median", "minimum" (numpy's
+        defaults for the last five).
 
     Returns
     -------
@@ -452,7 +455,9 @@ def fbp_reconstruction(sinogram, center, angles=None, ratio=1.0,
     pad : int, optional
         To apply padding before the FFT; min(int(0.15*ncol), 150) if None.
     pad_mode : str, optional
-        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap".
+        numpy.pad mode: "edge", "constant", "reflect", "symmetric", "wrap",
+        "linear_ramp", "maximum", "mean", "median", "minimum" (numpy's
+        defaults for the last five).
     apply_log : bool, optional
         Apply the logarithm function to the sinogram before reconstruction.
     gpu : bool, optional

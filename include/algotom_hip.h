@@ -43,6 +43,19 @@ extern "C" {
 #define ATH_PAD_REFLECT 2
 #define ATH_PAD_SYMMETRIC 3
 #define ATH_PAD_WRAP 4
+/* numpy.pad's value modes with their defaults (end_values=0,
+ * stat_length=None: the statistic of the whole row, the same value on both
+ * sides), computed per row of the (logged) input by the filter kernel.
+ * Accepted by ath_ramp_filter, ath_ramp_filter_batched and
+ * ath_fbp_reconstruction only.  maximum / minimum / median are numpy's
+ * exactly, linear_ramp as numpy 2's linspace (float64, rounded once to the
+ * row's type); the mean's sum is float64 (numpy: pairwise in the row's
+ * type), so a float32 mean may differ from numpy's in the last bit. */
+#define ATH_PAD_LINEAR_RAMP 5
+#define ATH_PAD_MAXIMUM 6
+#define ATH_PAD_MEAN 7
+#define ATH_PAD_MEDIAN 8
+#define ATH_PAD_MINIMUM 9
 
 /* back-projection accumulation (which reference kernel's arithmetic):
  *  ATH_ACC_F64    float64 register sum, one rounding at the end

@@ -1,11 +1,13 @@
 """Oracle for the centre search by slice metrics -- TEST INFRASTRUCTURE ONLY.
 
-Restates algotom/rec/reconstruction.py:1211-1228 (the two slice metrics),
+Restates algotom/rec/reconstruction.py:1183-1208 (_reconstruct_slice, the
+methods "fbp", "bpf" and "dfi"), :1211-1228 (the two slice metrics),
 :1231-1251 (metric of one slice), :1254-1362
-(_find_center_based_slice_metric, method "fbp") and :1365-1458
+(_find_center_based_slice_metric) and :1365-1458
 (find_center_based_slice_metric).  The metrics are compositions of numpy /
 scipy calls in the reference, so they are evaluated with the same calls
-here; the reconstructions come from the oracle's C port (oracle/port.py).
+here; the reconstructions come from the oracle's FBP port (oracle/port.py),
+BPF (oracle/bpf_np.py) and DFI (oracle/dfi_np.py) restatements.
 
 Pinned in tests/test_oracle_center.py by closed forms (the entropy of a
 constant image, the sharpness of a linear ramp, the histogram level of
@@ -17,7 +19,7 @@ import numpy as np
 import scipy.ndimage as ndi
 from scipy import signal
 
-from oracle import port
+from oracle import bpf_np, dfi_np, port
 
 
 def histogram_entropy(img):
@@ -58,9 +60,25 @@ def preprocess(sinogram, sigma=2, apply_log=True):
     return sino
 
 
+def reconstruct_slice(sino, center, method="fbp", angles=None, ratio=1.0,
+                      filter_name="hann", gpu=True):
+    """_reconstruct_slice (reconstruction.py:1183-1208) of a logged
+    sinogram: every name but "fbp" and "bpf" goes to DFI, as there
+    ("gridrec" and "astra" need external libraries and are not restated)."""
+    if method == "fbp":
+        return port.fbp(sino, center, angles=angles, ratio=ratio,
+                        filter_name=filter_name, apply_log=False, gpu=gpu)
+    if method == "bpf":
+        return bpf_np.bpf(sino, center, angles=angles, ratio=ratio,
+                          filter_name=filter_name, apply_log=False, gpu=gpu)
+    return dfi_np.dfi(sino, center, angles=angles, ratio=ratio,
+                      filter_name=filter_name, apply_log=False)
+
+
 def center_metrics(sinogram, start, stop, step=1, metric="entropy",
                    gpu=True, angles=None, ratio=1.0, filter_name="hann",
-                   apply_log=True, sigma=2, invert_metric=False):
+                   apply_log=True, sigma=2, invert_metric=False,
+                   method="fbp"):
     """_find_center_based_slice_metric (reconstruction.py:1254-1362) with
     return_metric=True: (list_center, list_metric)."""
     centers = np.arange(start, stop, step)
@@ -69,8 +87,8 @@ def center_metrics(sinogram, start, stop, step=1, metric="entropy",
     sino = preprocess(sinogram, sigma, apply_log)
 
     def recon(c):
-        return port.fbp(sino, c, angles=angles, ratio=ratio,
-                        filter_name=filter_name, apply_log=False, gpu=gpu)
+        return reconstruct_slice(sino, c, method, angles, ratio, filter_name,
+                                 gpu)
 
     nmin, nmax = 0.0, 1.0
     if metric == "entropy":
@@ -88,7 +106,7 @@ def center_metrics(sinogram, start, stop, step=1, metric="entropy",
 
 def find_center(sinogram, start, stop, step=0.5, metric="entropy",
                 radius=2, zoom=1.0, gpu=True, angles=None, ratio=1.0,
-                filter_name="hann", apply_log=True, sigma=2):
+                filter_name="hann", apply_log=True, sigma=2, method="fbp"):
     """find_center_based_slice_metric (reconstruction.py:1365-1458):
     integer-step coarse search (on the zoomed sinogram), then a sub-pixel
     search within +-radius of the coarse centre."""
@@ -101,7 +119,7 @@ def find_center(sinogram, start, stop, step=0.5, metric="entropy",
         if angles is not None:
             angles_z = ndi.zoom(np.tile(angles, (1, 1)), (1.0, zoom))[0]
     kw = dict(metric=metric, gpu=gpu, ratio=ratio, filter_name=filter_name,
-              apply_log=apply_log, sigma=sigma)
+              apply_log=apply_log, sigma=sigma, method=method)
     if stop > start:
         cs, vs = center_metrics(sino_z, start, stop + 1, 1.0,
                                 angles=angles_z, **kw)

@@ -147,11 +147,21 @@ def pad_index(width_in, pad, width_out, mode):
     raise ValueError(mode)
 
 
+# numpy.pad's value modes (reconstruction.py:147 passes any mode to np.pad):
+# with their defaults the pad values are a statistic of the whole row or a
+# linspace ramp, which np.pad itself evaluates here
+VALUE_PAD_MODES = ("linear_ramp", "maximum", "mean", "median", "minimum")
+
+
 def pad_rows(rows, pad, mode):
-    """numpy.pad((0,0),(pad,pad)) restated with explicit index maps."""
+    """numpy.pad((0,0),(pad,pad)) restated with explicit index maps (the
+    value modes through np.pad itself, in the rows' own type)."""
     rows = np.asarray(rows)
     n = rows.shape[-1]
     w = n + 2 * pad
+    if mode in VALUE_PAD_MODES:
+        return np.pad(rows, [(0, 0)] * (rows.ndim - 1) + [(pad, pad)],
+                      mode=mode)
     if mode == "constant":
         out = np.zeros(rows.shape[:-1] + (w,), dtype=rows.dtype)
         out[..., pad:pad + n] = rows

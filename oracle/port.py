@@ -153,6 +153,15 @@ def ramp_filter(rows, g, pad, pad_mode="edge", apply_log=False,
     r = np.asarray(rows)
     in_f64 = r.dtype == np.float64
     r = np.ascontiguousarray(r, dtype=np.float64 if in_f64 else np.float32)
+    if pad_mode in fbp_np.VALUE_PAD_MODES and pad > 0:
+        # the pad values depend on the logged row: log (float64, rounded to
+        # the row's type), np.pad, then the same filter with nothing to pad
+        if apply_log:
+            r = (-np.log(r.astype(np.float64))).astype(r.dtype)
+        ncol = r.shape[-1]
+        full = ramp_filter(fbp_np.pad_rows(r, pad, pad_mode), g, 0, "edge",
+                           False, nthreads)
+        return np.ascontiguousarray(full[..., pad:pad + ncol])
     ncol = r.shape[-1]
     nrows = r.size // ncol
     g = np.ascontiguousarray(g, dtype=np.float64)

@@ -72,6 +72,37 @@ def test_filter_pad_modes(mode, pad):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("mode", ["linear_ramp", "maximum", "mean",
+                                  "median", "minimum"])
+@pytest.mark.parametrize("ncol,pad", [(1, 3), (2, 5), (7, 2), (90, 37),
+                                      (91, 200), (2560, 150)])
+@pytest.mark.parametrize("dtype,apply_log", [("float32", False),
+                                             ("float32", True),
+                                             ("float64", False)])
+def test_filter_value_pad_modes(mode, ncol, pad, dtype, apply_log):
+    """numpy.pad's value modes, computed per (logged) row in the kernel.
+    One slice's rows are quantised to steps of 1/8 so the median meets
+    ties.  Bar: 1e-12 * max|ref|; a float32 mean may sit one float32 ulp
+    from numpy's pairwise float32 mean, so that case gets 1e-6."""
+    import torch
+    from oracle import fbp_np
+    from algotom_amd.rec import reconstruction as rec
+    sino = _rows(5, 3, ncol, 31 * ncol + pad, positive=apply_log)
+    sino[:, 1, :] = np.round(sino[:, 1, :] * 8.0) / 8.0
+    if apply_log:
+        sino[:, 1, :] = np.maximum(sino[:, 1, :], 0.125)
+    sino = sino.astype(dtype)
+    win = fbp_np.ramp_window_row(ncol + 2 * pad, "hann")
+    got = _gpu_filter(sino, pad, mode, apply_log, rec._taps(win)[None, :],
+                      torch.float64)
+    ref = _oracle(sino, pad, mode, apply_log, win)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    bound = 1e-6 if (mode == "mean" and dtype == "float32") else 1e-12
+    print("%s ncol %d pad %d %s log=%s: rel err %.3g" % (
+        mode, ncol, pad, dtype, apply_log, err))
+    assert err <= bound
+
+
 @pytest.mark.parametrize("name", [None, "hann", "bartlett", "blackman",
                                   "hamming", "nuttall", "parzen", "triang"])
 def test_filter_windows_and_log(name):

@@ -105,9 +105,10 @@ def test_invalid_filter_arguments():
     st = lib.ath_ramp_filter(dummy, 0, 4, 1, 100, 10, 0, 0, dummy, 128, 1,
                              dummy, 0, None)
     assert st == nat.ATH_ERR_INVALID_ARG
-    st = lib.ath_ramp_filter(dummy, 0, 4, 1, 100, 10, 9, 0, dummy, 256, 1,
-                             dummy, 0, None)
-    assert st == nat.ATH_ERR_INVALID_ARG  # bad pad mode
+    for mode in (-1, 10):  # pad modes are 0..9
+        st = lib.ath_ramp_filter(dummy, 0, 4, 1, 100, 10, mode, 0, dummy, 256,
+                                 1, dummy, 0, None)
+        assert st == nat.ATH_ERR_INVALID_ARG  # bad pad mode
     st = lib.ath_ramp_filter(dummy, 2, 4, 1, 100, 10, 0, 0, dummy, 256, 1,
                              dummy, 0, None)
     assert st == nat.ATH_ERR_INVALID_ARG  # float16 input

@@ -161,6 +161,26 @@ def test_reference_thresholds_on_oracle():
     assert np.median(np.abs(mat - rs[:, 0, :])) <= 0.2
 
 
+def test_reference_case_fixture_is_current():
+    """tests/golden/reference_case.npz holds the reference test's input
+    and what the oracle makes of it now (read by the GPU tests)."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "golden", "reference_case.npz")
+    with np.load(path, allow_pickle=False) as d:
+        mat, sino, ang, center = _reference_case()
+        assert np.array_equal(d["sinogram"], sino)
+        assert np.array_equal(d["phantom"], mat)
+        assert np.array_equal(d["angles"], ang)
+        assert float(d["center"]) == center
+        assert np.array_equal(d["fbp_acc_f64"],
+                              port.fbp(sino, center, ang, apply_log=False,
+                                       gpu=True))
+        assert np.array_equal(d["fbp_acc_f32seq"],
+                              port.fbp(sino, center, ang, apply_log=False,
+                                       gpu=False))
+
+
 def test_numpy_and_port_fbp_agree():
     mat, sino, ang, center = _reference_case()
     for gpu in (True, False):

@@ -90,3 +90,23 @@ def test_search_finds_known_centre(metric):
     c = center_np.find_center(sino, 46, 56, 0.5, metric=metric,
                               angles=angles)
     assert abs(c - 51.0) <= 0.5
+
+
+@pytest.mark.parametrize("method", ["bpf", "dfi"])
+def test_search_finds_known_centre_bpf_dfi(method):
+    """The BPF and DFI slices (oracles pinned in test_oracle_bpf.py and
+    test_oracle_dfi.py) also place the metric minimum at the phantom's
+    rotation centre."""
+    sino, angles = _phantom_sinogram(center=51.0)
+    cs, vs = center_np.center_metrics(sino, 48, 55, 1.0, angles=angles,
+                                      method=method)
+    assert cs[np.argmin(vs)] == 51.0
+
+
+def test_reconstruct_slice_dispatch():
+    """reconstruction.py:1183-1208: any other name reconstructs with DFI."""
+    sino, angles = _phantom_sinogram(center=51.0)
+    logged = center_np.preprocess(sino, 2, True)
+    a = center_np.reconstruct_slice(logged, 51.0, "other", angles)
+    b = center_np.reconstruct_slice(logged, 51.0, "dfi", angles)
+    assert a.dtype == np.float64 and np.array_equal(a, b)

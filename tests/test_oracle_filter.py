@@ -103,6 +103,56 @@ def test_filter_vs_brute_force(mode, ncol, pad):
     assert np.abs(c - ref).max() <= 1e-14 * scale
 
 
+@pytest.mark.parametrize("mode", ["linear_ramp", "maximum", "mean",
+                                  "median", "minimum"])
+@pytest.mark.parametrize("ncol,pad", [(9, 3), (12, 5), (7, 11), (1, 4)])
+def test_value_pad_modes_vs_brute_force(mode, ncol, pad):
+    """numpy.pad's value modes (the reference passes pad_mode to np.pad,
+    reconstruction.py:147): padded in the row's float32 type, then the
+    O(W^2) circular convolution; the numpy and C-port filters agree."""
+    rng = np.random.default_rng(ncol * 100 + pad + 7)
+    row = rng.uniform(-1.0, 1.0, ncol).astype(np.float32)
+    r = fbp_np.ramp_window_row(ncol + 2 * pad, "hann")
+    p = np.pad(row, (pad, pad), mode=mode).astype(np.float64)
+    g = np.real(np.fft.ifftshift(np.fft.ifft(np.fft.ifftshift(r))))
+    w = p.size
+    ref = np.array([sum(p[k] * g[(n - k) % w] for k in range(w))
+                    for n in range(w)])[pad:pad + ncol]
+    got = fbp_np.ramp_filter(row[None], r, None, pad, mode)[0]
+    c = port.ramp_filter(row[None], fbp_np.filter_kernel(r), pad, mode)[0]
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-14 * scale
+    assert np.abs(c - ref).max() <= 1e-14 * scale
+
+
+def test_value_pad_mode_closed_forms():
+    """What numpy.pad's value modes put beside a float32 row: the row's
+    max / min / median / mean on both sides, and linear ramps from 0 that
+    stop one step short of each edge value."""
+    row = np.array([3.0, -1.0, 7.0, 2.0, 2.0, 5.0], dtype=np.float32)
+    pad = 4
+    p = fbp_np.pad_rows(row[None], pad, "maximum")[0]
+    assert p.dtype == np.float32
+    assert (p[:pad] == 7.0).all() and (p[-pad:] == 7.0).all()
+    assert (fbp_np.pad_rows(row[None], pad, "minimum")[0][:pad] == -1.0).all()
+    # even length: mean of the two middle values (2 and 3)
+    assert (fbp_np.pad_rows(row[None], pad, "median")[0][-pad:] == 2.5).all()
+    assert (fbp_np.pad_rows(row[None], pad, "mean")[0][:pad]
+            == np.float32(18.0 / 6.0)).all()
+    lr = fbp_np.pad_rows(row[None], pad, "linear_ramp")[0]
+    assert np.array_equal(lr[:pad], np.float32([0.0, 0.75, 1.5, 2.25]))
+    assert np.array_equal(lr[-pad:], np.float32([3.75, 2.5, 1.25, 0.0]))
+    assert np.array_equal(lr[pad:-pad], row)
+    # the C port's log-then-pad order: pad values come from the logged row
+    x = np.array([[0.5, 0.25, 0.125, 1.0, 0.75]], dtype=np.float32)
+    g = fbp_np.filter_kernel(fbp_np.ramp_window_row(5 + 6, None))
+    a = port.ramp_filter(x, g, 3, "maximum", apply_log=True)
+    lx = (-np.log(x.astype(np.float64))).astype(np.float32)
+    b = fbp_np.ramp_filter(lx, fbp_np.ramp_window_row(11, None), None, 3,
+                           "maximum")
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+
+
 def test_pad_index_matches_numpy_pad():
     rng = np.random.default_rng(1)
     for n in (1, 2, 5, 9):
