@@ -30,7 +30,7 @@ PAD_MODES = {"edge": 0, "constant": 1, "reflect": 2, "symmetric": 3,
 # and the value modes (numpy's defaults: end value 0, whole-row statistics)
 FILTER_PAD_MODES = dict(PAD_MODES, linear_ramp=5, maximum=6, mean=7,
                         median=8, minimum=9)
-ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV = 0, 1, 2, 3
+ACC_F64, ACC_F32SEQ, ACC_F32, ACC_DERIV, ACC_F64_DERIV = 0, 1, 2, 3, 4
 
 # (name, restype, argtypes) of every exported entry point
 _P = ctypes.c_void_p
@@ -50,6 +50,8 @@ SIGNATURES = (
                              _I, _P]),
     ("ath_back_projection", _I, [_P, _I, _I, _I, _I, _P, _F, _I, _I, _D, _I,
                                  _P, _P]),
+    ("ath_back_projection_with_derivative", _I, [_P, _I, _I, _I, _P, _F,
+                                                 _I, _D, _I, _P, _P, _P]),
     ("ath_back_projection_tiled", _I, [_P, _I, _I, _I, _I, _P, _F, _I, _I,
                                        _D, _I, _P, _I, _I, _P]),
     ("ath_bp_job_tiles", ctypes.c_longlong, [_I, _I, _I, _I]),

@@ -92,6 +92,7 @@ int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
 struct BpArgs {
     const void* sino;      // (nrow, nslice, ncol)
     float* recon;          // (ncol, nslice, ncol)
+    float* recon2;         // second output of ATH_ACC_F64_DERIV, else null
     const float* angles;   // nrow float32 radians
     int nrow, nslice, ncol;
     double center;         // float32 centre widened to double

@@ -95,6 +95,7 @@ struct Mode;
 template <>
 struct Mode<ATH_ACC_F64> {
     static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
     using Rec = float2;
     using W = double;
     using Acc = double;
@@ -114,6 +115,7 @@ struct Mode<ATH_ACC_F64> {
 template <>
 struct Mode<ATH_ACC_F32SEQ> {
     static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
     using Rec = float2;
     using W = double;
     using Acc = float;
@@ -133,6 +135,7 @@ struct Mode<ATH_ACC_F32SEQ> {
 template <>
 struct Mode<ATH_ACC_F32> {
     static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
     using Rec = float2;
     using W = float;
     using Acc = float;
@@ -155,6 +158,7 @@ struct Mode<ATH_ACC_F32> {
 template <>
 struct Mode<ATH_ACC_DERIV> {
     static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
     using Rec = float2;
     using W = double;
     using Acc = double;
@@ -169,10 +173,44 @@ struct Mode<ATH_ACC_DERIV> {
     static __device__ __forceinline__ float result(Acc a) { return (float)a; }
 };
 
+// ATH_ACC_F64 into recon and, from the same positions and records,
+// ATH_ACC_DERIV into recon2: two float64 sums per pixel, bit-identical to
+// the two modes run separately.
+struct AccPair {
+    double v, dv;
+};
+
+template <>
+struct Mode<ATH_ACC_F64_DERIV> {
+    static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = true;
+    using Rec = float2;
+    using W = double;
+    using Acc = AccPair;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_float2(yd, yu - yd);
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a.v += (double)q.x + (double)q.y * w;
+        a.dv += (double)q.y;
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a.v += (double)e;
+    }
+    static __device__ __forceinline__ float result(Acc a) {
+        return (float)a.v;
+    }
+    static __device__ __forceinline__ float result2(Acc a) {
+        return (float)a.dv;
+    }
+};
+
 template <>
 struct Mode<kAccH16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
+    static constexpr bool kTwoOut = false;
     using Rec = half2_t;
     using W = half2_t;
     using Acc = float;
@@ -203,6 +241,7 @@ template <>
 struct Mode<kAccB16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
+    static constexpr bool kTwoOut = false;
     using Rec = bf162_t;
     using W = bf162_t;
     using Acc = float;
@@ -330,7 +369,7 @@ __device__ __forceinline__ void bp_tile(
 #pragma unroll
     for (int k = 0; k < PY; ++k)
 #pragma unroll
-        for (int js = 0; js < SB; ++js) acc[k][js] = 0;
+        for (int js = 0; js < SB; ++js) acc[k][js] = typename M::Acc{};
 
     for (int a0 = 0; a0 < N; a0 += A) {
         const int na = min(A, N - a0);
@@ -457,9 +496,13 @@ __device__ __forceinline__ void bp_tile(
             if (y >= C) continue;
 #pragma unroll
             for (int js = 0; js < SB; ++js) {
-                if (js < nsb)
-                    a.recon[((size_t)y * S + j0 + js) * C + x] =
-                        finalize(a, M::result(acc[k][js]), x, y);
+                if (js < nsb) {
+                    const size_t o = ((size_t)y * S + j0 + js) * C + x;
+                    a.recon[o] = finalize(a, M::result(acc[k][js]), x, y);
+                    if constexpr (M::kTwoOut)
+                        a.recon2[o] = finalize(a, M::result2(acc[k][js]), x,
+                                               y);
+                }
             }
         }
     }
@@ -513,6 +556,7 @@ __global__ __launch_bounds__(kThreads) void bp_batched_kernel(
         BpArgs a;
         a.sino = jb.sino;
         a.recon = jb.recon;
+        a.recon2 = nullptr;
         a.angles = jb.angles;
         a.nrow = jb.nrow;
         a.nslice = jb.nslice;
@@ -652,11 +696,18 @@ template <int MODE>
 void default_config(int nslice, int& sb, int& py) {
     sb = nslice <= 2 ? nslice : 4;
     py = 8;
-    if (nslice > 2 && MODE == ATH_ACC_F64) py = 4;
+    if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == ATH_ACC_F64_DERIV))
+        py = 4;
     // the centre derivative reads one value per record: 2 slices x 16 rows
     // (16 x 901 x 1024: 3.47 ms against 4.84 ms for 4 x 32; 16 x 1801 x
     // 7192: 308 against 429 ms)
     if (MODE == ATH_ACC_DERIV && nslice >= 2) {
+        sb = 2;
+        py = 4;
+    }
+    // two float64 sums per pixel and slice: 2 x 16 keeps 120 VGPRs (4 x 16
+    // takes 244)
+    if (MODE == ATH_ACC_F64_DERIV && nslice >= 2) {
         sb = 2;
         py = 4;
     }
@@ -672,11 +723,16 @@ int dispatch(const BpArgs& a, int sb, int py, hipStream_t s) {
 
 int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb,
                     int py, hipStream_t s) {
-    if (acc == ATH_ACC_DERIV) {
+    if (acc == ATH_ACC_DERIV || acc == ATH_ACC_F64_DERIV) {
         if (sino_dtype != ATH_F32)
             return set_error(ATH_ERR_UNSUPPORTED,
                              "centre derivative needs a float32 sinogram");
-        return dispatch<ATH_ACC_DERIV, float>(a, sb, py, s);
+        if (acc == ATH_ACC_DERIV)
+            return dispatch<ATH_ACC_DERIV, float>(a, sb, py, s);
+        if (!a.recon2)
+            return set_error(ATH_ERR_INVALID_ARG,
+                             "ATH_ACC_F64_DERIV needs the derivative output");
+        return dispatch<ATH_ACC_F64_DERIV, float>(a, sb, py, s);
     }
     if (sino_dtype == ATH_F32) {
         switch (acc) {

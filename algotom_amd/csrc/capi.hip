@@ -129,6 +129,7 @@ int fill_bp_args(BpArgs& a, const void* sino, int nrow, int nslice, int ncol,
                          "the pi/(nrow-1) scale needs at least two angles");
     a.sino = sino;
     a.recon = recon;
+    a.recon2 = nullptr;
     a.angles = angles;
     a.nrow = nrow;
     a.nslice = nslice;
@@ -354,6 +355,23 @@ int ath_back_projection(const void* sino, int sino_dtype, int nrow,
                           edge_pad, mask_radius, apply_scale, recon);
     if (st != ATH_OK) return st;
     return back_projection(a, sino_dtype, acc, 0, 0,
+                           static_cast<hipStream_t>(stream));
+}
+
+int ath_back_projection_with_derivative(const float* sino, int nrow,
+                                        int nslice, int ncol,
+                                        const float* angles, float center,
+                                        int edge_pad, double mask_radius,
+                                        int apply_scale, float* recon,
+                                        float* deriv, void* stream) {
+    clear_error();
+    if (!deriv) return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
+    BpArgs a;
+    int st = fill_bp_args(a, sino, nrow, nslice, ncol, angles, center,
+                          edge_pad, mask_radius, apply_scale, recon);
+    if (st != ATH_OK) return st;
+    a.recon2 = deriv;
+    return back_projection(a, ATH_F32, ATH_ACC_F64_DERIV, 0, 0,
                            static_cast<hipStream_t>(stream));
 }
 

@@ -5,44 +5,190 @@
 // The back-projector adds, for pixel (x, y) and angle i with
 // f = ((x - ic) c + (y - ic) s) + center in [0, ncol-1], d = floor f,
 // t = f - d, the value (1 - t) s[d] + t s[d+1] (reconstruction.py:170-181).
-// Its transpose therefore sends (1 - t) g(x, y) to detector bin d and
-// t g(x, y) to bin d + 1.  One thread owns one (angle, bin) pair and gathers
-// every pixel whose f lies in [bin - 1, bin + 1]: it walks the image along
-// the axis closer to the detector normal (rows when |c| >= |s|), and per
-// step tests the <= 2/max(|c|,|s|) + 3 candidate pixels with the exact
-// float64 f of the back-projector.  No atomics: the sum order per bin is
-// fixed, so results are deterministic, and the weights are bit-identical to
-// the forward pass's.  Accumulation is float64.
+// Its transpose sends pixel value g(x, y) to bin b with the hat weight
+// w_b(f) = max(0, 1 - |f - b|) (that is 1 - t for b = d and t for b = d + 1),
+// restricted to 0 <= f <= ncol - 1.  One thread owns one (angle, bin) pair
+// and walks the image along the axis closer to the detector normal (rows
+// when |c| >= |s|, so |slope| >= ~0.69): on each line the pixels with
+// f in (b - 1, b + 1) are the K = ceil(2 / |slope|) (2 or 3) integers after
+// the line's crossing of f = b - sign(slope), found with one multiply; each
+// candidate costs its hat weight and one float32 FMA per slice, with no
+// branch (a candidate just outside the interval gets weight 0).  Sums run in
+// float32 over two chunks of kRows lines and in float64 across those; the
+// order per bin is fixed (no atomics), so results are deterministic.
 #include "ath_internal.h"
 
 namespace ath {
 namespace {
 
-
-__device__ __forceinline__ double fp_det_pos(double xc_c, double yc, double s,
-                                             double cen) {
-    return fma(yc, s, xc_c) + cen;
-}
-
-// LDS-staged variant.  A workgroup = 4 waves = 4 consecutive angles x the
-// same 64 detector bins x FSB slices.  The image is walked in chunks of
-// kRows lines along the major axis; for each chunk the union of the four
-// angles' candidate windows over the 64 (+2) bins is staged into LDS once
-// (line-major, FSB slices interleaved) and every lane then gathers its
-// candidates from LDS.  Each staged pixel serves up to 4 angles x 2 bins.
-// Chunks whose union window is wider than kMaxMinor columns read global
-// memory instead.  The walking axis keeps |slope| >= ~0.69, so 64 bins span
-// <= 93 columns plus the margins and the 4 angles' spread: 128 covers every
-// block of closely spaced angles, and the small tile (16 KB at 8 slices x 4
-// lines) lets 6+ workgroups share a CU to hide the staging loads.
+// A workgroup = 4 waves = 4 consecutive angles x the same 64 detector bins x
+// FSB slices.  The image is walked in chunks of kRows lines along the major
+// axis; for each chunk the union of the four angles' windows over bins
+// [d0 - 2, d0 + 65] is staged into LDS once (line-major, the FSB slices of a
+// pixel in one record of RS floats; columns outside the image or the mask
+// staged as zeros, so candidates there add nothing) and every lane gathers
+// its candidates from LDS.  Each staged pixel serves up to 4 angles x 2
+// bins.  Chunks whose window is wider than kMaxMinor columns, and angles
+// with |slope| < 2 / kMaxK, read global memory instead (exact windows, the
+// same hat weights).  Records are padded to RS = FSB + 4 floats for FSB 8
+// and 16 so that lanes on neighbouring pixels start on different 16-byte
+// bank groups.
+// staged columns per line (the window of 68 bins at |slope| >= 0.69 is
+// about 100 columns).  A 120-column tile (39 KB, four workgroups per CU,
+// VGPRs capped at 128 with spills) measured slower: 6.52 against 5.99 ms.
 constexpr int kMaxMinor = 128;
+constexpr int kMaxK = 8;
 // below this |slope| a line runs (almost) along the ray: take it whole
 constexpr double kMinSlope = 1.0 / 1024.0;
 
+template <int FSB> struct FpRec { static constexpr int rs = FSB; };
+template <> struct FpRec<8> { static constexpr int rs = 12; };
+template <> struct FpRec<16> { static constexpr int rs = 20; };
+
+// acc[j] += w(t) * q[j] for the K candidates of one (bin, line) run:
+// t = f - bin of the first candidate, consecutive candidates RS floats apart
+// in LDS and `slope` apart in t.  EDGE: the block holds bin 0 or the last
+// bin, whose weights also need t in [t_lo, t_hi] (0 <= f <= ncol - 1).
+template <int FSB, int RS, int K, bool EDGE>
+__device__ __forceinline__ void hat_run(const float* __restrict__ q, double t,
+                                        double slope, double t_lo,
+                                        double t_hi, float (&acc)[FSB]) {
+#pragma unroll
+    for (int n = 0; n < K; ++n) {
+        double w = fmax(1.0 - fabs(t), 0.0);
+        if constexpr (EDGE) w = (t >= t_lo && t <= t_hi) ? w : 0.0;
+        const float w32 = (float)w;
+        const float* r = q + n * RS;
+        // the last of three or more candidates is outside (d-1, d+1) for
+        // most lanes when |slope| is near 1: those lanes skip its record
+        // (the same bits; less LDS traffic: 5.81 against 6.05 ms)
+        const bool skip = n == K - 1 && K >= 3 && w32 == 0.0f;
+        if (!skip) {
+            if constexpr (FSB % 4 == 0) {
+#pragma unroll
+                for (int h = 0; h < FSB / 4; ++h) {
+                    const float4 v = reinterpret_cast<const float4*>(r)[h];
+                    acc[4 * h] = fmaf(w32, v.x, acc[4 * h]);
+                    acc[4 * h + 1] = fmaf(w32, v.y, acc[4 * h + 1]);
+                    acc[4 * h + 2] = fmaf(w32, v.z, acc[4 * h + 2]);
+                    acc[4 * h + 3] = fmaf(w32, v.w, acc[4 * h + 3]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < FSB; ++j)
+                    acc[j] = fmaf(w32, r[j], acc[j]);
+            }
+        }
+        t += slope;
+    }
+}
+
+// The lines m0..m1 of a staged chunk for one (angle, bin) lane with K
+// candidates per line (K = 0: the runtime length kdyn).  Along the walking
+// axis everything is linear in m: the crossing u(m) = ic + (dedge -
+// base(m)) / slope steps by du per line, the run starts at kf = floor(u) + 1
+// and its first t = f(kf) - d = (kf - u) slope + (dedge - d).  Returns the
+// lanes (ballot bits) whose run left the staged range on some line: those
+// runs read the zero records, and the caller redoes the lanes' lines from
+// global memory.
+template <int FSB, int RS, int K, bool EDGE>
+__device__ __forceinline__ unsigned long long run_lines(
+    const float* tile, const float* zpad, int m0, int m1, int ka, int wid,
+    int kdyn, double u, double du, double slope, double dedge_d,
+    double t_lo, double t_hi, float (&acc)[FSB]) {
+    const int kk = K > 0 ? K : kdyn;
+    const unsigned span = (unsigned)(wid - kk);
+    unsigned long long bad = 0;
+    for (int m = m0; m <= m1; ++m) {
+        const int kf = (int)floor(u) + 1;
+        const int k0 = kf - ka;
+        // the window's one-column margins keep the run inside the staged
+        // range; the test guards the LDS reads
+        const bool ok = (unsigned)k0 <= span;
+        bad |= __ballot(!ok);
+        const float* q =
+            ok ? tile + __umul24((unsigned)((m - m0) * kMaxMinor + k0), RS)
+               : zpad;
+        const double t = fma((double)kf - u, slope, dedge_d);
+        if constexpr (K > 0) {
+            hat_run<FSB, RS, K, EDGE>(q, t, slope, t_lo, t_hi, acc);
+        } else {
+            double tn = t;
+            for (int n = 0; n < kdyn; ++n) {
+                hat_run<FSB, RS, 1, EDGE>(q + n * RS, tn, slope, t_lo, t_hi,
+                                          acc);
+                tn += slope;
+            }
+        }
+        u += du;
+    }
+    return bad;
+}
+
+// run_lines with K fixed at compile time for the common K = 2, 3 (|slope|
+// >= 2/3) and 4; K is wave-uniform, so the branch is too.
+template <int FSB, int RS, bool EDGE>
+__device__ __forceinline__ unsigned long long run_lines_k(
+    int K, const float* tile, const float* zpad, int m0, int m1, int ka,
+    int wid, double u, double du, double slope, double dedge_d, double t_lo,
+    double t_hi, float (&acc)[FSB]) {
+#define ATH_RUN(KK)                                                          \
+    run_lines<FSB, RS, KK, EDGE>(tile, zpad, m0, m1, ka, wid, K, u, du,      \
+                                 slope, dedge_d, t_lo, t_hi, acc)
+    if (K == 3) return ATH_RUN(3);
+    if (K == 2) return ATH_RUN(2);
+    if (K == 4) return ATH_RUN(4);
+    return ATH_RUN(0);
+#undef ATH_RUN
+}
+
+// Every pixel of line m whose f lies in (d - 1, d + 1), read from global
+// memory with the exact window, the mask and the 0 <= f <= ncol - 1 test.
+template <int FSB>
+__device__ void gather_line_global(const FpArgs& a, int m, bool by_rows,
+                                   bool steep, double slope, double inv_slope,
+                                   double base, double dd, double t_lo,
+                                   double t_hi, int j0, int ns,
+                                   float (&acc)[FSB]) {
+    const int C = a.ncol, S = a.nslice;
+    const double ic = a.icenter;
+    int k0 = 0, k1 = C - 1;
+    if (steep) {
+        double u0 = (dd - 1.0 - base) * inv_slope;
+        double u1 = (dd + 1.0 - base) * inv_slope;
+        if (u0 > u1) {
+            const double t = u0;
+            u0 = u1;
+            u1 = t;
+        }
+        k0 = max((int)ceil(u0 + ic - 1e-6), 0);
+        k1 = min((int)floor(u1 + ic + 1e-6), C - 1);
+    }
+    for (int k = k0; k <= k1; ++k) {
+        const double t = ((double)k - ic) * slope + base - dd;
+        const double w = 1.0 - fabs(t);
+        if (!(w > 0.0 && t >= t_lo && t <= t_hi)) continue;
+        const int x = by_rows ? k : m;
+        const int y = by_rows ? m : k;
+        if (a.use_mask) {
+            const long long dx = x - a.mask_c, dy = y - a.mask_c;
+            if (!((double)(dx * dx + dy * dy) <= a.mask_r2)) continue;
+        }
+        const float w32 = (float)w;
+        const float* px = a.image + ((size_t)y * S + j0) * C + x;
+#pragma unroll
+        for (int j = 0; j < FSB; ++j)
+            if (j < ns) acc[j] = fmaf(w32, px[(size_t)j * C], acc[j]);
+    }
+}
+
 template <int FSB, int kRows>
-__global__ __launch_bounds__(256) void forward_project_lds_kernel(
+__global__ __launch_bounds__(256) void forward_project_hat_kernel(
     FpArgs a, int by_rows_block) {
-    __shared__ __align__(16) float tile[kRows * kMaxMinor * FSB];
+    constexpr int RS = FpRec<FSB>::rs;
+    // the tile, then kMaxK zero records: a run that would leave the staged
+    // range reads those instead and its line is redone from global memory
+    __shared__ __align__(16) float tile[(kRows * kMaxMinor + kMaxK) * RS];
     __shared__ double s_cs[4][2];
     __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
     __shared__ int s_ab[2];       // the chunk's staged column range
@@ -53,7 +199,9 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
     const int j0 = blockIdx.z * FSB;
     const int ns = min(FSB, S - j0);
     const int d = d0 + lane;
-    const double ic = a.icenter, cen = a.center, w1 = (double)(C - 1);
+    const double ic = a.icenter, cen = a.center;
+    float* const zpad = tile + kRows * kMaxMinor * RS;
+    for (int e = tid; e < kMaxK * RS; e += 256) zpad[e] = 0.0f;
     if (tid < 4) {
         const int ia = min(blockIdx.y * 4 + tid, N - 1);
         const float th = -a.angles[ia];
@@ -72,19 +220,38 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
         const double sl = by_rows ? s_cs[tid][0] : s_cs[tid][1];
         s_inv[tid] = fabs(sl) >= kMinSlope ? 1.0 / sl : 0.0;
     }
+    // f = (k - ic) * slope + base(m) along a line m of the walking axis
     const double slope = by_rows ? c : s;
-    // windows only need to be a superset (every candidate is tested with
-    // the exact f), so a reciprocal replaces the per-line divisions
+    const double other = by_rows ? s : c;
     const bool steep = fabs(slope) >= kMinSlope;
     const double inv_slope = steep ? 1.0 / slope : 0.0;
+    // candidates per line (wave-uniform): the integers in an open interval
+    // of length 2 / |slope|
+    const int K = __builtin_amdgcn_readfirstlane(
+        steep ? (int)ceil(2.0 * fabs(inv_slope)) : kMaxK + 1);
+    const bool fast_wave = K <= kMaxK;
+    // the crossing f = d - sign(slope) starts the candidate run
+    const double dd = (double)d;
+    const double dedge = slope > 0.0 ? dd - 1.0 : dd + 1.0;
+    // 0 <= f <= ncol - 1 matters only for bins 0 and ncol - 1 (elsewhere a
+    // positive weight already puts f inside): t = f - d must stay >= 0 for
+    // bin 0 and <= 0 for the last bin
+    const bool edge_block = __builtin_amdgcn_readfirstlane(
+                                (d0 == 0 || d0 + 64 > C - 1) ? 1 : 0) != 0;
+    const double t_lo = d == 0 ? 0.0 : -2.0;
+    const double t_hi = d == C - 1 ? 0.0 : 2.0;
+    float acc32[FSB];
     double acc[FSB];
 #pragma unroll
-    for (int j = 0; j < FSB; ++j) acc[j] = 0.0;
+    for (int j = 0; j < FSB; ++j) {
+        acc32[j] = 0.0f;
+        acc[j] = 0.0;
+    }
 
     for (int m0 = 0; m0 < C; m0 += kRows) {
         const int m1 = min(m0 + kRows, C) - 1;
         // union window of the minor coordinate over 4 angles, bins
-        // [d0 - 1, d0 + 64] and lines m0, m1 (windows are linear in m):
+        // [d0 - 2, d0 + 65] and lines m0, m1 (windows are linear in m):
         // lanes 0-15 of wave 0 each evaluate one end, reduce min / max
         // across those 16 lanes and lane 0 publishes the column range
         if (tid < 64) {
@@ -108,19 +275,21 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
                 hi = fmax(hi, __shfl_xor(hi, off));
             }
             // a negative inverse swaps the two bin ends: min / max over
-            // all sixteen covers both orders
+            // all sixteen covers both orders.  The range is not clipped to
+            // the image: columns outside it are staged as zeros.
             if (tid == 0) {
-                s_ab[0] = lo <= -1e299 ? 0
-                                       : max((int)floor(lo + ic) - 1, 0);
-                s_ab[1] = hi >= 1e299 ? C - 1
-                                      : min((int)ceil(hi + ic) + 1, C - 1);
+                const bool whole = lo <= -1e299 || hi >= 1e299 ||
+                                   hi - lo > 2.0 * kMaxMinor;
+                s_ab[0] = whole ? 0 : (int)floor(lo + ic) - 1;
+                s_ab[1] = whole ? C - 1 + kMaxMinor
+                                : (int)ceil(hi + ic) + 1;
             }
         }
         __syncthreads();
         const int ka = s_ab[0], kb = s_ab[1];
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
-        if (wid > 0 && staged) {
+        if (staged) {
             // one pixel per thread and step: the mask test once, then its
             // FSB slices (kMaxMinor = 128 = 2^7 columns per line)
             const int nrows = m1 - m0 + 1;
@@ -134,19 +303,26 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
                     k = e / kRows;
                 }
                 if (r >= nrows || k >= wid) continue;
-                const int x = by_rows ? ka + k : m0 + r;
-                const int y = by_rows ? m0 + r : ka + k;
-                bool keep = true;
-                if (a.use_mask) {
-                    const long long dx = x - a.mask_c, dy = y - a.mask_c;
-                    keep = (double)(dx * dx + dy * dy) <= a.mask_r2;
+                const int kk = ka + k;
+                const int x = by_rows ? kk : m0 + r;
+                const int y = by_rows ? m0 + r : kk;
+                bool keep = kk >= 0 && kk < C;
+                if (keep && a.use_mask) {
+                    // integer squares and their sum are exact in float64
+                    const double dx = x - a.mask_c, dy = y - a.mask_c;
+                    keep = fma(dx, dx, dy * dy) <= a.mask_r2;
                 }
-                const float* src = a.image + ((size_t)y * S + j0) * C + x;
                 float v[FSB];
+                if (keep) {
+                    const float* src = a.image + ((size_t)y * S + j0) * C + x;
 #pragma unroll
-                for (int js = 0; js < FSB; ++js)
-                    v[js] = (keep && js < ns) ? src[(size_t)js * C] : 0.0f;
-                float* dst = tile + (r * kMaxMinor + k) * FSB;
+                    for (int js = 0; js < FSB; ++js)
+                        v[js] = js < ns ? src[(size_t)js * C] : 0.0f;
+                } else {
+#pragma unroll
+                    for (int js = 0; js < FSB; ++js) v[js] = 0.0f;
+                }
+                float* dst = tile + (r * kMaxMinor + k) * RS;
                 if constexpr (FSB % 4 == 0) {
 #pragma unroll
                     for (int q = 0; q < FSB / 4; ++q)
@@ -159,65 +335,55 @@ __global__ __launch_bounds__(256) void forward_project_lds_kernel(
             }
         }
         __syncthreads();
-        if (valid && wid > 0) {
-            const double lo_t = (double)d - 1.0, hi_t = (double)d + 1.0;
-            for (int m = m0; m <= m1; ++m) {
-                const double mc = (double)m - ic;
-                const double base = by_rows ? mc * s + cen : mc * c + cen;
-                int k0 = 0, k1 = C - 1;
-                if (steep) {
-                    double u0 = (lo_t - base) * inv_slope;
-                    double u1 = (hi_t - base) * inv_slope;
-                    if (u0 > u1) {
-                        const double t = u0;
-                        u0 = u1;
-                        u1 = t;
+        if (valid) {
+            if (staged && fast_wave && wid >= K) {
+                const double u0 =
+                    (dedge - (((double)m0 - ic) * other + cen)) * inv_slope +
+                    ic;
+                const double du = -other * inv_slope;
+                const unsigned long long bad =
+                    edge_block
+                        ? run_lines_k<FSB, RS, true>(
+                              K, tile, zpad, m0, m1, ka, wid, u0, du, slope,
+                              dedge - dd, t_lo, t_hi, acc32)
+                        : run_lines_k<FSB, RS, false>(
+                              K, tile, zpad, m0, m1, ka, wid, u0, du, slope,
+                              dedge - dd, t_lo, t_hi, acc32);
+                if ((bad >> lane) & 1ull) {
+                    // this lane's runs that left the staged range (the
+                    // same test as in run_lines), from global memory
+                    double u = u0;
+                    for (int m = m0; m <= m1; ++m, u += du) {
+                        const int k0 = (int)floor(u) + 1 - ka;
+                        if ((unsigned)k0 <= (unsigned)(wid - K)) continue;
+                        gather_line_global<FSB>(
+                            a, m, by_rows, steep, slope, inv_slope,
+                            ((double)m - ic) * other + cen, dd, t_lo, t_hi,
+                            j0, ns, acc32);
                     }
-                    // contributors have f in [d-1, d+1), i.e. k - ic in
-                    // [u0, u1]; u carries a relative error of a few ulp
-                    // (reciprocal slope), so a 1e-6 px margin keeps the
-                    // candidate list a superset (each is tested exactly).
-                    // Every candidate costs the whole slice loop once any
-                    // lane of the wave contributes, so the list is kept
-                    // to the integers inside the margin.
-                    k0 = max((int)ceil(u0 + ic - 1e-6), 0);
-                    k1 = min((int)floor(u1 + ic + 1e-6), C - 1);
                 }
-                if (staged) {
-                    k0 = max(k0, ka);
-                    k1 = min(k1, kb);
-                }
-                for (int k = k0; k <= k1; ++k) {
-                    const int x = by_rows ? k : m;
-                    const int y = by_rows ? m : k;
-                    const double f = fp_det_pos(((double)x - ic) * c,
-                                                (double)y - ic, s, cen);
-                    if (!(f >= 0.0 && f <= w1)) continue;
-                    const double fl = floor(f);
-                    const double t = f - fl;
-                    double w;
-                    if (fl == (double)d) w = 1.0 - t;
-                    else if (fl == (double)(d - 1)) w = t;
-                    else continue;
-                    if (staged) {
-                        const float* q =
-                            tile + ((m - m0) * kMaxMinor + (k - ka)) * FSB;
+            } else {
+                // global memory, one line at a time (a near-parallel line
+                // holds many candidates: fold per line)
+                for (int m = m0; m <= m1; ++m) {
+                    gather_line_global<FSB>(a, m, by_rows, steep, slope,
+                                            inv_slope,
+                                            ((double)m - ic) * other + cen,
+                                            dd, t_lo, t_hi, j0, ns, acc32);
 #pragma unroll
-                        for (int j = 0; j < FSB; ++j)
-                            acc[j] += w * (double)q[j];
-                    } else {
-                        if (a.use_mask) {
-                            const long long dx = x - a.mask_c,
-                                            dy = y - a.mask_c;
-                            if (!((double)(dx * dx + dy * dy) <= a.mask_r2))
-                                continue;
-                        }
-                        const float* px =
-                            a.image + ((size_t)y * S + j0) * C + x;
-#pragma unroll
-                        for (int j = 0; j < FSB; ++j)
-                            if (j < ns) acc[j] += w * (double)px[(size_t)j * C];
+                    for (int j = 0; j < FSB; ++j) {
+                        acc[j] += (double)acc32[j];
+                        acc32[j] = 0.0f;
                     }
+                }
+            }
+            // float32 sums over two chunks (at most 2 * kRows * K terms),
+            // then into float64
+            if (((m0 / kRows) & 1) || m1 == C - 1) {
+#pragma unroll
+                for (int j = 0; j < FSB; ++j) {
+                    acc[j] += (double)acc32[j];
+                    acc32[j] = 0.0f;
                 }
             }
         }
@@ -249,18 +415,18 @@ int forward_projection(const FpArgs& a, hipStream_t stream) {
     const dim3 grid((a.ncol + 63) / 64, (a.nrow + 3) / 4,
                     (a.nslice + fsb - 1) / fsb);
     if (fsb == 16)
-        hipLaunchKernelGGL((forward_project_lds_kernel<16, 4>), grid,
+        hipLaunchKernelGGL((forward_project_hat_kernel<16, 4>), grid,
                            dim3(256), 0, stream, a, -1);
     else if (fsb == 8)
-        hipLaunchKernelGGL((forward_project_lds_kernel<8, 4>), grid,
+        hipLaunchKernelGGL((forward_project_hat_kernel<8, 4>), grid,
                            dim3(256), 0, stream, a, -1);
     else if (fsb == 4)
-        hipLaunchKernelGGL((forward_project_lds_kernel<4, 8>), grid,
+        hipLaunchKernelGGL((forward_project_hat_kernel<4, 8>), grid,
                            dim3(256), 0, stream, a, -1);
     else
-        hipLaunchKernelGGL((forward_project_lds_kernel<1, 16>), grid,
+        hipLaunchKernelGGL((forward_project_hat_kernel<1, 16>), grid,
                            dim3(256), 0, stream, a, -1);
-    return check_hip(hipGetLastError(), "forward_project_lds_kernel launch");
+    return check_hip(hipGetLastError(), "forward_project_hat_kernel launch");
 }
 
 }  // namespace ath

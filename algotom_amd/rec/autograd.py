@@ -19,7 +19,11 @@ Backward:
 Checkpointing: ``checkpoint="filtered"`` keeps the filtered sinogram from the
 forward pass (needed by dL/dc); ``checkpoint="input"`` keeps only the input
 and re-filters in backward, trading one filter launch for the memory of a
-second copy of the stack.
+second copy of the stack; ``checkpoint="derivative"`` computes the centre
+derivative image in the forward pass, in the same launch as the
+back-projection (ath_back_projection_with_derivative: the positions and
+staged samples are shared, each output bit-identical to its separate pass),
+and keeps that image instead, so backward runs no derivative pass.
 """
 
 import numpy as np
@@ -37,8 +41,9 @@ class FBPGeometry:
         import torch
         if pad_mode not in nat.PAD_MODES:
             raise NotImplementedError("pad_mode %r" % pad_mode)
-        if checkpoint not in ("filtered", "input"):
-            raise ValueError("checkpoint must be 'filtered' or 'input'")
+        if checkpoint not in ("filtered", "input", "derivative"):
+            raise ValueError("checkpoint must be 'filtered', 'input' or "
+                             "'derivative'")
         self.dev = nat.device()
         self.lib = nat.load()
         self.nrow, self.ncol = int(nrow), int(ncol)
@@ -85,6 +90,21 @@ class FBPGeometry:
                                       mask_radius=self.mask_radius,
                                       apply_scale=True)
 
+    def backproject_with_derivative(self, filt, center):
+        """(sigma * mask * B_c filt, sigma * mask * dB_c/dc filt) from one
+        launch."""
+        import torch
+        nrow, nslice, ncol = filt.shape
+        out = torch.empty((ncol, nslice, ncol), dtype=torch.float32,
+                          device=self.dev)
+        der = torch.empty_like(out)
+        nat.check(self.lib.ath_back_projection_with_derivative(
+            nat.ptr(filt), nrow, nslice, ncol, nat.ptr(self.angles),
+            float(np.float32(center)), 0, self.mask_radius, 1, nat.ptr(out),
+            nat.ptr(der), nat.stream_ptr(self.dev)),
+            "ath_back_projection_with_derivative")
+        return out, der
+
     def forward_project(self, image, center):
         """sigma * B^T(mask * image): (ncol, nslice, ncol) -> (nrow, ...)."""
         import torch
@@ -118,9 +138,16 @@ def _function():
             s = sino.detach().to(torch.float32).contiguous()
             c = float(center.detach().double().item())
             filt = geom.filter(s)
-            out = geom.backproject(filt, c)
             ctx.geom = geom
             ctx.center = c
+            if geom.checkpoint == "derivative":
+                if ctx.needs_input_grad[1]:
+                    out, deriv = geom.backproject_with_derivative(filt, c)
+                    ctx.save_for_backward(deriv)
+                else:
+                    out = geom.backproject(filt, c)
+                return out
+            out = geom.backproject(filt, c)
             ctx.save_for_backward(filt if geom.checkpoint == "filtered"
                                   else s)
             return out
@@ -128,16 +155,19 @@ def _function():
         @staticmethod
         def backward(ctx, grad):
             geom = ctx.geom
-            (saved,) = ctx.saved_tensors
+            saved = ctx.saved_tensors[0] if ctx.saved_tensors else None
             g = grad.detach().to(torch.float32).contiguous()
             g_sino = g_center = None
             if ctx.needs_input_grad[0]:
                 g_sino = geom.filter_adjoint(geom.forward_project(g,
                                                                  ctx.center))
             if ctx.needs_input_grad[1]:
-                filt = saved if geom.checkpoint == "filtered" else \
-                    geom.filter(saved)
-                deriv = geom.backproject(filt, ctx.center, nat.ACC_DERIV)
+                if geom.checkpoint == "derivative":
+                    deriv = saved
+                else:
+                    filt = saved if geom.checkpoint == "filtered" else \
+                        geom.filter(saved)
+                    deriv = geom.backproject(filt, ctx.center, nat.ACC_DERIV)
                 g_center = torch.sum(g.double() * deriv.double()).reshape(())
             return g_sino, g_center, None
 

@@ -10,11 +10,18 @@ stack (log, pad, ramp filter, back-projection, mask, scale).
 value = pixel-angle updates per second (GUPS = 1e9 updates/s),
 U = 64 * 2560^2 * 1801 = 7.554e11 updates per step.
 
-roofline (back-projection kernel, the dominant one): algorithmic bytes per
-launch = U * 8 B (two float32 taps per update, the gather-byte model) over
-the kernel's mean duration from HIP events on the launch stream; peak = 8 TB/s.
+roofline (back-projection kernel, the dominant one): it is a gather with no
+MFMA work and its HBM traffic is 8 % of peak, so the bound is VALU issue.
+achieved = the kernel's VALU instructions per launch (SQ_INSTS_VALU, from the
+committed rocprofv3 counters of this kernel and workload,
+profiles/bp_counters.json) x 64 lanes over the kernel's mean duration from
+HIP events on the launch stream; peak = 256 CU x 64 lanes x 2.4 GHz = 39.3 T
+lane-op/s; traffic = the same profile's calibrated FETCH_SIZE + WRITE_SIZE
+bytes per launch.  The gather-byte model (U x 8 B) and the compulsory-byte
+HBM fraction are reported in detail.
 cpu_baseline: the oracle's OpenMP C port (numba-typed restatement of the
-reference's CPU path) on 4 of the 64 sinograms, on this host's cores.
+reference's CPU path) on 4 of the 64 sinograms, on this host's cores
+(median of 5 after a warm-up).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--precision fp32]
 
@@ -22,8 +29,9 @@ reference's CPU path) on 4 of the 64 sinograms, on this host's cores.
 sinograms of 901 angles over a 120-degree arc x 1024 columns (seed 4000),
 Gaussian noise sigma = 1% of max; loss = ||FBP(s; c) - phantom||^2 +
 1e-3 TV(FBP(s; c)); Adam on the sinogram s and the centre c (started 1.5 px
-off) through the hand-written adjoint; value = grad steps/s over --steps
-steps (default 200).
+off) through the hand-written adjoint, the centre derivative image computed
+in the forward back-projection's launch (checkpoint "derivative"); value =
+grad steps/s over --steps steps (default 200).
 
 --workload volume: 128 slices of a 3601-angle x 4096-column half-acquisition
 scan over [0, 2 pi) with the rotation axis at column 499.5 (overlap 1000),
@@ -54,6 +62,11 @@ slice, then a sub-pixel pass; every pass smooths, logs and filters once and
 back-projects all its centres in one batched launch.  value = candidate
 slices reconstructed and scored per second.
 
+--workload c1: the plain-FBP CPU workload at full size (721 x 512
+Fourier-slice sinogram of a 20-ellipse phantom, committed under
+tests/golden/), the CPU back-projector's arithmetic; the GPU time beside the
+oracle's C port timed on the host cores.
+
 --workload bpf: BPF (back-projection filtering) of a 16 x 1801 x 2560
 transmission stack (seed 1000): log, edge-padded back-projection, the 2D
 circular ramp filter (FFT length 8192) of every slice, mask and scale;
@@ -81,6 +94,8 @@ sys.path.insert(0, ROOT)
 
 NSLICE, NROW, NCOL, CENTER = 64, 1801, 2560, 1280.37
 PEAK_HBM_GBS = 8000.0
+# MI355X VALU issue peak: 256 CUs x 64 lanes per clock x 2.4 GHz
+PEAK_VALU_TLANE = 256 * 64 * 2.4e9 / 1e12
 METRIC = ("GUPS (pixel·angle updates/s) FBP 2560²×1801 ang; "
           "% HBM roofline; grad steps/s")
 
@@ -119,32 +134,47 @@ def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "bp_traffic.json")
+def load_counters():
+    """rocprofv3 counters of the reference-arithmetic back-projection kernel
+    on this workload (profiles/bp_counters.json), or None."""
+    path = os.path.join(ROOT, "profiles", "bp_counters.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
 
 
+def time_cpu(fn, runs=5):
+    """One warm-up call, then the median (and spread) of `runs` timed calls
+    of fn(); returns (median_s, [times], last result)."""
+    fn()
+    times, res = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        res = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times, res
+
+
 def cpu_baseline(sino_dev, angles, gpu_out, nsample=4):
-    """Times the oracle C port on nsample slices; also checks parity of the
-    GPU result on those slices."""
+    """Times the oracle C port on nsample slices (median of 5 after a
+    warm-up of the same size); also checks parity of the GPU result on
+    those slices."""
     from oracle import port
     sample = np.ascontiguousarray(sino_dev[:, :nsample, :].cpu().numpy())
     threads = port.default_threads()
-    port.fbp(sample[:, :1, :], CENTER, angles, nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    ref = port.fbp(sample, CENTER, angles, nthreads=threads)
-    dt = time.perf_counter() - t0
+    dt, times, ref = time_cpu(lambda: port.fbp(sample, CENTER, angles,
+                                               nthreads=threads))
     updates = nsample * NCOL * NCOL * NROW
     got = gpu_out[:, :nsample, :].cpu().numpy()
     err = float(np.abs(got.astype(np.float64) - ref).max()
                 / np.abs(ref).max())
     return {"value": updates / dt / 1e9, "unit": "GUPS", "cores": threads,
             "kind": "port",
-            "sample": "%d of %d sinograms (%dx%d), log+filter+BP+mask+scale, "
-                      "%.2f s" % (nsample, NSLICE, NROW, NCOL, dt)}, err
+            "sample": "%d of %d sinograms (%dx%d), log+filter+BP+mask+scale; "
+                      "median of 5 after a warm-up: %.2f s (%.2f-%.2f s)"
+                      % (nsample, NSLICE, NROW, NCOL, dt, min(times),
+                         max(times))}, err
 
 
 def host_roundtrip(sino_dev, angles, reps=2):
@@ -439,7 +469,7 @@ def run_refine(args, dev):
     sino = torch.stack(sinos, dim=1).contiguous()
     target = torch.stack(targets, dim=1).to(dev)
     geom = autograd.FBPGeometry(nrow, ncol, angles, ratio=1.0,
-                                checkpoint="filtered")
+                                checkpoint="derivative")
     s_param = sino.clone().requires_grad_(True)
     c_param = torch.tensor(c_true + 1.5, dtype=torch.float64,
                            requires_grad=True)
@@ -474,7 +504,7 @@ def run_refine(args, dev):
         "data": "synthetic",
         "config": {"workload": "fbp_refine_16x901x1024_arc120",
                    "nslice": nsl, "nrow": nrow, "ncol": ncol,
-                   "checkpoint": "filtered", "parallelism": "single-gpu"},
+                   "checkpoint": "derivative", "parallelism": "single-gpu"},
         "detail": {"loss_first": first, "loss_last": float(loss.item()),
                    "center_start": c_true + 1.5,
                    "center_end": float(c_param.item()),
@@ -587,6 +617,67 @@ def run_bpf(args, dev):
                    "single_sinogram_ms (float64 output)": ms2d,
                    "single_sinogram_1801x4096_ms": msw,
                    "finite": bool(torch.isfinite(out).all().item())},
+    }
+    print(json.dumps(line))
+
+
+def run_c1(args, dev):
+    """The plain-FBP CPU workload at full size: the committed 721 x 512
+    Fourier-slice sinogram of a 20-ellipse phantom (tests/golden/
+    c1_fourier_slice.npz), centre 256.0, hann, edge padding, apply_log off,
+    the CPU back-projector's arithmetic (gpu=False).  value = GUPS of
+    fbp_reconstruction on the device-resident sinogram; cpu_baseline = the
+    oracle's C port of the same call on this host's cores (median of 5
+    after a warm-up), its parity and both errors against the fixture's
+    float64 FBP in detail."""
+    import torch
+    from algotom_amd.rec import reconstruction as rec
+    from oracle import port
+    path = os.path.join(ROOT, "tests", "golden", "c1_fourier_slice.npz")
+    with np.load(path, allow_pickle=False) as d:
+        sino_np, angles, ref64 = d["sinogram"], d["angles"], d["fbp_f64"]
+    nrow, ncol = sino_np.shape
+    updates = nrow * ncol * ncol
+    sino = torch.from_numpy(sino_np).to(dev)
+    kw = dict(apply_log=False, gpu=False)
+    ms, out = _timed(lambda: rec.fbp_reconstruction(sino, 256.0, angles,
+                                                    **kw),
+                     max(args.steps, 20), args.warmup)
+    got = out.cpu().numpy()
+    scale = np.abs(ref64).max()
+    detail = {"gpu_max_rel_err_vs_fp64": float(np.abs(got - ref64).max()
+                                               / scale)}
+    t0 = time.perf_counter()
+    rec.fbp_reconstruction(sino_np, 256.0, angles, **kw)
+    detail["host_numpy_ms"] = (time.perf_counter() - t0) * 1e3
+    base = None
+    if not args.no_cpu_baseline:
+        threads = port.default_threads()
+        dt, times, ref = time_cpu(lambda: port.fbp(sino_np, 256.0, angles,
+                                                   nthreads=threads, **kw))
+        detail["cpu_max_rel_err_vs_fp64"] = float(np.abs(ref - ref64).max()
+                                                  / scale)
+        detail["gpu_vs_cpu_port_max_rel"] = float(np.abs(got - ref).max()
+                                                  / np.abs(ref).max())
+        detail["cpu_port_ms"] = dt * 1e3
+        base = {"value": updates / dt / 1e9, "unit": "GUPS",
+                "cores": threads, "kind": "port",
+                "sample": "the whole workload (721x512, filter+BP+mask+"
+                          "scale); median of 5 after a warm-up: %.1f ms "
+                          "(%.1f-%.1f ms)" % (dt * 1e3, min(times) * 1e3,
+                                             max(times) * 1e3)}
+    line = {
+        "metric": METRIC, "value": updates / (ms * 1e-3) / 1e9,
+        "unit": "GUPS", "n_gpus": 1, "steps": max(args.steps, 20),
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (committed fixture)",
+        "config": {"workload": "fbp_c1_721x512_fourier_slice",
+                   "nrow": nrow, "ncol": ncol, "center": 256.0,
+                   "apply_log": False, "filter": "hann",
+                   "arithmetic": "back_projection_cpu (gpu=False)",
+                   "parallelism": "single-gpu"},
+        "cpu_baseline": base, "detail": detail,
     }
     print(json.dumps(line))
 
@@ -715,7 +806,7 @@ def main():
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-numpy round trip (profiling runs)")
     ap.add_argument("--workload", default="stack",
-                    choices=["stack", "uneven", "refine", "volume",
+                    choices=["stack", "c1", "uneven", "refine", "volume",
                              "volume_refine",
                              "center", "bpf", "vertrec", "dfi"])
     args = ap.parse_args()
@@ -748,6 +839,8 @@ def main():
         return
     if args.workload == "dfi":
         return run_dfi(args, dev)
+    if args.workload == "c1":
+        return run_c1(args, dev)
     if args.workload == "bpf":
         run_bpf(args, dev)
         return
@@ -788,14 +881,22 @@ def main():
     dt = time.perf_counter() - t0
     bp_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     ms = dt / args.steps * 1e3
-    achieved = updates * 8.0 / (bp_ms * 1e-3) / 1e9
+    gather_gbs = updates * 8.0 / (bp_ms * 1e-3) / 1e9
 
     copy_gbs = copy_bandwidth(dev)
     host_ms = None if args.no_host else host_roundtrip(sino, angles)
     base, err = (None, None)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(sino, angles, out)
-    traffic = load_traffic()
+    cnt = load_counters() if args.precision is None else None
+    roof = {"bound": "valu", "achieved": None, "peak": PEAK_VALU_TLANE,
+            "unit": "T VALU lane-op/s", "frac": None, "traffic": None}
+    if cnt:
+        roof["achieved"] = (cnt["valu_insts_per_launch"] * 64.0
+                            / (bp_ms * 1e-3) / 1e12)
+        roof["frac"] = roof["achieved"] / PEAK_VALU_TLANE
+        roof["traffic"] = cnt["traffic_bytes"]
+    compulsory = 4.0 * NROW * NSLICE * NCOL + 4.0 * NCOL * NSLICE * NCOL
     line = {
         "metric": METRIC,
         "value": updates / (ms * 1e-3) / 1e9,
@@ -815,14 +916,21 @@ def main():
                    "apply_log": True, "filter": "hann", "pad": pad,
                    "precision": args.precision or "reference",
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": (traffic or {}).get("bytes_per_launch")},
+        "roofline": roof,
         "cpu_baseline": base,
         "detail": {"bp_kernel_ms": bp_ms, "filter_and_rest_ms": ms - bp_ms,
                    "hbm_copy_GBs": copy_gbs,
-                   "frac_of_measured_copy_bw": achieved / copy_gbs,
+                   "bp_valu_lane_ops_per_update":
+                       cnt["valu_lane_ops_per_update"] if cnt else None,
+                   "bp_hbm_GBs": (cnt["traffic_bytes"] / (bp_ms * 1e-3) / 1e9
+                                  if cnt else None),
+                   "bp_hbm_frac_of_peak": (cnt["traffic_bytes"]
+                                           / (bp_ms * 1e-3) / 1e9
+                                           / PEAK_HBM_GBS if cnt else None),
+                   "compulsory_bytes": compulsory,
+                   "compulsory_frac_of_hbm_peak":
+                       compulsory / (bp_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                   "gather_byte_model_GBs": gather_gbs,
                    "bp_gups": updates / (bp_ms * 1e-3) / 1e9,
                    "host_numpy_roundtrip_ms": host_ms,
                    "host_numpy_roundtrip_gups":

@@ -72,6 +72,9 @@ extern "C" {
  *                 derivative of the back-projection with respect to the
  *                 rotation centre (float64 sum; no reference counterpart) */
 #define ATH_ACC_DERIV 3
+/*  ATH_ACC_F64_DERIV  ATH_ACC_F64 and ATH_ACC_DERIV in one pass (two
+ *                 outputs, ath_back_projection_with_derivative only) */
+#define ATH_ACC_F64_DERIV 4
 
 /* One stack of a batched launch (ath_ramp_filter_batched): rows of
  * `sino` (nrow, nslice, ncol) are filtered into `out`; `plan` comes from
@@ -166,6 +169,20 @@ int ath_back_projection(const void* sino, int sino_dtype, int nrow,
                         float center, int edge_pad, int acc,
                         double mask_radius, int apply_scale, float* recon,
                         void* stream);
+
+/* The back-projection of ath_back_projection with acc ATH_ACC_F64 into
+ * recon and, in the same pass over the same positions and staged samples,
+ * its derivative with respect to the rotation centre (acc ATH_ACC_DERIV)
+ * into deriv (ncol, nslice, ncol) float32, both with the mask and scale;
+ * each output is bit-identical to its separate call.  sino: float32.  No
+ * reference counterpart: the forward pass of differentiable FBP when the
+ * centre needs a gradient, saving the derivative's own pass. */
+int ath_back_projection_with_derivative(const float* sino, int nrow,
+                                        int nslice, int ncol,
+                                        const float* angles, float center,
+                                        int edge_pad, double mask_radius,
+                                        int apply_scale, float* recon,
+                                        float* deriv, void* stream);
 
 /* ath_back_projection with an explicit tile configuration: slice_batch
  * slices per workgroup (1, 2, 4; 8 for float16/bfloat16) and tile_rows

@@ -96,7 +96,7 @@ def _fp64_loss(sino, center, angles, target, ratio=1.0):
 
 
 @pytest.mark.parametrize("arc", ARCS)
-@pytest.mark.parametrize("checkpoint", ["filtered", "input"])
+@pytest.mark.parametrize("checkpoint", ["filtered", "input", "derivative"])
 def test_gradients_vs_dense_and_finite_differences(checkpoint, arc):
     import torch
     from oracle import fbp_np
@@ -155,3 +155,58 @@ def test_center_derivative_image_vs_fd(arc):
     # all others must agree to float32 rounding
     close = np.abs(d_img - fd) <= 1e-4 * np.abs(fd).max()
     assert close.mean() > 0.999
+
+
+@pytest.mark.parametrize("shape,center,mask", [
+    ((91, 3, 64), 31.7, 32.0), ((181, 1, 97), 40.2, -1.0),
+    ((60, 5, 130), 70.9, 65.0)])
+def test_fused_derivative_pass_is_bitwise_the_two_passes(shape, center,
+                                                          mask):
+    """ath_back_projection_with_derivative: both outputs equal the separate
+    ATH_ACC_F64 and ATH_ACC_DERIV calls bit for bit."""
+    import torch
+    from algotom_amd import _native as nat
+    from algotom_amd.rec import reconstruction as rec
+    nrow, nslice, ncol = shape
+    rng = np.random.default_rng(nrow)
+    filt = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)
+                            ).cuda()
+    angles = np.linspace(0.0, np.pi, nrow, endpoint=False)
+    ang = rec._angles_device(angles, filt.device)
+    out = torch.empty((ncol, nslice, ncol), dtype=torch.float32,
+                      device=filt.device)
+    der = torch.empty_like(out)
+    nat.check(nat.load().ath_back_projection_with_derivative(
+        nat.ptr(filt), nrow, nslice, ncol, nat.ptr(ang),
+        float(np.float32(center)), 0, mask, 1, nat.ptr(out), nat.ptr(der),
+        nat.stream_ptr(filt.device)), "with_derivative")
+    ref = rec._backproject_stack(filt, angles, center, nat.ACC_F64,
+                                 mask_radius=mask, apply_scale=True)
+    dref = rec._backproject_stack(filt, angles, center, nat.ACC_DERIV,
+                                  mask_radius=mask, apply_scale=True)
+    assert torch.equal(out, ref)
+    assert torch.equal(der, dref)
+
+
+def test_derivative_checkpoint_matches_filtered_gradients():
+    """checkpoint='derivative' gives the same gradients as 'filtered',
+    and runs without a centre gradient too."""
+    import torch
+    from algotom_amd.rec import autograd
+    rng, angles, center, sino = _setup(nrow=128, ncol=64, arc=ARCS[0])
+    grads = {}
+    for ck in ("filtered", "derivative"):
+        geom = autograd.FBPGeometry(128, 64, angles, ratio=1.0,
+                                    checkpoint=ck)
+        s = torch.from_numpy(sino.copy()).cuda().requires_grad_(True)
+        c = torch.tensor(float(center), dtype=torch.float64,
+                         requires_grad=True)
+        y = autograd.fbp_differentiable(s, c, geom)
+        torch.sum(y * y).backward()
+        grads[ck] = (s.grad.clone(), c.grad.clone())
+        s2 = torch.from_numpy(sino.copy()).cuda().requires_grad_(True)
+        y2 = autograd.fbp_differentiable(s2, float(center), geom)
+        torch.sum(y2 * y2).backward()
+        assert torch.equal(s2.grad, grads[ck][0])
+    assert torch.equal(grads["filtered"][0], grads["derivative"][0])
+    assert torch.equal(grads["filtered"][1], grads["derivative"][1])
