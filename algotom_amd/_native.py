@@ -62,6 +62,9 @@ SIGNATURES = (
     ("ath_back_projection_naive", _I, [_P, _I, _I, _I, _P, _F, _I, _I, _D,
                                        _I, _P, _P]),
     ("ath_forward_projection", _I, [_P, _I, _I, _I, _P, _F, _D, _D, _P, _P]),
+    ("ath_forward_projection_workspace_bytes", _S, [_I, _I]),
+    ("ath_forward_projection_ws", _I,
+     [_P, _I, _I, _I, _P, _F, _D, _D, _P, _S, _P, _P]),
     ("ath_ramp_filter_adjoint", _I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _P,
                                      _P]),
     ("ath_conversion_workspace_bytes", _S, [_I, _I, _I, _I, _D]),

@@ -113,6 +113,7 @@ int back_projection_naive(const BpArgs& a, int acc, hipStream_t stream);
 // ---- forward projection = adjoint of the back-projection ------------------
 struct FpArgs {
     const float* image;    // (ncol, nslice, ncol) = (y, slice, x)
+    const float* image_t;  // nullptr, or the same image as (x, slice, y)
     float* sino;           // (nrow, nslice, ncol)
     const float* angles;
     int nrow, nslice, ncol;
@@ -123,6 +124,9 @@ struct FpArgs {
     double scale;          // multiplies every output
 };
 int forward_projection(const FpArgs& a, hipStream_t stream);
+// out(x, j, y) = in(y, j, x) for the (ncol, nslice, ncol) image
+int transpose_slices(const float* in, float* out, int nslice, int ncol,
+                     hipStream_t stream);
 int gaussian_rows(const float* in, float* out, int nrows, int ncol,
                   const double* weights, int radius, hipStream_t s);
 size_t circular_plan_bytes(int L);

@@ -458,20 +458,33 @@ int ath_back_projection_naive(const float* sino, int nrow, int nslice,
     return back_projection_naive(a, acc, static_cast<hipStream_t>(stream));
 }
 
-int ath_forward_projection(const float* image, int nrow, int nslice,
-                           int ncol, const float* angles, float center,
-                           double mask_radius, double scale, float* sino,
-                           void* stream) {
+namespace {
+
+int forward_projection_entry(const float* image, int nrow, int nslice,
+                             int ncol, const float* angles, float center,
+                             double mask_radius, double scale, void* work,
+                             size_t work_bytes, bool use_work, float* sino,
+                             void* stream) {
     clear_error();
     if (!image || !angles || !sino)
         return set_error(ATH_ERR_INVALID_ARG, "null device pointer");
     int st = check_geometry(nrow, nslice, ncol);
     if (st != ATH_OK) return st;
-    if (nrow > 65535)
-        return set_error(ATH_ERR_UNSUPPORTED, "forward projection: nrow > 65535");
     if (!std::isfinite(center) || !std::isfinite(scale))
         return set_error(ATH_ERR_INVALID_ARG, "center and scale must be finite");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
     FpArgs a;
+    a.image_t = nullptr;
+    if (use_work) {
+        if (!work ||
+            work_bytes < ath_forward_projection_workspace_bytes(nslice, ncol))
+            return set_error(ATH_ERR_WORKSPACE,
+                             "forward projection: workspace too small");
+        float* t = static_cast<float*>(work);
+        st = transpose_slices(image, t, nslice, ncol, s);
+        if (st != ATH_OK) return st;
+        a.image_t = t;
+    }
     a.image = image;
     a.sino = sino;
     a.angles = angles;
@@ -484,7 +497,33 @@ int ath_forward_projection(const float* image, int nrow, int nslice,
     a.mask_r2 = mask_radius * mask_radius;
     a.mask_c = ncol / 2;
     a.scale = scale;
-    return forward_projection(a, static_cast<hipStream_t>(stream));
+    return forward_projection(a, s);
+}
+
+}  // namespace
+
+int ath_forward_projection(const float* image, int nrow, int nslice,
+                           int ncol, const float* angles, float center,
+                           double mask_radius, double scale, float* sino,
+                           void* stream) {
+    return forward_projection_entry(image, nrow, nslice, ncol, angles,
+                                    center, mask_radius, scale, nullptr, 0,
+                                    false, sino, stream);
+}
+
+size_t ath_forward_projection_workspace_bytes(int nslice, int ncol) {
+    if (nslice <= 0 || ncol <= 0) return 0;
+    return (size_t)ncol * (size_t)nslice * (size_t)ncol * sizeof(float);
+}
+
+int ath_forward_projection_ws(const float* image, int nrow, int nslice,
+                              int ncol, const float* angles, float center,
+                              double mask_radius, double scale,
+                              void* workspace, size_t workspace_bytes,
+                              float* sino, void* stream) {
+    return forward_projection_entry(image, nrow, nslice, ncol, angles,
+                                    center, mask_radius, scale, workspace,
+                                    workspace_bytes, true, sino, stream);
 }
 
 int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,

@@ -33,9 +33,21 @@ namespace {
 // same hat weights).  Records are padded to RS = FSB + 4 floats for FSB 8
 // and 16 so that lanes on neighbouring pixels start on different 16-byte
 // bank groups.
-// staged columns per line (the window of 68 bins at |slope| >= 0.69 is
-// about 100 columns).  A 120-column tile (39 KB, four workgroups per CU,
-// VGPRs capped at 128 with spills) measured slower: 6.52 against 5.99 ms.
+// Walking rows, a chunk's window is contiguous in memory.  Walking columns
+// (x = line), it is four floats wide per image row; with a transposed copy
+// of the image (image_t) those blocks stage contiguous runs as well.
+// Without it, at 16 x 7192^2, their 16-byte pieces of 128-byte lines hit
+// the L2 27 % of the time, 4.9 TB came from beyond it per launch and the
+// kernel took 873 against 516 ms.
+// staged columns per line: the window of 68 bins at |slope| >= 0.69 is
+// about 100 columns, plus the spread of the four angles' windows, which
+// grows with the distance from the centre (up to about 27 columns at 3600
+// pixels and 0.1 degree steps: 3 % of the chunks of a 7192-wide image then
+// read global memory).  160 columns stage those too but measured slower
+// (548.6 against 516.1 ms at 16 x 7192^2, 5.59 against 5.41 ms at
+// 16 x 1024^2): the staging loop takes three steps instead of two.  A
+// 120-column tile (39 KB, four workgroups per CU, VGPRs capped at 128 with
+// spills) measured slower too: 6.52 against 5.99 ms at 1024 columns.
 constexpr int kMaxMinor = 128;
 constexpr int kMaxK = 8;
 // below this |slope| a line runs (almost) along the ray: take it whole
@@ -182,6 +194,51 @@ __device__ void gather_line_global(const FpArgs& a, int m, bool by_rows,
     }
 }
 
+// True when no pixel of lines m0..m1 and minor columns ka..kb is inside
+// both the image and the mask: with lines at distance >= dm from the mask
+// centre, a kept pixel has |k - mc| <= sqrt(r2 - dm^2).  Two columns of
+// slack cover runs that leave the staged range by a rounding step (those
+// are redone from global memory, where the mask decides again).
+__device__ __forceinline__ bool chunk_is_empty(const FpArgs& a, int m0,
+                                               int m1, int ka, int kb) {
+    const int C = a.ncol;
+    if (kb < -2 || ka > C + 1) return true;
+    if (!a.use_mask) return false;
+    const int mc = a.mask_c;
+    const int dm = mc < m0 ? m0 - mc : (mc > m1 ? mc - m1 : 0);
+    const double rest = a.mask_r2 - (double)dm * (double)dm;
+    if (rest < 0.0) return true;
+    const double h = sqrt(rest) + 2.0;
+    return (double)kb < (double)mc - h || (double)ka > (double)mc + h;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2.  The blocks that read the same image columns are the
+// consecutive angle blocks of one bin block, so the launch is 1-D and the
+// id is remapped: XCD x runs a contiguous range of the order (angle block
+// fastest, then bin block, then slice block).  Without it, a width of
+// 113 bin blocks (7192 columns) put neighbouring angle blocks on different
+// XCDs and every chunk window came from HBM.
+struct FpBlock {
+    int bin, ang, slc;  // bin block, angle block, slice block
+};
+
+template <int FSB>
+__device__ __forceinline__ FpBlock fp_block(const FpArgs& a) {
+    const int nab = (a.nrow + 3) / 4, nbb = (a.ncol + 63) / 64;
+    const int nsb = (a.nslice + FSB - 1) / FSB;
+    const int total = nab * nbb * nsb;
+    const int p = blockIdx.x, x = p & 7, slot = p >> 3;
+    const int per = total >> 3, rem = total & 7;
+    const int l = x * per + min(x, rem) + slot;
+    FpBlock b;
+    b.ang = l % nab;
+    const int rest = l / nab;
+    b.bin = rest % nbb;
+    b.slc = rest / nbb;
+    return b;
+}
+
 template <int FSB, int kRows>
 __global__ __launch_bounds__(256) void forward_project_hat_kernel(
     FpArgs a, int by_rows_block) {
@@ -191,19 +248,20 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
     __shared__ __align__(16) float tile[(kRows * kMaxMinor + kMaxK) * RS];
     __shared__ double s_cs[4][2];
     __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
-    __shared__ int s_ab[2];       // the chunk's staged column range
+    __shared__ int s_ab[3];       // the chunk's staged column range, skip
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
-    const int d0 = blockIdx.x * 64;
-    const int i = blockIdx.y * 4 + wv;
-    const int j0 = blockIdx.z * FSB;
+    const FpBlock blk = fp_block<FSB>(a);
+    const int d0 = blk.bin * 64;
+    const int i = blk.ang * 4 + wv;
+    const int j0 = blk.slc * FSB;
     const int ns = min(FSB, S - j0);
     const int d = d0 + lane;
     const double ic = a.icenter, cen = a.center;
     float* const zpad = tile + kRows * kMaxMinor * RS;
     for (int e = tid; e < kMaxK * RS; e += 256) zpad[e] = 0.0f;
     if (tid < 4) {
-        const int ia = min(blockIdx.y * 4 + tid, N - 1);
+        const int ia = min(blk.ang * 4 + tid, N - 1);
         const float th = -a.angles[ia];
         s_cs[tid][0] = (double)(float)cos((double)th);
         s_cs[tid][1] = (double)(float)sin((double)th);
@@ -220,6 +278,7 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
         const double sl = by_rows ? s_cs[tid][0] : s_cs[tid][1];
         s_inv[tid] = fabs(sl) >= kMinSlope ? 1.0 / sl : 0.0;
     }
+    const bool rowwise = by_rows || a.image_t != nullptr;
     // f = (k - ic) * slope + base(m) along a line m of the walking axis
     const double slope = by_rows ? c : s;
     const double other = by_rows ? s : c;
@@ -280,24 +339,35 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
             if (tid == 0) {
                 const bool whole = lo <= -1e299 || hi >= 1e299 ||
                                    hi - lo > 2.0 * kMaxMinor;
-                s_ab[0] = whole ? 0 : (int)floor(lo + ic) - 1;
-                s_ab[1] = whole ? C - 1 + kMaxMinor
-                                : (int)ceil(hi + ic) + 1;
+                const int ka = whole ? 0 : (int)floor(lo + ic) - 1;
+                const int kb = whole ? C - 1 + kMaxMinor
+                                     : (int)ceil(hi + ic) + 1;
+                s_ab[0] = ka;
+                s_ab[1] = kb;
+                s_ab[2] = chunk_is_empty(a, m0, m1, ka, kb) ? 1 : 0;
             }
         }
         __syncthreads();
         const int ka = s_ab[0], kb = s_ab[1];
+        // every pixel the chunk's runs (and their margins) can reach is
+        // outside the mask or the image: its terms are all w * 0, so the
+        // chunk is skipped (the flush schedule below is unchanged)
+        const bool empty = s_ab[2] != 0;
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
-        if (staged) {
+        if (staged && !empty) {
             // one pixel per thread and step: the mask test once, then its
-            // FSB slices (kMaxMinor = 128 = 2^7 columns per line)
+            // FSB slices.  The trip count is a constant, so the loads of
+            // successive steps are in flight together (a loop over exactly
+            // nrows x wid entries measured 6 % slower).  Lanes run along
+            // the staged columns when those are contiguous in memory: rows
+            // of the image, or columns of its transposed copy.
             const int nrows = m1 - m0 + 1;
             for (int e = tid; e < kRows * kMaxMinor; e += 256) {
                 int r, k;
-                if (by_rows) {  // lanes along x within a line
-                    r = e >> 7;
-                    k = e & (kMaxMinor - 1);
+                if (rowwise) {
+                    r = e / kMaxMinor;
+                    k = e % kMaxMinor;
                 } else {        // lanes along the lines (x) first
                     r = e % kRows;
                     k = e / kRows;
@@ -314,7 +384,10 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
                 }
                 float v[FSB];
                 if (keep) {
-                    const float* src = a.image + ((size_t)y * S + j0) * C + x;
+                    const float* src =
+                        by_rows || !a.image_t
+                            ? a.image + ((size_t)y * S + j0) * C + x
+                            : a.image_t + ((size_t)x * S + j0) * C + y;
 #pragma unroll
                     for (int js = 0; js < FSB; ++js)
                         v[js] = js < ns ? src[(size_t)js * C] : 0.0f;
@@ -335,7 +408,7 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
             }
         }
         __syncthreads();
-        if (valid) {
+        if (valid && !empty) {
             if (staged && fast_wave && wid >= K) {
                 const double u0 =
                     (dedge - (((double)m0 - ic) * other + cen)) * inv_slope +
@@ -377,6 +450,8 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
                     }
                 }
             }
+        }
+        if (valid) {
             // float32 sums over two chunks (at most 2 * kRows * K terms),
             // then into float64
             if (((m0 / kRows) & 1) || m1 == C - 1) {
@@ -398,7 +473,41 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
     }
 }
 
+// out(x, j, y) = in(y, j, x): 64 x 64 tiles through LDS (a padded row
+// keeps the transposed reads conflict-free), slices over grid.y.
+__global__ __launch_bounds__(256) void transpose_slices_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int S, int C,
+    int ntx) {
+    __shared__ float t[64][65];
+    const int x0 = (blockIdx.x % ntx) * 64, y0 = (blockIdx.x / ntx) * 64;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int j = blockIdx.y; j < S; j += gridDim.y) {
+        for (int r = ly; r < 64; r += 4) {
+            const int y = y0 + r, x = x0 + lx;
+            if (y < C && x < C) t[r][lx] = in[((size_t)y * S + j) * C + x];
+        }
+        __syncthreads();
+        for (int r = ly; r < 64; r += 4) {
+            const int x = x0 + r, y = y0 + lx;
+            if (x < C && y < C) out[((size_t)x * S + j) * C + y] = t[lx][r];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+int transpose_slices(const float* in, float* out, int nslice, int ncol,
+                     hipStream_t stream) {
+    const int ntx = (ncol + 63) / 64;
+    const long long ntile = (long long)ntx * ntx;
+    if (ntile > 0x7fffffffLL)
+        return set_error(ATH_ERR_UNSUPPORTED, "transpose: image too large");
+    const dim3 grid((unsigned)ntile, (unsigned)std::min(nslice, 65535));
+    hipLaunchKernelGGL(transpose_slices_kernel, grid, dim3(256), 0, stream,
+                       in, out, nslice, ncol, ntx);
+    return check_hip(hipGetLastError(), "transpose_slices_kernel launch");
+}
 
 // Angle blocks of 4 share one walking direction: rows when the block's
 // first angle has |cos| >= |sin|.  The candidate windows are exact for any
@@ -412,8 +521,14 @@ int forward_projection(const FpArgs& a, hipStream_t stream) {
     // (16 x 901 x 1024: 16/4 9.9 ms, 8/4 11.2 ms, 16/8 12.7 ms)
     const int fsb = a.nslice >= 16 ? 16
                   : a.nslice >= 8 ? 8 : (a.nslice >= 4 ? 4 : 1);
-    const dim3 grid((a.ncol + 63) / 64, (a.nrow + 3) / 4,
-                    (a.nslice + fsb - 1) / fsb);
+    // one 1-D grid in the order fp_block decodes
+    const long long nblk = (long long)((a.ncol + 63) / 64) *
+                           ((a.nrow + 3) / 4) *
+                           ((a.nslice + fsb - 1) / fsb);
+    if (nblk > 0x7fffffffLL)
+        return set_error(ATH_ERR_UNSUPPORTED,
+                         "forward projection: too many workgroups");
+    const dim3 grid((unsigned)nblk);
     if (fsb == 16)
         hipLaunchKernelGGL((forward_project_hat_kernel<16, 4>), grid,
                            dim3(256), 0, stream, a, -1);

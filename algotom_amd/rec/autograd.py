@@ -111,10 +111,15 @@ class FBPGeometry:
         ncol, nslice, _ = image.shape
         out = torch.empty((self.nrow, nslice, ncol), dtype=torch.float32,
                           device=self.dev)
-        nat.check(self.lib.ath_forward_projection(
+        # a transposed copy of the image lets every angle stage contiguous
+        # rows (same bits as ath_forward_projection)
+        nbytes = self.lib.ath_forward_projection_workspace_bytes(nslice, ncol)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        nat.check(self.lib.ath_forward_projection_ws(
             nat.ptr(image), self.nrow, nslice, ncol, nat.ptr(self.angles),
             float(np.float32(center)), self.mask_radius, self.sigma,
-            nat.ptr(out), nat.stream_ptr(self.dev)), "ath_forward_projection")
+            nat.ptr(work), nbytes, nat.ptr(out), nat.stream_ptr(self.dev)),
+            "ath_forward_projection_ws")
         return out
 
     def filter_adjoint(self, u):

@@ -23,7 +23,7 @@ treat them as ordinary functional ops on device tensors:
         the back-projection of ath_back_projection, (ncol, nslice, ncol).
     algotom_amd::forward_projection(image, angles, center, mask_radius,
                                     scale)
-        its adjoint (ath_forward_projection), (nrow, nslice, ncol).
+        its adjoint (ath_forward_projection_ws), (nrow, nslice, ncol).
 
 Angles are a float tensor (any device); the ops compute on the device of
 their first tensor argument.  Importing this module registers the ops.
@@ -87,10 +87,14 @@ def forward_projection(image: torch.Tensor, angles: torch.Tensor,
     img = image.contiguous().to(torch.float32)
     out = torch.empty((ang.numel(), nslice, ncol), dtype=torch.float32,
                       device=dev)
-    nat.check(nat.load().ath_forward_projection(
+    lib = nat.load()
+    nbytes = lib.ath_forward_projection_workspace_bytes(nslice, ncol)
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    nat.check(lib.ath_forward_projection_ws(
         nat.ptr(img), ang.numel(), nslice, ncol, nat.ptr(ang),
         float(np.float32(center)), float(mask_radius), float(scale),
-        nat.ptr(out), nat.stream_ptr(dev)), "ath_forward_projection")
+        nat.ptr(work), nbytes, nat.ptr(out), nat.stream_ptr(dev)),
+        "ath_forward_projection_ws")
     return out
 
 

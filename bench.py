@@ -352,9 +352,10 @@ def run_volume_refine(args, dev):
     499.5, seed 5000), converted on the GPU to 1801 x 7192 sinograms; loss =
     ||FBP(s; c) - phantom||^2 + 1e-3 TV(FBP(s; c)) over the 128 x 7192^2
     volume, Adam on the sinograms s and the centre c (started 1.5 px off).
-    Every step runs the volume in chunks of 16 slices (forward, then the
-    hand-written adjoint with the filtered sinogram checkpointed); the
-    centre's gradient sums over the chunks before the update."""
+    Every step runs the volume in chunks of 16 slices (forward, with the
+    centre derivative from the same back-projection pass kept for the
+    backward, then the hand-written adjoint); the centre's gradient sums
+    over the chunks before the update."""
     import torch
     from algotom_amd.prep import conversion
     from algotom_amd.rec import autograd
@@ -378,7 +379,7 @@ def run_volume_refine(args, dev):
     for j in range(nsl):
         target[:, j, :] = phantom.ellipse_image_torch(params[j], width, dev)
     geom = autograd.FBPGeometry(n180, width, angles, ratio=1.0,
-                                checkpoint="filtered")
+                                checkpoint="derivative")
     print("volume_refine: data ready (%d x %d x %d sinograms)"
           % (n180, nsl, width), file=sys.stderr, flush=True)
     s_param = stitched.contiguous().requires_grad_(True)
@@ -437,7 +438,7 @@ def run_volume_refine(args, dev):
         "data": "synthetic",
         "config": {"workload": "fbp_refine_volume_128x1801x7192",
                    "nslice": nsl, "nrow_180": n180, "width": width,
-                   "chunk": chunk, "checkpoint": "filtered",
+                   "chunk": chunk, "checkpoint": "derivative",
                    "parallelism": "single-gpu"},
         "detail": {"loss_first": first, "loss_last": float(loss.item()),
                    "rel_error_first_forward": rel[0],

@@ -238,13 +238,30 @@ int ath_back_projection_naive(const float* sino, int nrow, int nslice,
  * (ncol, nslice, ncol) float32; sino: (nrow, nslice, ncol) float32 output,
  * sino = scale * B^T (mask * image) with B the back-projection of
  * reconstruction.py:156-190 and mask as in ath_back_projection
- * (mask_radius < 0: none).  Deterministic gather, float64 sums.  nrow must
- * not exceed 65535.  No reference counterpart (the backward pass of
- * differentiable FBP). */
+ * (mask_radius < 0: none).  Deterministic gather, float64 sums.  No
+ * reference counterpart (the backward pass of differentiable FBP). */
 int ath_forward_projection(const float* image, int nrow, int nslice,
                            int ncol, const float* angles, float center,
                            double mask_radius, double scale, float* sino,
                            void* stream);
+
+/* Bytes of the workspace ath_forward_projection_ws needs: one float32 copy
+ * of the (ncol, nslice, ncol) image; 0 for an invalid geometry. */
+size_t ath_forward_projection_workspace_bytes(int nslice, int ncol);
+
+/* ath_forward_projection with a caller workspace of at least
+ * ath_forward_projection_workspace_bytes(nslice, ncol): the image is first
+ * transposed per slice into it, so the angles whose rays run closer to the
+ * y axis stage contiguous memory too.  Bit-identical to
+ * ath_forward_projection; faster where the image does not fit the L2 and
+ * last-level caches (1.26x at 16 x 7192^2).  ATH_ERR_WORKSPACE when the
+ * workspace is missing or too small.  No reference counterpart (as
+ * ath_forward_projection). */
+int ath_forward_projection_ws(const float* image, int nrow, int nslice,
+                              int ncol, const float* angles, float center,
+                              double mask_radius, double scale,
+                              void* workspace, size_t workspace_bytes,
+                              float* sino, void* stream);
 
 /* Adjoint of ath_ramp_filter for float32 rows: circular correlation with
  * the taps (plan built by ath_filter_plan_build from the REVERSED taps

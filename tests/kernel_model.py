@@ -161,6 +161,35 @@ def bp_tile_footprints(ncol, angles, center, py=8, tx=64):
     return worst, span
 
 
+# ---- forward_project_hat_kernel's staging ---------------------------------
+def fp_block(p, nrow, ncol, nslice, fsb):
+    """forward_project_hat_kernel's workgroup id -> (bin block, angle block,
+    slice block): every XCD (id % 8) gets a contiguous run of the order
+    with the angle block fastest."""
+    nab, nbb = (nrow + 3) // 4, (ncol + 63) // 64
+    nsb = (nslice + fsb - 1) // fsb
+    total = nab * nbb * nsb
+    x, slot = p & 7, p >> 3
+    per, rem = total >> 3, total & 7
+    l = x * per + min(x, rem) + slot
+    return (l // nab) % nbb, l % nab, (l // nab) // nbb
+
+
+def fp_chunk_is_empty(ncol, use_mask, mask_r2, mc, m0, m1, ka, kb):
+    """The kernel's chunk_is_empty: no pixel of lines m0..m1 and columns
+    ka..kb (plus slack) inside the image and the mask."""
+    if kb < -2 or ka > ncol + 1:
+        return True
+    if not use_mask:
+        return False
+    dm = m0 - mc if mc < m0 else (mc - m1 if mc > m1 else 0)
+    rest = mask_r2 - float(dm) * float(dm)
+    if rest < 0.0:
+        return True
+    h = math.sqrt(rest) + 2.0
+    return kb < mc - h or ka > mc + h
+
+
 # ---- summation order of metrics.hip (numpy's float32 mean) ---------------
 def np_pairwise_leaf(a, f=np.float32):
     """np_pairwise_leaf: <= 128 values, 8 partial sums of type f."""

@@ -47,6 +47,43 @@ def test_forward_projection_is_dense_transpose(arc):
         assert np.abs(got[:, j, :] - ref).max() <= 1e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nrow,nslice,ncol,ratio", [
+    (37, 1, 65, None), (90, 3, 120, 1.0), (181, 9, 200, 0.8),
+    (64, 16, 257, 1.0), (45, 20, 130, None)])
+def test_forward_projection_ws_matches_plain(nrow, nslice, ncol, ratio):
+    # the transposed-copy entry stages the same values into the same LDS
+    # records: bit-identical to the plain entry, for every slice batch (1,
+    # 4, 8, 16) and both walking directions
+    import torch
+    from algotom_amd import _native as nat
+    lib = nat.load()
+    dev = nat.device()
+    rng = np.random.default_rng(nrow * ncol)
+    angles = np.sort(rng.uniform(0.0, 2 * np.pi, nrow))
+    ang = torch.from_numpy(angles.astype(np.float32)).to(dev)
+    img = torch.from_numpy(rng.standard_normal((ncol, nslice, ncol))
+                           .astype(np.float32)).to(dev)
+    mr = -1.0 if ratio is None else float(ratio * (ncol // 2))
+    center = float(np.float32((ncol - 1) / 2 + 1.3))
+    plain = torch.empty((nrow, nslice, ncol), dtype=torch.float32,
+                        device=dev)
+    ws = torch.empty_like(plain)
+    nat.check(lib.ath_forward_projection(
+        nat.ptr(img), nrow, nslice, ncol, nat.ptr(ang), center, mr, 0.5,
+        nat.ptr(plain), nat.stream_ptr(dev)), "ath_forward_projection")
+    nbytes = lib.ath_forward_projection_workspace_bytes(nslice, ncol)
+    assert nbytes == ncol * nslice * ncol * 4
+    work = torch.full((nbytes,), 255, dtype=torch.uint8, device=dev)
+    nat.check(lib.ath_forward_projection_ws(
+        nat.ptr(img), nrow, nslice, ncol, nat.ptr(ang), center, mr, 0.5,
+        nat.ptr(work), nbytes, nat.ptr(ws), nat.stream_ptr(dev)),
+        "ath_forward_projection_ws")
+    assert torch.equal(plain, ws)
+    # the workspace holds the transposed image
+    t = work.view(torch.float32).view(ncol, nslice, ncol)
+    assert torch.equal(t, img.permute(2, 1, 0))
+
+
 @pytest.mark.parametrize("arc", ARCS)
 def test_forward_projection_adjoint_identity_with_mask(arc):
     import torch

@@ -62,3 +62,54 @@ def test_bp_staging_span_bound(ncol, nang, center):
     worst, span = km.bp_tile_footprints(ncol, ang,
                                         np.float64(np.float32(center)))
     assert worst <= span - 2
+
+
+@pytest.mark.parametrize("ncol,ratio", [(64, 1.0), (101, 0.8), (256, None),
+                                        (300, 0.0)])
+def test_fp_empty_chunks_hold_no_kept_pixel(ncol, ratio):
+    # a chunk the kernel skips may not hold a pixel inside the image and the
+    # mask, within one column of its staged window (the reach of a run that
+    # leaves the window by a rounding step)
+    rng = np.random.default_rng(ncol)
+    mc = ncol // 2
+    use_mask = ratio is not None
+    r2 = (ratio * mc) ** 2 if use_mask else 0.0
+    yy, xx = np.mgrid[:ncol, :ncol]
+    kept = ((xx - mc) ** 2 + (yy - mc) ** 2 <= r2) if use_mask else \
+        np.ones((ncol, ncol), bool)
+    skipped = 0
+    for _ in range(4000):
+        m0 = int(rng.integers(0, ncol))
+        m1 = min(m0 + int(rng.integers(1, 9)), ncol) - 1
+        ka = int(rng.integers(-80, ncol + 40))
+        kb = ka + int(rng.integers(2, 130))
+        if not km.fp_chunk_is_empty(ncol, use_mask, r2, mc, m0, m1, ka, kb):
+            continue
+        skipped += 1
+        lo, hi = max(ka - 1, 0), min(kb + 1, ncol - 1)
+        if lo <= hi:
+            assert not kept[m0:m1 + 1, lo:hi + 1].any()
+    assert skipped > 200
+
+
+@pytest.mark.parametrize("nrow,ncol,nslice,fsb", [(901, 1024, 16, 16),
+                                                  (1801, 7192, 16, 16),
+                                                  (5, 65, 3, 1),
+                                                  (181, 300, 9, 8),
+                                                  (7, 64, 1, 1)])
+def test_fp_block_remap_is_a_bijection(nrow, ncol, nslice, fsb):
+    # the 1-D launch covers every (bin, angle, slice) block exactly once,
+    # and the ids one XCD runs are consecutive angle blocks
+    nab, nbb = (nrow + 3) // 4, (ncol + 63) // 64
+    nsb = (nslice + fsb - 1) // fsb
+    total = nab * nbb * nsb
+    seen = set()
+    for p in range(total):
+        b = km.fp_block(p, nrow, ncol, nslice, fsb)
+        assert 0 <= b[0] < nbb and 0 <= b[1] < nab and 0 <= b[2] < nsb
+        seen.add(b)
+    assert len(seen) == total
+    if total >= 16:
+        b0 = km.fp_block(0, nrow, ncol, nslice, fsb)
+        b8 = km.fp_block(8, nrow, ncol, nslice, fsb)
+        assert b8[1] == (b0[1] + 1) % nab

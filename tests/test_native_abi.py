@@ -117,6 +117,19 @@ def test_invalid_filter_arguments():
     assert st == nat.ATH_ERR_INVALID_ARG  # nwin neither 1 nor nrow
 
 
+def test_forward_projection_ws_rejects_small_workspace():
+    # the size check comes before any device work
+    lib = nat.load()
+    dummy = ctypes.c_void_p(16)
+    need = lib.ath_forward_projection_workspace_bytes(3, 50)
+    assert need == 50 * 3 * 50 * 4
+    assert lib.ath_forward_projection_workspace_bytes(0, 50) == 0
+    for work, size in ((dummy, need - 1), (None, need)):
+        st = lib.ath_forward_projection_ws(dummy, 10, 3, 50, dummy, 24.5,
+                                           -1.0, 1.0, work, size, dummy, None)
+        assert st == nat.ATH_ERR_WORKSPACE
+
+
 def test_dropin_fails_loudly_without_device():
     import torch
     if torch.cuda.is_available():
