@@ -262,9 +262,9 @@ int ath_filter_fft_length(int ncol, int pad) {
     if (ncol < 1 || pad < 0)
         return set_error(ATH_ERR_INVALID_ARG, "ncol must be > 0, pad >= 0");
     const int L = fft_length(ncol, pad);
-    if (L < 0 || L > 16384)
+    if (L < 0 || L > kMaxFilterFft)
         return set_error(ATH_ERR_UNSUPPORTED,
-                         "ramp filter: padded width needs an FFT above 16384");
+                         "ramp filter: padded width needs an FFT above 32768");
     return L;
 }
 
@@ -280,9 +280,9 @@ int ath_filter_plan_build(const double* taps, int nwin, int width,
     if (!taps || !plan) return set_error(ATH_ERR_INVALID_ARG, "null pointer");
     if (nwin < 1 || width < 1)
         return set_error(ATH_ERR_INVALID_ARG, "nwin and width must be > 0");
-    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)))
+    if (fft_len < 64 || fft_len > kMaxFilterFft || (fft_len & (fft_len - 1)))
         return set_error(ATH_ERR_INVALID_ARG,
-                         "fft_len must be a power of two in [64, 16384]");
+                         "fft_len must be a power of two in [64, 32768]");
     if (plan_bytes_given < plan_bytes(fft_len, nwin))
         return set_error(ATH_ERR_WORKSPACE, "filter plan buffer too small");
     return plan_build(taps, nwin, width, fft_len, plan,
@@ -334,8 +334,8 @@ int ath_ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
     if (nwin != 1 && nwin != nrow)
         return set_error(ATH_ERR_INVALID_ARG, "nwin must be 1 or nrow");
     const long long width = (long long)ncol + 2LL * pad;
-    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)) ||
-        (long long)fft_len < 2 * width - pad)
+    if (fft_len < 64 || fft_len > kMaxFilterFft ||
+        (fft_len & (fft_len - 1)) || (long long)fft_len < 2 * width - pad)
         return set_error(ATH_ERR_INVALID_ARG,
                          "fft_len must be a power of two >= 2*(ncol+2*pad)-pad "
                          "(see ath_filter_fft_length)");
@@ -434,9 +434,9 @@ int ath_ramp_filter_batched(const ath_filter_job* jobs, int njobs,
     if (!valid_dtype(out_dtype) ||
         (in_dtype != ATH_F32 && in_dtype != ATH_F64))
         return set_error(ATH_ERR_INVALID_ARG, "bad dtype");
-    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)))
+    if (fft_len < 64 || fft_len > kMaxFilterFft || (fft_len & (fft_len - 1)))
         return set_error(ATH_ERR_INVALID_ARG,
-                         "fft_len must be a power of two in [64, 16384]");
+                         "fft_len must be a power of two in [64, 32768]");
     // every job must satisfy fft_len >= 2*(ncol+2*pad) - pad: checked by the
     // caller, who built each job's plan for this fft_len
     return ramp_filter_batched(jobs, njobs, total_rows, in_dtype, out_dtype,
@@ -540,11 +540,11 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
     if (nwin != 1 && nwin != nrow)
         return set_error(ATH_ERR_INVALID_ARG, "nwin must be 1 or nrow");
     const long long width = (long long)ncol + 2LL * pad;
-    if (fft_len < 64 || fft_len > 16384 || (fft_len & (fft_len - 1)) ||
-        (long long)fft_len < 2 * width)
+    if (fft_len < 64 || fft_len > kMaxFilterAdjointFft ||
+        (fft_len & (fft_len - 1)) || (long long)fft_len < 2 * width)
         return set_error(ATH_ERR_INVALID_ARG,
                          "adjoint filter: fft_len must be a power of two "
-                         ">= 2*(ncol+2*pad)");
+                         ">= 2*(ncol+2*pad), at most 16384");
     return ramp_filter_adjoint(in, nrow, nslice, ncol, pad, pad_mode, plan,
                                fft_len, nwin, out,
                                static_cast<hipStream_t>(stream));

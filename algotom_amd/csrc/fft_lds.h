@@ -110,9 +110,13 @@ __device__ __forceinline__ void dft8(double2 (&v)[8]) {
 // A radix-4 butterfly fetches one twiddle w and forms w^2, w^3 by complex
 // products (one table access instead of three).  With QT the twiddles come
 // from the LDS quarter table tq (stage_quarter) instead of global memory.
-template <int DIR, int MAXB, bool QT = false, int NT = kFftThreads>
+// TWS: the table's index stride (tw[n * TWS] = exp(-2 pi i n / L)), so a
+// transform can read the table of a multiple of its length.
+template <int DIR, int MAXB, bool QT = false, int NT = kFftThreads,
+          int TWS = 1>
 __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                         const double2* tq = nullptr) {
+    static_assert(TWS == 1 || !QT, "the quarter table has stride 1");
     const int tid = threadIdx.x;
     const int L = 2 * M;
     int ns = 1;
@@ -142,7 +146,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                     // so that the 16 lanes of a group, which write 8
                     // consecutive elements each, hit 16 distinct banks
                     const int rr = (j >> 1) & 7;
-                    double2 wr = tw[rr * (L >> 3)];  // exp(-2 pi i rr/8)
+                    double2 wr = tw[rr * (L >> 3) * TWS];  // W8^rr
                     if (DIR > 0) wr = conjg(wr);
                     double2 w = wr;
 #pragma unroll
@@ -152,7 +156,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                     }
                 } else if (k != 0) {
                     double2 w1 = QT ? tq[swz_tw(k * (tstep >> 1))]
-                                    : tw[k * tstep];
+                                    : tw[k * tstep * TWS];
                     if (DIR > 0) w1 = conjg(w1);
                     const double2 w2 = cmul(w1, w1);
                     const double2 w3 = cmul(w2, w1);
@@ -202,7 +206,7 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                 const int k = j & (ns - 1);
                 if (k != 0) {
                     double2 w1 = QT ? tq[swz_tw(k * (tstep >> 1))]
-                                    : tw[k * tstep];
+                                    : tw[k * tstep * TWS];
                     if (DIR > 0) w1 = conjg(w1);
                     const double2 w2 = cmul(w1, w1);
                     const double2 w3 = cmul(w1, w2);
@@ -244,7 +248,8 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                 v[b][0] = buf[j];
                 double2 x1 = buf[j + nb];
                 if (j != 0) {
-                    double2 w = QT ? twiddle_half(tq, j, M) : tw[2 * j];
+                    double2 w = QT ? twiddle_half(tq, j, M)
+                                   : tw[2 * j * TWS];
                     if (DIR > 0) w = conjg(w);
                     x1 = cmul(x1, w);
                 }

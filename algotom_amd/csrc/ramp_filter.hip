@@ -178,6 +178,52 @@ __device__ __forceinline__ double pad_value(const PadVals& pv, int mode,
     return (double)(Tin)((double)j * (left ? pv.lo : pv.hi));
 }
 
+// Sample k of the zero-extended padded row (0 beyond the padded width W):
+// the pad index map, the optional log, and numpy.pad's value modes.  ADJ
+// embeds the row with zeros instead (the transpose of the padding).
+template <typename Tin, bool ADJ>
+__device__ __forceinline__ double padded_sample(const Tin* __restrict__ row,
+                                                int k, int width, int ncol,
+                                                int pad, int pad_mode,
+                                                int apply_log, bool value_mode,
+                                                const PadVals& pv) {
+    if (k >= width) return 0.0;
+    const int s = ADJ ? pad_source(k - pad, ncol, ATH_PAD_CONSTANT)
+                      : pad_source(k - pad, ncol, pad_mode);
+    if (s >= 0) return load_sample<Tin>(row, s, apply_log);
+    if (value_mode) return pad_value<Tin>(pv, pad_mode, k, pad, ncol);
+    return 0.0;
+}
+
+// The kernel product in the packed domain for the index pair (k, km = (M -
+// k) mod M), 0 <= k <= M/2, of Z = FFT_M(p[2n] + i p[2n+1]): the real-FFT
+// split, the products with the kernel's spectrum G (M + 1 values) and the
+// merge for the inverse.  zk, zm: Z[k], Z[km] in, the inverse's input at k
+// and km out (zm untouched when k == km).  tw: the length-2M table.
+__device__ __forceinline__ void spectral_pair(double2& zk, double2& zm, int k,
+                                              int M,
+                                              const double2* __restrict__ tw,
+                                              const double2* __restrict__ G) {
+    const int km = (M - k) & (M - 1);
+    const double2 wk = tw[k];
+    double2 xk, xm;
+    split_real(zk, k == km ? zk : zm, wk, xk, xm);
+    const double2 yk = cmul(xk, G[k]);
+    const double2 ym = cmul(xm, G[M - k]);
+    // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
+    const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
+    const double2 ok = rot<1>(cmul(conjg(wk),
+                                   cscale(csub(yk, conjg(ym)), 0.5)));
+    zk = cadd(ek, ok);
+    if (k != km) {
+        // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
+        const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
+        const double2 om = rot<-1>(cmul(wk,
+                                        cscale(csub(ym, conjg(yk)), 0.5)));
+        zm = cadd(em, om);
+    }
+}
+
 // Filters one row in LDS buffer `buf` (M double2): pad, rfft, multiply by
 // the kernel half spectrum G, irfft, crop into dst.  All 256 threads.
 // ADJ: the transpose of the filter -- the row is embedded with zeros
@@ -203,48 +249,22 @@ __device__ __forceinline__ void filter_row(
                                          apply_log);
     }
     // packed, padded, zero-extended row: buf[n] = (p[2n], p[2n+1])
-    for (int n = tid; n < M; n += NT) {
-        double v[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int k = 2 * n + h;
-            v[h] = 0.0;
-            if (k < width) {
-                const int s = ADJ ? pad_source(k - pad, ncol, ATH_PAD_CONSTANT)
-                                  : pad_source(k - pad, ncol, pad_mode);
-                if (s >= 0)
-                    v[h] = load_sample<Tin>(src_row, s, apply_log);
-                else if (value_mode)
-                    v[h] = pad_value<Tin>(pv, pad_mode, k, pad, ncol);
-            }
-        }
-        buf[n] = make_double2(v[0], v[1]);
-    }
+    for (int n = tid; n < M; n += NT)
+        buf[n] = make_double2(
+            padded_sample<Tin, ADJ>(src_row, 2 * n, width, ncol, pad,
+                                    pad_mode, apply_log, value_mode, pv),
+            padded_sample<Tin, ADJ>(src_row, 2 * n + 1, width, ncol, pad,
+                                    pad_mode, apply_log, value_mode, pv));
     __syncthreads();
     lds_fft<-1, B, QT, NT>(buf, M, tw, tq);
 
     // rfft split, multiply by the kernel spectrum, merge for the inverse
     for (int k = tid; k <= M / 2; k += NT) {
         const int km = (M - k) & (M - 1);
-        const double2 wk = tw[k];
-        double2 xk, xm;
-        split_real(buf[k], buf[km], wk, xk, xm);
-        const double2 yk = cmul(xk, G[k]);
-        const double2 ym = cmul(xm, G[M - k]);
-        // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
-        const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
-        const double2 ok = rot<1>(cmul(conjg(wk),
-                                       cscale(csub(yk, conjg(ym)), 0.5)));
-        if (k == 0 || k == km) {
-            buf[k] = cadd(ek, ok);
-        } else {
-            // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
-            const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
-            const double2 om = rot<-1>(cmul(wk,
-                                            cscale(csub(ym, conjg(yk)), 0.5)));
-            buf[k] = cadd(ek, ok);
-            buf[km] = cadd(em, om);
-        }
+        double2 zk = buf[k], zm = buf[km];
+        spectral_pair(zk, zm, k, M, tw, G);
+        buf[k] = zk;
+        if (km != k) buf[km] = zm;
     }
     __syncthreads();
     lds_fft<1, B, QT, NT>(buf, M, tw, tq);
@@ -385,10 +405,266 @@ __global__ __launch_bounds__(kThreads) void kernel_spectrum_kernel(
     }
 }
 
+// ---- FFT length L = 32768: M = 16384 = 2H packed points --------------------
+// One workgroup of kBigRowNT threads per row, H = 8192 complex float64 in
+// LDS (128 KB) and no global scratch.  Forward, decimation in time: E =
+// FFT_H(z[2j]) moves into registers (thread t keeps E[k] and E[H-k] for its
+// k = t + NT b <= H/2) while O = FFT_H(z[2j+1]) runs in LDS; then Z[k] =
+// E[k] + W^k O[k] and Z[k+H] = E[k] - W^k O[k] (W = exp(-2 pi i/M)) at the
+// four indices k, H-k, H+k, 2H-k, which are the two pairs (k, M-k) and
+// (H-k, H+k) of the kernel product.  Inverse, decimation in frequency: U[k]
+// = Y[k] + Y[k+H] stays in registers and V[k] = (Y[k] - Y[k+H]) W^-k goes to
+// LDS; IFFT_H(V) gives the odd outputs y[2j+1] (stored at once), IFFT_H(U)
+// the even ones.  The H-point transforms read the L-point table at stride 2.
+constexpr int kBigRowNT = 512;
+constexpr int kBigRowKB = kBigH / 2 / kBigRowNT;  // k slots per thread (8)
+constexpr int kBigRowMaxb = 4;   // lds_fft butterflies per thread
+// the row's LDS plus room for the median's histogram past ncol keys
+constexpr size_t kBigRowLds = (size_t)kBigH * sizeof(double2) + 256;
+
+// Registers of one thread across the two halves: for its k = t + NT b
+// (0 <= k < H/2) the values at k and H-k, and thread 0's value at H/2.
+struct BigRegs {
+    double2 v[kBigRowKB][2];
+    double2 mid;
+};
+
+// E = FFT_H of the packed samples val(4j), val(4j+1) into r; then O =
+// FFT_H of val(4j+2), val(4j+3) left in buf.  All threads; ends
+// synchronised.
+template <typename Val>
+__device__ __forceinline__ void big_forward_halves(
+    double2* buf, const double2* __restrict__ tw, Val val, BigRegs& r) {
+    constexpr int H = kBigH, NT = kBigRowNT;
+    const int tid = threadIdx.x;
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1) {
+#pragma unroll
+            for (int b = 0; b < kBigRowKB; ++b) {
+                const int k = tid + NT * b;
+                r.v[b][0] = buf[k];
+                r.v[b][1] = buf[(H - k) & (H - 1)];
+            }
+            if (tid == 0) r.mid = buf[H / 2];
+            __syncthreads();
+        }
+        for (int j = tid; j < H; j += NT)
+            buf[j] = make_double2(val(4 * j + 2 * h), val(4 * j + 2 * h + 1));
+        __syncthreads();
+        lds_fft<-1, kBigRowMaxb, false, NT, 2>(buf, H, tw);
+    }
+}
+
+// Z[k] and Z[k + H] from E[k] (register) and O[k] (buf)
+__device__ __forceinline__ void big_z(const double2* buf,
+                                      const double2* __restrict__ tw, int k,
+                                      double2 ek, double2& zlo, double2& zhi) {
+    const double2 o = cmul(buf[k], tw[2 * k]);
+    zlo = cadd(ek, o);
+    zhi = csub(ek, o);
+}
+
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void filter_row_big(
+    double2* buf, const Tin* __restrict__ src_row,
+                               Tout* __restrict__ dst, int ncol, int pad,
+                               int pad_mode, int apply_log,
+                               const double2* __restrict__ tw,
+                               const double2* __restrict__ G) {
+    constexpr int H = kBigH, M = 2 * H, NT = kBigRowNT;
+    const int tid = threadIdx.x;
+    const int width = ncol + 2 * pad;
+    PadVals pv{0.0, 0.0};
+    const bool value_mode = pad_mode >= ATH_PAD_LINEAR_RAMP && pad > 0;
+    if (value_mode)
+        pv = row_pad_values<NT, Tin>(buf, src_row, ncol, pad, pad_mode,
+                                     apply_log);
+    auto val = [&](int k) {
+        return padded_sample<Tin, false>(src_row, k, width, ncol, pad,
+                                         pad_mode, apply_log, value_mode, pv);
+    };
+    BigRegs r;
+    big_forward_halves(buf, tw, val, r);
+    // kernel product: U into r, V into buf (every index has one owner)
+#pragma unroll
+    for (int b = 0; b < kBigRowKB; ++b) {
+        const int k = tid + NT * b;
+        double2 z0, z1;  // Z[k], Z[H+k]
+        big_z(buf, tw, k, r.v[b][0], z0, z1);
+        if (k == 0) {
+            // the self pairs Z[0] and Z[H]
+            spectral_pair(z0, z0, 0, M, tw, G);
+            spectral_pair(z1, z1, H, M, tw, G);
+            r.v[b][0] = cadd(z0, z1);
+            buf[0] = csub(z0, z1);
+        } else {
+            double2 z2, z3;  // Z[H-k], Z[2H-k]
+            big_z(buf, tw, H - k, r.v[b][1], z2, z3);
+            spectral_pair(z0, z3, k, M, tw, G);      // (k, M-k)
+            spectral_pair(z2, z1, H - k, M, tw, G);  // (H-k, H+k)
+            r.v[b][0] = cadd(z0, z1);
+            r.v[b][1] = cadd(z2, z3);
+            buf[k] = cmul(csub(z0, z1), conjg(tw[2 * k]));
+            buf[H - k] = cmul(csub(z2, z3), conjg(tw[2 * (H - k)]));
+        }
+    }
+    if (tid == 0) {  // the one pair (H/2, 3H/2)
+        double2 z0, z1;
+        big_z(buf, tw, H / 2, r.mid, z0, z1);
+        spectral_pair(z0, z1, H / 2, M, tw, G);
+        r.mid = cadd(z0, z1);
+        buf[H / 2] = cmul(csub(z0, z1), conjg(tw[H]));
+    }
+    __syncthreads();
+    // IFFT_H(V) gives the odd outputs y[2j+1], then IFFT_H(U) the even
+    // ones.  Output c is padded sample q = W + pad + c: component q & 1 of
+    // y[q >> 1], at buf[q >> 2] of the transform of parity (q >> 1) & 1.
+    const int off = width + pad;
+    const double* y = reinterpret_cast<const double*>(buf);
+    for (int h = 1; h >= 0; --h) {
+        if (h == 0) {
+#pragma unroll
+            for (int b = 0; b < kBigRowKB; ++b) {
+                const int k = tid + NT * b;
+                buf[k] = r.v[b][0];
+                if (k != 0) buf[H - k] = r.v[b][1];
+            }
+            if (tid == 0) buf[H / 2] = r.mid;
+            __syncthreads();
+        }
+        lds_fft<1, kBigRowMaxb, false, NT, 2>(buf, H, tw);
+        for (int c = tid; c < ncol; c += NT) {
+            const int q = off + c;
+            if (((q >> 1) & 1) == h)
+                store_elem(dst + c, y[2 * (q >> 2) + (q & 1)]);
+        }
+        __syncthreads();  // buf free for U, then for the next row
+    }
+}
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kBigRowNT) void ramp_filter_big_kernel(
+    const Tin* __restrict__ sino, Tout* __restrict__ out, size_t rows,
+    int nslice, int ncol, int pad, int pad_mode, int apply_log,
+    const double2* __restrict__ tw, const double2* __restrict__ spec,
+    int nwin) {
+    extern __shared__ double2 buf[];
+    constexpr int M = 2 * kBigH;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const double2* G =
+            spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+        filter_row_big<Tin, Tout>(buf, sino + row * (size_t)ncol,
+                                  out + row * (size_t)ncol, ncol, pad,
+                                  pad_mode, apply_log, tw, G);
+    }
+}
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kBigRowNT) void ramp_filter_batched_big_kernel(
+    const ath_filter_job* __restrict__ jobs, int njobs, long long total_rows,
+    int pad_mode, int apply_log) {
+    extern __shared__ double2 buf[];
+    constexpr int M = 2 * kBigH;
+    for (long long grow = blockIdx.x; grow < total_rows; grow += gridDim.x) {
+        int lo = 0, hi = njobs - 1;  // last job with row_begin <= grow
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].row_begin <= grow) lo = mid;
+            else hi = mid - 1;
+        }
+        const ath_filter_job jb = jobs[lo];
+        const size_t row = (size_t)(grow - jb.row_begin);
+        const double2* tw = static_cast<const double2*>(jb.plan);
+        const double2* G =
+            tw + 2 * M +
+            (jb.nwin > 1 ? (row / jb.nslice) : 0) * (size_t)(M + 1);
+        filter_row_big<Tin, Tout>(
+            buf, static_cast<const Tin*>(jb.sino) + row * (size_t)jb.ncol,
+            static_cast<Tout*>(jb.out) + row * (size_t)jb.ncol, jb.ncol,
+            jb.pad, pad_mode, apply_log, tw, G);
+    }
+}
+
+// spec[r][k] = rfft_L([g_r, g_r, 0...])[k] / M at L = 32768, one block per
+// window row: the forward halves as in filter_row_big, then the real split
+// of the pairs (k, M-k) and (H-k, H+k).
+__global__ __launch_bounds__(kBigRowNT) void kernel_spectrum_big_kernel(
+    const double* __restrict__ taps, int width,
+    const double2* __restrict__ tw, double2* __restrict__ spec) {
+    extern __shared__ double2 buf[];
+    constexpr int H = kBigH, M = 2 * H, NT = kBigRowNT;
+    const int tid = threadIdx.x;
+    const double* g = taps + (size_t)blockIdx.x * width;
+    double2* G = spec + (size_t)blockIdx.x * (M + 1);
+    auto val = [&](int k) {
+        return k < 2 * width ? g[k < width ? k : k - width] : 0.0;
+    };
+    BigRegs r;
+    big_forward_halves(buf, tw, val, r);
+    const double inv = 1.0 / (double)M;
+    auto put = [&](int k, double2 zk, double2 zm) {
+        double2 xk, xm;
+        split_real(zk, zm, tw[k], xk, xm);
+        G[k] = cscale(xk, inv);
+        G[M - k] = cscale(xm, inv);
+    };
+#pragma unroll
+    for (int b = 0; b < kBigRowKB; ++b) {
+        const int k = tid + NT * b;
+        double2 z0, z1;
+        big_z(buf, tw, k, r.v[b][0], z0, z1);
+        if (k == 0) {
+            put(0, z0, z0);
+            put(H, z1, z1);
+        } else {
+            double2 z2, z3;
+            big_z(buf, tw, H - k, r.v[b][1], z2, z3);
+            put(k, z0, z3);
+            put(H - k, z2, z1);
+        }
+    }
+    if (tid == 0) {
+        double2 z0, z1;
+        big_z(buf, tw, H / 2, r.mid, z0, z1);
+        put(H / 2, z0, z1);
+    }
+}
+
+// Size class of the row kernels for a packed length M: maxb_for(M) up to
+// M = 8192, kBigClass for the two-half path at M = 16384, 0 above.
+constexpr int kBigClass = 16;
+inline int filter_class(int M) {
+    if (M == 2 * kBigH) return kBigClass;
+    return maxb_for(M);
+}
+
+template <typename Tin, typename Tout>
+int launch_filter_big(const void* sino, void* out, int nrow, int nslice,
+                      int ncol, int pad, int pad_mode, int apply_log,
+                      const void* plan, int nwin, hipStream_t stream) {
+    const double2* tw = static_cast<const double2*>(plan);
+    const double2* spec = tw + 4 * kBigH;
+    auto kern = ramp_filter_big_kernel<Tin, Tout>;
+    prepare_lds(kern, kBigRowLds);
+    const size_t rows = (size_t)nrow * nslice;
+    const size_t grid = std::min(
+        rows, (size_t)resident_blocks(kern, kBigRowLds, kBigRowNT));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kBigRowNT),
+                       kBigRowLds, stream, static_cast<const Tin*>(sino),
+                       static_cast<Tout*>(out), rows, nslice, ncol, pad,
+                       pad_mode, apply_log, tw, spec, nwin);
+    return check_hip(hipGetLastError(), "ramp_filter_big_kernel launch");
+}
+
 template <int MAXB, typename Tin, typename Tout>
 int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
                   int pad, int pad_mode, int apply_log, const void* plan,
                   int fft_len, int nwin, hipStream_t stream) {
+  if constexpr (MAXB == kBigClass) {
+    return launch_filter_big<Tin, Tout>(sino, out, nrow, nslice, ncol, pad,
+                                        pad_mode, apply_log, plan, nwin,
+                                        stream);
+  } else {
     const int M = fft_len / 2;
     const double2* tw = static_cast<const double2*>(plan);
     const double2* spec = tw + fft_len;
@@ -405,6 +681,7 @@ int launch_filter(const void* sino, void* out, int nrow, int nslice, int ncol,
                        rows, nslice, ncol, pad, pad_mode, apply_log, tw, spec,
                        nwin, M);
     return check_hip(hipGetLastError(), "ramp_filter_kernel launch");
+  }
 }
 
 template <int MAXB, typename Tin>
@@ -436,6 +713,17 @@ template <int MAXB, typename Tin, typename Tout>
 int launch_filter_batched(const ath_filter_job* jobs, int njobs,
                           long long total_rows, int fft_len, int pad_mode,
                           int apply_log, hipStream_t stream) {
+  if constexpr (MAXB == kBigClass) {
+    auto kern = ramp_filter_batched_big_kernel<Tin, Tout>;
+    prepare_lds(kern, kBigRowLds);
+    const long long grid = std::min<long long>(
+        total_rows, resident_blocks(kern, kBigRowLds, kBigRowNT));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kBigRowNT),
+                       kBigRowLds, stream, jobs, njobs, total_rows, pad_mode,
+                       apply_log);
+    return check_hip(hipGetLastError(),
+                     "ramp_filter_batched_big_kernel launch");
+  } else {
     const int M = fft_len / 2;
     const size_t lds = row_lds_bytes<MAXB>(M);
     auto kern = ramp_filter_batched_kernel<MAXB, Tin, Tout>;
@@ -447,6 +735,7 @@ int launch_filter_batched(const ath_filter_job* jobs, int njobs,
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, stream,
                        jobs, njobs, total_rows, pad_mode, apply_log, M);
     return check_hip(hipGetLastError(), "ramp_filter_batched_kernel launch");
+  }
 }
 
 template <int MAXB, typename Tin>
@@ -537,21 +826,29 @@ int plan_build(const double* taps, int nwin, int width, int fft_len,
                        0, stream, tw, fft_len);
     int st = check_hip(hipGetLastError(), "twiddle_kernel launch");
     if (st != ATH_OK) return st;
-    switch (maxb_for(M)) {
+    switch (filter_class(M)) {
     case 1: return launch_spectrum<1>(taps, nwin, width, tw, spec, M, stream);
     case 2: return launch_spectrum<2>(taps, nwin, width, tw, spec, M, stream);
     case 4: return launch_spectrum<4>(taps, nwin, width, tw, spec, M, stream);
     case 8: return launch_spectrum<8>(taps, nwin, width, tw, spec, M, stream);
+    case kBigClass: {
+        auto kern = kernel_spectrum_big_kernel;
+        prepare_lds(kern, kBigRowLds);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nwin), dim3(kBigRowNT),
+                           kBigRowLds, stream, taps, width, tw, spec);
+        return check_hip(hipGetLastError(),
+                         "kernel_spectrum_big_kernel launch");
+    }
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "ramp filter: FFT length above 16384 is not supported");
+                     "ramp filter: FFT length above 32768 is not supported");
 }
 
 int ramp_filter_batched(const ath_filter_job* jobs, int njobs,
                         long long total_rows, int in_dtype, int out_dtype,
                         int fft_len, int pad_mode, int apply_log,
                         hipStream_t s) {
-    switch (maxb_for(fft_len / 2)) {
+    switch (filter_class(fft_len / 2)) {
     case 1: return batched_in<1>(in_dtype, out_dtype, jobs, njobs, total_rows,
                                  fft_len, pad_mode, apply_log, s);
     case 2: return batched_in<2>(in_dtype, out_dtype, jobs, njobs, total_rows,
@@ -560,9 +857,13 @@ int ramp_filter_batched(const ath_filter_job* jobs, int njobs,
                                  fft_len, pad_mode, apply_log, s);
     case 8: return batched_in<8>(in_dtype, out_dtype, jobs, njobs, total_rows,
                                  fft_len, pad_mode, apply_log, s);
+    case kBigClass:
+        return batched_in<kBigClass>(in_dtype, out_dtype, jobs, njobs,
+                                     total_rows, fft_len, pad_mode, apply_log,
+                                     s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "ramp filter: FFT length above 16384 is not supported");
+                     "ramp filter: FFT length above 32768 is not supported");
 }
 
 namespace {
@@ -602,7 +903,8 @@ int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                                      plan, fft_len, nwin, out, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "ramp filter: FFT length above 16384 is not supported");
+                     "ramp filter adjoint: FFT length above 16384 is not "
+                     "supported");
 }
 
 int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
@@ -610,7 +912,11 @@ int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
                 const void* plan, int fft_len, int nwin, void* out,
                 int out_dtype, hipStream_t s) {
     const int M = fft_len / 2;
-    switch (maxb_for(M)) {
+    switch (filter_class(M)) {
+    case kBigClass:
+        return dispatch_in<kBigClass>(in_dtype, out_dtype, sino, out, nrow,
+                                      nslice, ncol, pad, pad_mode, apply_log,
+                                      plan, fft_len, nwin, s);
     case 1: return dispatch_in<1>(in_dtype, out_dtype, sino, out, nrow, nslice,
                                   ncol, pad, pad_mode, apply_log, plan,
                                   fft_len, nwin, s);
@@ -625,7 +931,7 @@ int ramp_filter(const void* sino, int in_dtype, int nrow, int nslice,
                                   fft_len, nwin, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "ramp filter: FFT length above 16384 is not supported");
+                     "ramp filter: FFT length above 32768 is not supported");
 }
 
 }  // namespace ath

@@ -117,7 +117,10 @@ int ath_device_count(int* count);
 /* FFT length used by the ramp filter for ncol columns and `pad` padding:
  * the smallest power of two L >= max(2*(ncol+2*pad) - pad, 64).  The filter
  * evaluates the reference's length-W circular convolution exactly as a
- * linear convolution with the doubled kernel [g, g] on this length. */
+ * linear convolution with the doubled kernel [g, g] on this length.  The
+ * filter runs up to L = 32768 (ncol <= 16159 at pad 150);
+ * ATH_ERR_UNSUPPORTED above.  ath_ramp_filter_adjoint and the circular
+ * (BPF) filter stop at 16384. */
 int ath_filter_fft_length(int ncol, int pad);
 
 /* Device bytes of a filter plan holding `nwin` kernel spectra. */

@@ -41,6 +41,23 @@ def test_batch_matches_single_calls(precision):
         assert np.array_equal(g, ref), (s.shape, c)
 
 
+def test_batch_with_fft_32768_rows():
+    # a width past the 16384-point filter joins the batch: its rows take the
+    # batched two-half kernel and match the single call bit for bit
+    from algotom_amd.rec import batch, reconstruction as rec
+    rng = np.random.default_rng(9)
+    shapes = [(31, 8100), (45, 300), (29, 9001)]
+    sinos = [rng.uniform(0.05, 1.0, s).astype(np.float32) for s in shapes]
+    centers = [4049.7, 151.2, 4500.0]
+    angles = [np.linspace(0.0, np.pi, s[0], endpoint=False) for s in shapes]
+    plan = batch.BatchFBP(shapes, centers, angles, apply_log=True)
+    plan.load(sinos)
+    plan.run()
+    for s, c, a, g in zip(sinos, centers, angles, plan.results()):
+        ref = rec.fbp_reconstruction(s.copy(), c, a, apply_log=True)
+        assert np.array_equal(g, ref), s.shape
+
+
 @pytest.mark.parametrize("gpu", [True, False])
 def test_batch_stacks_and_2d_mixed(gpu):
     # each member gets fbp_reconstruction's own arithmetic: float64 sums for

@@ -170,6 +170,29 @@ def test_stack_with_log_and_noise():
     torch.cuda.synchronize()
 
 
+def test_fbp_width_past_16384_point_filter():
+    """A phantom 8100 columns wide (padded row 8400: FFT length 32768, the
+    two-half filter path) through fbp_reconstruction against the C port's
+    reference arithmetic; bar as for the stack slices, 1e-4 * max|ref|."""
+    from algotom_amd.rec import reconstruction as rec
+    from algotom_amd.util import phantom
+    from oracle import port
+    ncol, nrow, center = 8100, 121, 4049.63
+    angles = np.linspace(0.0, np.pi, nrow, endpoint=False)
+    prm = phantom.random_ellipses(ncol, 20, seed=2024)
+    p = phantom.ellipse_sinogram(prm, angles, ncol, np.float32(center),
+                                 device="cuda")
+    sino = phantom.transmission(p, seed=2024).cpu().numpy()
+    got = rec.fbp_reconstruction(sino.copy(), center, angles)
+    print("8100 wide: GPU done, oracle port", flush=True)
+    ref = port.fbp(sino, center, angles)
+    err = np.abs(got.astype(np.float64) - ref).max()
+    scale = np.abs(ref).max()
+    print("121x8100: max|d| rel %.3g, mismatching pixels %d of %d"
+          % (err / scale, int((got != ref).sum()), got.size))
+    assert err <= 1e-4 * scale
+
+
 def test_device_tensor_in_tensor_out():
     import torch
     from algotom_amd.rec import reconstruction as rec

@@ -4,7 +4,8 @@ Bar: float64 output within 1e-12 * max|ref| of the oracle's FP64 filter
 (oracle/fbp_np.py ramp_filter, itself pinned by brute-force convolution);
 float32 output within one float32 ulp of the rounded oracle.  Covers every
 in-kernel pad mode, all windows, explicit pads (0, 1, large), per-row
-windows, apply_log, float64 input and every FFT length 64..16384.
+windows, apply_log, float64 input and every FFT length 64..32768 (32768:
+the two-half path, widths 7968..16384 at the default pads).
 """
 
 import numpy as np
@@ -39,7 +40,8 @@ def _oracle(sino, pad, pad_mode, apply_log, ramp_rows):
                               pad_mode).transpose(1, 0, 2)
 
 
-@pytest.mark.parametrize("ncol", [5, 64, 65, 512, 1024, 2560, 4096, 7192])
+@pytest.mark.parametrize("ncol", [5, 64, 65, 512, 1024, 2560, 4096, 7192,
+                                  7967])
 def test_filter_f64_widths(ncol):
     import torch
     from oracle import fbp_np
@@ -179,3 +181,66 @@ def test_apply_ramp_filter_dropin():
                                  filter_name="hann", pad=10)
     ref2 = fbp_np.ramp_filter(sino, None, "hann", 10, "edge")
     assert np.abs(got2 - ref2).max() <= 1e-12 * np.abs(ref2).max()
+
+
+# FFT length 32768 (the two-half path): the first width past 16384 at the
+# default pad, the widest at pad 150 and at pad 0, and every index-map mode
+@pytest.mark.parametrize("ncol,pad,mode", [
+    (7968, 150, "edge"), (8191, 150, "reflect"), (9000, 37, "constant"),
+    (12000, 150, "wrap"), (16159, 150, "symmetric"), (16384, 0, "edge")])
+def test_filter_fft_32768(ncol, pad, mode):
+    import torch
+    import kernel_model as km
+    from oracle import fbp_np
+    from algotom_amd.rec import reconstruction as rec
+    assert km.fft_length(ncol, pad) == 32768
+    sino = _rows(4, 2, ncol, ncol + pad)
+    win = fbp_np.ramp_window_row(ncol + 2 * pad, "hann")
+    print("ncol", ncol, "pad", pad, mode, flush=True)
+    got = _gpu_filter(sino, pad, mode, True, rec._taps(win)[None, :],
+                      torch.float64)
+    ref = _oracle(sino, pad, mode, True, win)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("  rel err %.3g" % err)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("mode", ["linear_ramp", "maximum", "mean",
+                                  "median", "minimum"])
+def test_filter_fft_32768_value_modes(mode):
+    import torch
+    from oracle import fbp_np
+    from algotom_amd.rec import reconstruction as rec
+    ncol, pad = 8500, 150
+    sino = _rows(3, 2, ncol, 77)
+    sino[:, 1, :] = np.maximum(np.round(sino[:, 1, :] * 8.0) / 8.0, 0.125)
+    win = fbp_np.ramp_window_row(ncol + 2 * pad, "hann")
+    got = _gpu_filter(sino, pad, mode, True, rec._taps(win)[None, :],
+                      torch.float64)
+    ref = _oracle(sino, pad, mode, True, win)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("%s: rel err %.3g" % (mode, err))
+    assert err <= (1e-6 if mode == "mean" else 1e-12)
+
+
+def test_filter_fft_32768_per_row_windows_and_f32():
+    # per-row kernel spectra (the plan's row stride at M + 1 = 16385) and
+    # float32 output within one ulp of the rounded oracle
+    import torch
+    from oracle import fbp_np
+    from algotom_amd.rec import reconstruction as rec
+    ncol, pad, nrow = 8000, 150, 3
+    sino = _rows(nrow, 2, ncol, 8)
+    wins = np.stack([fbp_np.ramp_window_row(ncol + 2 * pad, n)
+                     for n in (None, "hann", "parzen")])
+    got = _gpu_filter(sino, pad, "edge", False, rec._taps(wins),
+                      torch.float64)
+    got32 = _gpu_filter(sino, pad, "edge", False, rec._taps(wins),
+                        torch.float32)
+    for i in range(nrow):
+        ref = _oracle(sino[i:i + 1], pad, "edge", False, wins[i])[0]
+        assert np.abs(got[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+        r32 = ref.astype(np.float32)
+        ulps = np.abs(got32[i].view(np.int32).astype(np.int64)
+                      - r32.view(np.int32).astype(np.int64))
+        assert ulps.max() <= 1

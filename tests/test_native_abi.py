@@ -68,7 +68,9 @@ def test_fft_length_rule():
                       (2560, 150), (4096, 150), (7192, 150)]:
         assert lib.ath_filter_fft_length(ncol, pad) == \
             kernel_model.fft_length(ncol, pad)
-    assert lib.ath_filter_fft_length(9000, 150) == nat.ATH_ERR_UNSUPPORTED
+    assert lib.ath_filter_fft_length(9000, 150) == 32768
+    assert lib.ath_filter_fft_length(16159, 150) == 32768
+    assert lib.ath_filter_fft_length(16160, 150) == nat.ATH_ERR_UNSUPPORTED
     assert lib.ath_filter_fft_length(0, 1) == nat.ATH_ERR_INVALID_ARG
 
 
