@@ -827,7 +827,7 @@ int ath_dfi_plan_build(const double* window, int width, void* plan,
     const size_t need = dfi_plan_bytes(width);
     if (need == 0)
         return set_error(ATH_ERR_UNSUPPORTED,
-                         "DFI: the padded width must be odd and at most 8191");
+                         "DFI: the padded width must be odd and at most 16383");
     if (!plan) return set_error(ATH_ERR_INVALID_ARG, "null plan pointer");
     if (plan_bytes < need)
         return set_error(ATH_ERR_WORKSPACE, "DFI plan buffer too small");
@@ -871,7 +871,7 @@ int ath_dfi_reconstruction(const void* sino, int in_dtype, int nrow_in,
     if (width != ncol1 + 2 * pad || dfi_plan_bytes(width) == 0)
         return set_error(ATH_ERR_INVALID_ARG,
                          "DFI: width must be the odd width + 2*pad and at "
-                         "most 8191");
+                         "most 16383");
     if (angle_mode == ATH_DFI_NO_ANGLES || angle_mode == ATH_DFI_ANGLES) {
         if (nrow != nrow_in)
             return set_error(ATH_ERR_INVALID_ARG, "nrow must equal nrow_in");

@@ -210,6 +210,69 @@ __global__ __launch_bounds__(kThreads) void dfi_shift_kernel(
     }
 }
 
+// The same shift for lines longer than kMaxLine (the line and its
+// coefficients no longer fit in LDS together).  Outputs go in segments of
+// kShiftSeg; a segment needs the coefficients of a contiguous knot range
+// [c0, c1] (clamped to the line) and those the line values within kK3 of
+// it (a half-sample fold near an end lands inside the staged range, which
+// then starts or ends at the line's end).  line(p) is read from global
+// memory by the same index rule as the LDS form.
+constexpr int kShiftSeg = 4096;
+
+template <typename Tin, typename Tr>
+__global__ __launch_bounds__(kThreads) void dfi_shift_seg_kernel(
+    const Tin* __restrict__ in, long long row_stride, int nrow, int ncol,
+    int ncol1, double xshift, const double* __restrict__ h3,
+    double* __restrict__ S) {
+    constexpr int kCoef = kShiftSeg + 4;
+    __shared__ double line[kCoef + 2 * kK3];
+    __shared__ double coef[kCoef];
+    __shared__ double hs[kK3 + 1];
+    const int L1 = ncol1 + 2 * kNearestPad;
+    const int tid = threadIdx.x;
+    if (tid <= kK3) hs[tid] = h3[tid];
+    auto knot = [&](int j) {  // first knot of output j, clamped
+        const double pos = ((double)j - xshift) + (double)kNearestPad;
+        return min(max((int)floor(pos) - 1, 0), L1 - 1);
+    };
+    const int nseg = (ncol1 + kShiftSeg - 1) / kShiftSeg;
+    for (int item = blockIdx.x; item < nrow * nseg; item += gridDim.x) {
+        const int row = item / nseg, j0 = (item % nseg) * kShiftSeg;
+        const int j1 = min(j0 + kShiftSeg, ncol1) - 1;
+        const Tin* src = in + (long long)row * row_stride;
+        const int c0 = knot(j0), c1 = min(knot(j1) + 3, L1 - 1);
+        const int b0 = max(c0 - kK3, 0), b1 = min(c1 + kK3, L1 - 1);
+        __syncthreads();  // previous item's LDS consumed (and hs staged)
+        for (int p = b0 + tid; p <= b1; p += kThreads) {
+            int q = min(max(p - kNearestPad, 0), ncol1 - 1);
+            q = min(q, ncol - 1);  // the odd-width edge column
+            line[p - b0] = (double)src[q];
+        }
+        __syncthreads();
+        for (int p = c0 + tid; p <= c1; p += kThreads) {
+            double c = hs[0] * line[p - b0];
+            for (int t = 1; t <= kK3; ++t)
+                c += hs[t] * (line[fold_half(p - t, L1) - b0] +
+                              line[fold_half(p + t, L1) - b0]);
+            coef[p - c0] = c;
+        }
+        __syncthreads();
+        double* dst = S + (size_t)row * ncol1;
+        for (int j = j0 + tid; j <= j1; j += kThreads) {
+            const double pos = ((double)j - xshift) + (double)kNearestPad;
+            const int k0 = (int)floor(pos) - 1;
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + q;
+                v += beta3(pos - (double)k) *
+                     coef[min(max(k, 0), L1 - 1) - c0];
+            }
+            dst[j] = round_to<Tr>(v);
+        }
+    }
+}
+
 // ---- 2. angle handling ----------------------------------------------------
 // src(r, j) = the (folded) sinogram row r < rows; v360 = [src[:rows - 1];
 // fliplr(src)] (L = 2 rows - 1 rows); out[i] = sum_q beta3(pos - k_q)
@@ -699,9 +762,10 @@ __global__ __launch_bounds__(kThreads) void dfi_transpose_kernel(
         }
 }
 
-// ---- 7. Bluestein length 16384 (padded widths 4097 .. 8191) ---------------
-// The convolution of length M = 2H runs as H-point LDS transforms with a
-// per-workgroup global scratch (conv_big in fft_lds.h); bhat carries 1/M.
+// ---- 7. Bluestein lengths 16384 and 32768 (padded widths 4097 .. 16383) ----
+// The convolution of length M = LV * H (LV = 2 or 4) runs as H-point LDS
+// transforms with a per-workgroup global scratch of M values (conv_big and
+// conv_big4 in fft_lds.h); bhat carries 1/M.
 // bhat = FFT_M(b) / M for M = 2H (one workgroup; b as in bhat_kernel)
 __global__ __launch_bounds__(kBigNT) void bhat_big_kernel(
     const double2* __restrict__ chirp, const double2* __restrict__ tw, int N,
@@ -731,17 +795,40 @@ __global__ __launch_bounds__(kBigNT) void bhat_big_kernel(
     }
 }
 
+// bhat = FFT_M(b) / M for M = 4H: the quarters' transforms through bhat
+// itself (F_0..F_2 in its first 3H values), then the radix-4 combination,
+// each thread owning the four outputs of its k.
+__global__ __launch_bounds__(kBigNT) void bhat_big4_kernel(
+    const double2* __restrict__ chirp, const double2* __restrict__ tw, int N,
+    double2* __restrict__ bhat) {
+    extern __shared__ double2 lds[];
+    constexpr int H = kBigH, NT = kBigNT, M = kBig4M;
+    auto b = [&](int i) -> double2 {
+        if (i < N) return conjg(chirp[i]);
+        if (i > M - N) return conjg(chirp[M - i]);
+        return make_double2(0.0, 0.0);
+    };
+    fft_big4_quarters(lds, bhat, M, tw, b);
+    const double inv = 1.0 / (double)M;
+    for (int k = threadIdx.x; k < H; k += NT) {
+        double2 x[4];
+        big4_combine(bhat[k], bhat[H + k], bhat[2 * H + k], lds[k], k, tw, x);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bhat[k + q * H] = cscale(x[q], inv);
+    }
+}
+
 // Row pairs: pad, -log, the two rows' spectra split and windowed into rows
 // r1 and r1 + 1 of Xn in natural order (the row prefilter follows in
 // dfi_rows_fir_kernel).
-template <typename Tr>
+template <int LV, typename Tr>
 __global__ __launch_bounds__(kBigNT) void dfi_rows_fwd_big_kernel(
     const double* __restrict__ S, int rows, int ncol1, int pad, int pad_mode,
     int apply_log, const void* __restrict__ plan, int N, size_t o_tw,
     size_t o_chirp, size_t o_bhat, size_t o_win, double2* __restrict__ Xn,
     double2* __restrict__ scratch_all, int* __restrict__ flag) {
     extern __shared__ double2 buf[];
-    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    double2* scratch = scratch_all + (size_t)blockIdx.x * LV * kBigH;
     const double2* tw = at<double2>(plan, o_tw);
     const double2* chirp = at<double2>(plan, o_chirp);
     const double2* bhat = at<double2>(plan, o_bhat);
@@ -767,13 +854,15 @@ __global__ __launch_bounds__(kBigNT) void dfi_rows_fwd_big_kernel(
             const double2 ch = chirp[n];
             return make_double2(ch.x * v1 - ch.y * v2, ch.x * v2 + ch.y * v1);
         };
-        conv_big(buf, scratch, N, tw, bhat, load);
+        conv_lv<LV>(buf, scratch, N, tw, bhat, load);
         double2* d1 = Xn + (size_t)r1 * N;
         double2* d2 = d1 + N;
         for (int k = tid; k <= m; k += kBigNT) {
             const int kn = k == 0 ? 0 : N - k;
-            const double2 zk = cmul(chirp[k], big_conv(buf, scratch, k));
-            const double2 zn = cmul(chirp[kn], big_conv(buf, scratch, kn));
+            const double2 zk =
+                cmul(chirp[k], conv_lv_out<LV>(buf, scratch, k));
+            const double2 zn =
+                cmul(chirp[kn], conv_lv_out<LV>(buf, scratch, kn));
             const double2 x1 =
                 make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
             const double2 x2 =
@@ -839,14 +928,15 @@ __global__ __launch_bounds__(kThreads) void dfi_rows_fir_kernel(
     }
 }
 
-// Stage A at M = 2H (as dfi_cols_inv_kernel).
+// Stage A at M = LV * H (as dfi_cols_inv_kernel).
+template <int LV>
 __global__ __launch_bounds__(kBigNT) void dfi_cols_inv_big_kernel(
     const double2* __restrict__ in, int nrows, int ncol, int pad,
     const void* __restrict__ plan, int N, size_t o_tw, size_t o_chirp,
     size_t o_bhat, double2* __restrict__ out,
     double2* __restrict__ scratch_all) {
     extern __shared__ double2 buf[];
-    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    double2* scratch = scratch_all + (size_t)blockIdx.x * LV * kBigH;
     const double2* tw = at<double2>(plan, o_tw);
     const double2* chirp = at<double2>(plan, o_chirp);
     const double2* bhat = at<double2>(plan, o_bhat);
@@ -857,25 +947,26 @@ __global__ __launch_bounds__(kBigNT) void dfi_cols_inv_big_kernel(
         auto load = [&](int n) -> double2 {
             return cmul(chirp[n], conjg(src[n]));
         };
-        conv_big(buf, scratch, N, tw, bhat, load);
+        conv_lv<LV>(buf, scratch, N, tw, bhat, load);
         for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
             const int k = crop_index(kc, pad, m, N);
             out[(size_t)row * ncol + kc] = cscale(
-                conjg(cmul(chirp[k], big_conv(buf, scratch, k))), invN);
+                conjg(cmul(chirp[k], conv_lv_out<LV>(buf, scratch, k))),
+                invN);
         }
         __syncthreads();
     }
 }
 
-// Stage B at M = 2H (as dfi_rows_out_kernel).
-template <typename To>
+// Stage B at M = LV * H (as dfi_rows_out_kernel).
+template <int LV, typename To>
 __global__ __launch_bounds__(kBigNT) void dfi_rows_out_big_kernel(
     const double2* __restrict__ in, int ncol, int pad,
     const void* __restrict__ plan, int N, size_t o_tw, size_t o_chirp,
     size_t o_bhat, To* __restrict__ out, long long out_row_stride,
     double mask_r2, double2* __restrict__ scratch_all) {
     extern __shared__ double2 buf[];
-    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    double2* scratch = scratch_all + (size_t)blockIdx.x * LV * kBigH;
     const double2* tw = at<double2>(plan, o_tw);
     const double2* chirp = at<double2>(plan, o_chirp);
     const double2* bhat = at<double2>(plan, o_bhat);
@@ -895,11 +986,11 @@ __global__ __launch_bounds__(kBigNT) void dfi_rows_out_big_kernel(
             if (two) q = n <= m ? a2[n] : conjg(a2[N - n]);
             return cmul(chirp[n], make_double2(p.x - q.y, -(p.y + q.x)));
         };
-        conv_big(buf, scratch, N, tw, bhat, load);
+        conv_lv<LV>(buf, scratch, N, tw, bhat, load);
         for (int kc = threadIdx.x; kc < ncol; kc += kBigNT) {
             const int k = crop_index(kc, pad, m, N);
             const double2 y =
-                conjg(cmul(chirp[k], big_conv(buf, scratch, k)));
+                conjg(cmul(chirp[k], conv_lv_out<LV>(buf, scratch, k)));
             double v1 = y.x * invN, v2 = y.y * invN;
             if (mask_r2 >= 0.0) {
                 const int dx = kc - mc, d1 = y1 - mc, d2 = y1 + 1 - mc;
@@ -998,16 +1089,61 @@ int rows_out(int maxb, const double2* in, int ncol, int pad, const void* plan,
     }
 }
 
+// The M = LV * H stages: one workgroup of kBigNT threads per CU, each with
+// its scratch slot.
+template <int LV, typename Tr>
+int launch_rows_fwd_big(const double* S, int nrow, int ncol1, int pad,
+                        int pad_mode, int apply_log, const void* plan,
+                        const DfiPlan& p, double2* X, double2* scr, int nscr,
+                        int* flag, hipStream_t s) {
+    const size_t lds = (size_t)kBigH * sizeof(double2);
+    auto k = dfi_rows_fwd_big_kernel<LV, Tr>;
+    prepare_lds(k, lds);
+    const int grid = std::min((nrow + 1) / 2, nscr);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBigNT), lds, s, S, nrow, ncol1,
+                       pad, pad_mode, apply_log, plan, p.N, p.tw, p.chirp,
+                       p.bhat, p.window, X, scr, flag);
+    return check_hip(hipGetLastError(), "dfi_rows_fwd_big_kernel launch");
+}
+
+template <int LV>
+int launch_cols_inv_big(const double2* in, int h, int ncol, int pad,
+                        const void* plan, const DfiPlan& p, double2* out,
+                        double2* scr, int nscr, hipStream_t s) {
+    const size_t lds = (size_t)kBigH * sizeof(double2);
+    auto k = dfi_cols_inv_big_kernel<LV>;
+    prepare_lds(k, lds);
+    hipLaunchKernelGGL(k, dim3(std::min(h, nscr)), dim3(kBigNT), lds, s, in,
+                       h, ncol, pad, plan, p.N, p.tw, p.chirp, p.bhat, out,
+                       scr);
+    return check_hip(hipGetLastError(), "dfi_cols_inv_big_kernel launch");
+}
+
+template <int LV, typename To>
+int launch_rows_out_big(const double2* in, int ncol, int pad,
+                        const void* plan, const DfiPlan& p, To* out,
+                        long long out_row_stride, double mask_r2,
+                        double2* scr, int nscr, hipStream_t s) {
+    const size_t lds = (size_t)kBigH * sizeof(double2);
+    auto k = dfi_rows_out_big_kernel<LV, To>;
+    prepare_lds(k, lds);
+    const int grid = std::min((ncol + 1) / 2, nscr);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBigNT), lds, s, in, ncol, pad,
+                       plan, p.N, p.tw, p.chirp, p.bhat, out, out_row_stride,
+                       mask_r2, scr);
+    return check_hip(hipGetLastError(), "dfi_rows_out_big_kernel launch");
+}
+
 // Workspace: S1 (shifted sinogram), S2 (angle-handled), and two ping-pong
 // regions: R1 holds the row coefficients (rows x N), then the half grid
 // ((m + 1) x N), then the transposed first pass (ncol x (m + 1)); R2 holds
 // the column coefficients (rows x N), then the first pass ((m + 1) x ncol).
 struct DfiWork {
     size_t s1, s2, r1, r2, scr, total;
-    int nscr;  // scratch slots (workgroups) for the length-16384 path
+    int nscr;  // scratch slots (workgroups) of M values for M > 8192
 };
 
-// One 2H-value scratch per compute unit (the big kernels' 128 KB of LDS
+// One M-value scratch per compute unit (the big kernels' 128 KB of LDS
 // allow one workgroup per CU).
 inline int device_cus() {
     int dev = 0, cus = 0;
@@ -1031,8 +1167,9 @@ DfiWork work_layout(int nrow_in, int ncol, int nrow, int N) {
     const size_t r2 = std::max((size_t)nrow * N, h * ncol);
     w.r1 = o; o = align256(o + r1 * sizeof(double2));
     w.r2 = o; o = align256(o + r2 * sizeof(double2));
-    w.nscr = bluestein_length(N) > kBigH ? device_cus() : 0;
-    w.scr = o; o = align256(o + (size_t)w.nscr * 2 * kBigH * sizeof(double2));
+    const int M = bluestein_length(N);
+    w.nscr = M > kBigH ? device_cus() : 0;
+    w.scr = o; o = align256(o + (size_t)w.nscr * M * sizeof(double2));
     w.total = o;
     return w;
 }
@@ -1041,6 +1178,15 @@ template <typename Tin, typename Tr>
 int shift_rows(const Tin* in, long long row_stride, int nrow, int ncol,
                int ncol1, double xshift, const double* h3, double* S,
                hipStream_t s) {
+    if (ncol1 + 2 * kNearestPad > kMaxLine) {
+        const int nseg = (ncol1 + kShiftSeg - 1) / kShiftSeg;
+        const int grid = (int)std::min<long long>((long long)nrow * nseg,
+                                                  8192);
+        hipLaunchKernelGGL((dfi_shift_seg_kernel<Tin, Tr>), dim3(grid),
+                           dim3(kThreads), 0, s, in, row_stride, nrow, ncol,
+                           ncol1, xshift, h3, S);
+        return check_hip(hipGetLastError(), "dfi_shift_seg_kernel launch");
+    }
     auto k = dfi_shift_kernel<Tin, Tr>;
     const size_t lds = 2 * (size_t)(ncol1 + 2 * kNearestPad) * sizeof(double);
     prepare_lds(k, 2 * (size_t)kMaxLine * sizeof(double));
@@ -1053,7 +1199,7 @@ int shift_rows(const Tin* in, long long row_stride, int nrow, int ncol,
 }  // namespace
 
 size_t dfi_plan_bytes(int width) {
-    if (width < 3 || width % 2 == 0 || bluestein_length(width) > 2 * kBigH)
+    if (width < 3 || width % 2 == 0 || bluestein_length(width) > kBig4M)
         return 0;
     return plan_layout(width).total;
 }
@@ -1075,8 +1221,9 @@ int dfi_plan_build(const double* window, int width, void* plan,
                        at<double>(plan, p.h3), at<double>(plan, p.h5));
     if (big) {
         const size_t lds_big = (size_t)kBigH * sizeof(double2);
-        prepare_lds(bhat_big_kernel, lds_big);
-        hipLaunchKernelGGL(bhat_big_kernel, dim3(1), dim3(kBigNT), lds_big, s,
+        auto kern = p.M == kBig4M ? bhat_big4_kernel : bhat_big_kernel;
+        prepare_lds(kern, lds_big);
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kBigNT), lds_big, s,
                            at<double2>(plan, p.chirp),
                            at<double2>(plan, p.tw), p.N,
                            at<double2>(plan, p.bhat));
@@ -1124,7 +1271,7 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
     double2* R2 = at<double2>(workspace, w.r2);
     double2* scr = at<double2>(workspace, w.scr);
     const bool big = p.M > kBigH;
-    const size_t lds_big = (size_t)kBigH * sizeof(double2);
+    const int lv = p.M / kBigH;  // 2 or 4 when big
     const double* h3 = at<double>(plan, p.h3);
     const double* h5 = at<double>(plan, p.h5);
     const size_t esz = f64 ? sizeof(double) : sizeof(float);
@@ -1156,22 +1303,20 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
         }
         if (big) {
             // spectra into R2 (natural order), then the row prefilter into R1
-            const int grid = std::min((nrow + 1) / 2, w.nscr);
-            if (f64) {
-                prepare_lds(dfi_rows_fwd_big_kernel<double>, lds_big);
-                hipLaunchKernelGGL(dfi_rows_fwd_big_kernel<double>,
-                                   dim3(grid), dim3(kBigNT), lds_big, s, S,
-                                   nrow, ncol1, pad, pad_mode, apply_log,
-                                   plan, p.N, p.tw, p.chirp, p.bhat, p.window,
-                                   R2, scr, flag);
-            } else {
-                prepare_lds(dfi_rows_fwd_big_kernel<float>, lds_big);
-                hipLaunchKernelGGL(dfi_rows_fwd_big_kernel<float>, dim3(grid),
-                                   dim3(kBigNT), lds_big, s, S, nrow, ncol1,
-                                   pad, pad_mode, apply_log, plan, p.N, p.tw,
-                                   p.chirp, p.bhat, p.window, R2, scr, flag);
-            }
-            st = check_hip(hipGetLastError(), "dfi_rows_fwd_big_kernel launch");
+            if (f64)
+                st = lv == 4 ? launch_rows_fwd_big<4, double>(
+                                   S, nrow, ncol1, pad, pad_mode, apply_log,
+                                   plan, p, R2, scr, w.nscr, flag, s)
+                             : launch_rows_fwd_big<2, double>(
+                                   S, nrow, ncol1, pad, pad_mode, apply_log,
+                                   plan, p, R2, scr, w.nscr, flag, s);
+            else
+                st = lv == 4 ? launch_rows_fwd_big<4, float>(
+                                   S, nrow, ncol1, pad, pad_mode, apply_log,
+                                   plan, p, R2, scr, w.nscr, flag, s)
+                             : launch_rows_fwd_big<2, float>(
+                                   S, nrow, ncol1, pad, pad_mode, apply_log,
+                                   plan, p, R2, scr, w.nscr, flag, s);
             if (st != ATH_OK) return st;
             const dim3 fgrid((p.N + kFirSpan - 1) / kFirSpan,
                              std::min(nrow, 65535));
@@ -1199,12 +1344,10 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
         st = check_hip(hipGetLastError(), "dfi_grid_kernel launch");
         if (st != ATH_OK) return st;
         if (big) {
-            prepare_lds(dfi_cols_inv_big_kernel, lds_big);
-            hipLaunchKernelGGL(dfi_cols_inv_big_kernel,
-                               dim3(std::min(h, w.nscr)), dim3(kBigNT),
-                               lds_big, s, R1, h, ncol, pad, plan, p.N, p.tw,
-                               p.chirp, p.bhat, R2, scr);
-            st = check_hip(hipGetLastError(), "dfi_cols_inv_big_kernel launch");
+            st = lv == 4 ? launch_cols_inv_big<4>(R1, h, ncol, pad, plan, p,
+                                                  R2, scr, w.nscr, s)
+                         : launch_cols_inv_big<2>(R1, h, ncol, pad, plan, p,
+                                                  R2, scr, w.nscr, s);
         } else {
             st = cols_inv(maxb, R1, h, ncol, pad, plan, p, R2, s);
         }
@@ -1221,23 +1364,23 @@ int dfi_reconstruction(const void* sino, int dtype, int nrow_in, int nslice,
                   (size_t)sl * (size_t)out_slice_stride *
                       (out_dtype == ATH_F64 ? sizeof(double) : sizeof(float));
         if (big) {
-            const int grid = std::min((ncol + 1) / 2, w.nscr);
             if (out_dtype == ATH_F64) {
-                prepare_lds(dfi_rows_out_big_kernel<double>, lds_big);
-                hipLaunchKernelGGL(dfi_rows_out_big_kernel<double>, dim3(grid),
-                                   dim3(kBigNT), lds_big, s, R1, ncol, pad,
-                                   plan, p.N, p.tw, p.chirp, p.bhat,
-                                   reinterpret_cast<double*>(o),
-                                   out_row_stride, mask_r2, scr);
+                double* od = reinterpret_cast<double*>(o);
+                st = lv == 4 ? launch_rows_out_big<4, double>(
+                                   R1, ncol, pad, plan, p, od, out_row_stride,
+                                   mask_r2, scr, w.nscr, s)
+                             : launch_rows_out_big<2, double>(
+                                   R1, ncol, pad, plan, p, od, out_row_stride,
+                                   mask_r2, scr, w.nscr, s);
             } else {
-                prepare_lds(dfi_rows_out_big_kernel<float>, lds_big);
-                hipLaunchKernelGGL(dfi_rows_out_big_kernel<float>, dim3(grid),
-                                   dim3(kBigNT), lds_big, s, R1, ncol, pad,
-                                   plan, p.N, p.tw, p.chirp, p.bhat,
-                                   reinterpret_cast<float*>(o),
-                                   out_row_stride, mask_r2, scr);
+                float* of = reinterpret_cast<float*>(o);
+                st = lv == 4 ? launch_rows_out_big<4, float>(
+                                   R1, ncol, pad, plan, p, of, out_row_stride,
+                                   mask_r2, scr, w.nscr, s)
+                             : launch_rows_out_big<2, float>(
+                                   R1, ncol, pad, plan, p, of, out_row_stride,
+                                   mask_r2, scr, w.nscr, s);
             }
-            st = check_hip(hipGetLastError(), "dfi_rows_out_big_kernel launch");
         } else {
             st = out_dtype == ATH_F64
                      ? rows_out<double>(maxb, R1, ncol, pad, plan, p,

@@ -325,6 +325,116 @@ __device__ __forceinline__ double2 big_conv(const double2* buf,
     return (n & 1) ? scratch[kBigH + (n >> 1)] : buf[n >> 1];
 }
 
+// Transforms of length 4H = 32768: the radix-4 form of conv_big.  Forward,
+// decimation in time: F_r = FFT_H(x[4j + r]) for r = 0..3, then
+// X[k + qH] = sum_r (-i)^(rq) W^(rk) F_r[k], W = exp(-2 pi i / 4H).  The
+// product Y = X S.  Inverse, decimation in frequency:
+// V_s[k] = W^(-sk) sum_q i^(sq) Y[k + qH] and y[4j + s] = IFFT_H(V_s)[j].
+// F_0..F_2, then V_0..V_2 and the outputs of phases 1..3 pass through a
+// per-workgroup global scratch of 4H values; phase 0 ends in buf.  tw is the
+// 4H-point table (the H-point transforms read it at stride 2).
+// kBigNT threads.
+constexpr int kBig4M = 4 * kBigH;
+
+// F_r = FFT_H(x[4j + r]) (LOAD(n) for n < N, zero beyond): F_0..F_2 into
+// scratch[rH ..], F_3 left in buf.
+template <typename Load>
+__device__ void fft_big4_quarters(double2* buf, double2* scratch, int N,
+                                  const double2* __restrict__ tw, Load load) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int r = 0; r < 4; ++r) {
+        for (int j = tid; j < H; j += NT) {
+            const int n = 4 * j + r;
+            buf[j] = n < N ? load(n) : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        lds_fft<-1, kBigB, false, NT, 2>(buf, H, tw);
+        if (r < 3) {
+            for (int k = tid; k < H; k += NT) scratch[r * H + k] = buf[k];
+            __syncthreads();
+        }
+    }
+}
+
+// the four outputs X[k + qH] from the quarters' values at k
+__device__ __forceinline__ void big4_combine(double2 f0, double2 f1,
+                                             double2 f2, double2 f3, int k,
+                                             const double2* __restrict__ tw,
+                                             double2 (&x)[4]) {
+    const double2 t0 = f0;
+    const double2 t1 = cmul(f1, tw[k]);
+    const double2 t2 = cmul(f2, tw[2 * k]);
+    const double2 t3 = cmul(f3, tw[3 * k]);
+    const double2 a = cadd(t0, t2), b = csub(t0, t2);
+    const double2 c = cadd(t1, t3), d = rot<-1>(csub(t1, t3));  // -i(t1-t3)
+    x[0] = cadd(a, c);
+    x[1] = cadd(b, d);
+    x[2] = csub(a, c);
+    x[3] = csub(b, d);
+}
+
+template <typename Load>
+__device__ void conv_big4(double2* buf, double2* scratch, int N,
+                          const double2* __restrict__ tw,
+                          const double2* __restrict__ S, Load load) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    fft_big4_quarters(buf, scratch, N, tw, load);
+    for (int k = tid; k < H; k += NT) {
+        double2 y[4];
+        big4_combine(scratch[k], scratch[H + k], scratch[2 * H + k], buf[k], k,
+                     tw, y);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = cmul(y[q], S[k + q * H]);
+        // sum_q i^(sq) Y_q for s = 0..3
+        const double2 a = cadd(y[0], y[2]), b = csub(y[0], y[2]);
+        const double2 c = cadd(y[1], y[3]), d = rot<1>(csub(y[1], y[3]));
+        scratch[k] = cadd(a, c);
+        scratch[H + k] = cmul(cadd(b, d), conjg(tw[k]));
+        scratch[2 * H + k] = cmul(csub(a, c), conjg(tw[2 * k]));
+        buf[k] = cmul(csub(b, d), conjg(tw[3 * k]));
+    }
+    __syncthreads();
+    // phases 3, 2, 1 into scratch, then phase 0 in buf
+    for (int s = 3; s >= 0; --s) {
+        if (s < 3) {
+            for (int k = tid; k < H; k += NT) buf[k] = scratch[s * H + k];
+            __syncthreads();
+        }
+        lds_fft<1, kBigB, false, NT, 2>(buf, H, tw);
+        if (s > 0) {
+            for (int j = tid; j < H; j += NT) scratch[s * H + j] = buf[j];
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ double2 big4_conv(const double2* buf,
+                                             const double2* scratch, int n) {
+    const int s = n & 3, j = n >> 2;
+    return s == 0 ? buf[j] : scratch[s * kBigH + j];
+}
+
+// conv_big (LV = 2, M = 2H) or conv_big4 (LV = 4, M = 4H); scratch of
+// LV * H values
+template <int LV, typename Load>
+__device__ __forceinline__ void conv_lv(double2* buf, double2* scratch, int N,
+                                        const double2* __restrict__ tw,
+                                        const double2* __restrict__ S,
+                                        Load load) {
+    if constexpr (LV == 2) conv_big(buf, scratch, N, tw, S, load);
+    else conv_big4(buf, scratch, N, tw, S, load);
+}
+
+template <int LV>
+__device__ __forceinline__ double2 conv_lv_out(const double2* buf,
+                                               const double2* scratch, int n) {
+    if constexpr (LV == 2) return (n & 1) ? scratch[kBigH + (n >> 1)]
+                                          : buf[n >> 1];
+    else return big4_conv(buf, scratch, n);
+}
+
 // Forward FFT of length 2H of x (LOAD as in conv_big): STORE(k, X[k]) for
 // every k < 2H.  All kBigNT threads.
 template <typename Load, typename Store>

@@ -816,7 +816,7 @@ def _dfi_plan(width, filter_name, dev):
         if nbytes == 0:
             raise NotImplementedError(
                 "dfi_reconstruction on the GPU supports padded widths "
-                "ncol + 2*int(pad_rate*ncol) up to 8191 (got %d)" % width)
+                "ncol + 2*int(pad_rate*ncol) up to 16383 (got %d)" % width)
         plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         win = None
         if filter_name is not None:

@@ -393,8 +393,8 @@ int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
                                then shift */
 
 /* Bytes of a DFI plan for padded width `width` (odd; the Bluestein FFT
- * length, the smallest power of two >= 2*width - 1, must be <= 16384, so
- * width <= 8191); 0 if unsupported. */
+ * length, the smallest power of two >= 2*width - 1, must be <= 32768, so
+ * width <= 16383); 0 if unsupported. */
 size_t ath_dfi_plan_bytes(int width);
 
 /* Builds the plan: twiddles, Bluestein chirp and kernel spectrum for the

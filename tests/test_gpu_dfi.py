@@ -6,9 +6,10 @@ rounding (Bluestein vs numpy's FFT, order of the spline sums).  For float32
 sinograms the shifted sinogram and its log are rounded to float32 on both
 sides; on these seeded inputs every such rounding agrees (a disagreement
 would move one sample by a float32 ulp, ~1e-8 of the result, and fail).
-At the full detector widths (2560 and 4096 columns, Bluestein lengths 8192
-and 16384) where the oracle would take minutes: linearity in the sinogram
-(float64, apply_log off) and the analytic phantom's reconstruction error.
+At the full detector widths (2560, 4096 and 7192 columns, Bluestein lengths
+8192, 16384 and 32768) where the oracle would take minutes: linearity in the
+sinogram (float64, apply_log off) and the analytic phantom's reconstruction
+error.
 """
 
 import warnings
@@ -176,6 +177,9 @@ def test_dfi_stack_and_tensors():
     # width 4097, Bluestein length 16384 (two-level transform), an odd
     # row count (the last row pair has one row), float32 with the log
     (47, 4097, 2048.3, 0.0, np.float32, True),
+    # width 8193, Bluestein length 32768 (radix-4 level), rows longer than
+    # the LDS-resident centre shift (the segmented shift), float32 + log
+    (31, 8193, 4090.6, 0.0, np.float32, True),
 ])
 def test_dfi_large_vs_oracle(nrow, ncol, center, pad_rate, dtype, log):
     from oracle import dfi_np as D
@@ -189,10 +193,11 @@ def test_dfi_large_vs_oracle(nrow, ncol, center, pad_rate, dtype, log):
     assert err <= 1e-12
 
 
-@pytest.mark.parametrize("ncol", [2560, 4096])
+@pytest.mark.parametrize("ncol", [2560, 4096, 7192])
 def test_dfi_full_width_linearity_and_phantom(ncol):
-    """1801 x 2560 (width 3841, Bluestein length 8192) and 1801 x 4096
-    (width 6145, length 16384): the float64 path is linear in the sinogram
+    """1801 x 2560 (width 3841, Bluestein length 8192), 1801 x 4096
+    (width 6145, length 16384) and 1801 x 7192 (the stitched volume's
+    width: 10789, length 32768): the float64 path is linear in the sinogram
     (apply_log off), and an analytic ellipse phantom's sinogram
     reconstructs to the phantom."""
     import torch
@@ -225,8 +230,8 @@ def test_dfi_full_width_linearity_and_phantom(ncol):
 
 def test_dfi_limits_and_warnings():
     from algotom_amd.rec import reconstruction as rec
-    with pytest.raises(NotImplementedError):  # padded width 9001 > 8191
-        rec.dfi_reconstruction(_sino(64, 6000, np.float32, 1), 3000.0,
+    with pytest.raises(NotImplementedError):  # padded width 21001 > 16383
+        rec.dfi_reconstruction(_sino(8, 14000, np.float32, 1), 7000.0,
                                apply_log=False)
     with pytest.warns(Warning, match="undersampled"):
         rec.dfi_reconstruction(_sino(12, 65, np.float32, 2), 32.0,

@@ -132,6 +132,14 @@ def test_forward_projection_ws_rejects_small_workspace():
         assert st == nat.ATH_ERR_WORKSPACE
 
 
+def test_dfi_plan_sizes_cover_bluestein_32768():
+    lib = nat.load()
+    assert lib.ath_dfi_plan_bytes(8191) > 0       # length 16384
+    assert lib.ath_dfi_plan_bytes(16383) > lib.ath_dfi_plan_bytes(8191)
+    assert lib.ath_dfi_plan_bytes(16385) == 0     # length 65536
+    assert lib.ath_dfi_plan_bytes(1000) == 0      # even width
+
+
 def test_dropin_fails_loudly_without_device():
     import torch
     if torch.cuda.is_available():
