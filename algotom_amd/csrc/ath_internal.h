@@ -111,9 +111,9 @@ int back_projection(const BpArgs& a, int sino_dtype, int acc, int sb, int py,
 int back_projection_naive(const BpArgs& a, int acc, hipStream_t stream);
 
 // ---- forward projection = adjoint of the back-projection ------------------
-// largest FFT length of the 1D ramp filter (and of its adjoint)
+// largest FFT length of the 1D ramp filter and of its adjoint
 constexpr int kMaxFilterFft = 32768;
-constexpr int kMaxFilterAdjointFft = 16384;
+constexpr int kMaxFilterAdjointFft = 32768;
 
 struct FpArgs {
     const float* image;    // (ncol, nslice, ncol) = (y, slice, x)

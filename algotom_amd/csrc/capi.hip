@@ -544,7 +544,7 @@ int ath_ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
         (fft_len & (fft_len - 1)) || (long long)fft_len < 2 * width)
         return set_error(ATH_ERR_INVALID_ARG,
                          "adjoint filter: fft_len must be a power of two "
-                         ">= 2*(ncol+2*pad), at most 16384");
+                         ">= 2*(ncol+2*pad), at most 32768");
     return ramp_filter_adjoint(in, nrow, nslice, ncol, pad, pad_mode, plan,
                                fft_len, nwin, out,
                                static_cast<hipStream_t>(stream));

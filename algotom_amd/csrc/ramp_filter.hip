@@ -464,24 +464,36 @@ __device__ __forceinline__ void big_z(const double2* buf,
     zhi = csub(ek, o);
 }
 
-template <typename Tin, typename Tout>
+// ADJ (the filter's transpose, as in filter_row): the W outputs
+// z[n] = y[W + n] fold back through the transpose of the pad index map.
+// Outputs c that receive folded pad samples (c <= pad or c >= ncol - pad - 1
+// for the index-map modes) need values of both inverse phases, so the odd
+// phase's values they use (their own z[pad + c] and the pad samples) wait
+// in `stash` (kAdjStash doubles of LDS past the row); every other output is
+// stored by the phase that computes it.
+constexpr int kAdjMaxPad = 700;
+constexpr int kAdjStash = 4 * kAdjMaxPad + 2;
+
+template <typename Tin, typename Tout, bool ADJ = false>
 __device__ __forceinline__ void filter_row_big(
     double2* buf, const Tin* __restrict__ src_row,
                                Tout* __restrict__ dst, int ncol, int pad,
                                int pad_mode, int apply_log,
                                const double2* __restrict__ tw,
-                               const double2* __restrict__ G) {
+                               const double2* __restrict__ G,
+                               double* stash = nullptr) {
     constexpr int H = kBigH, M = 2 * H, NT = kBigRowNT;
     const int tid = threadIdx.x;
     const int width = ncol + 2 * pad;
     PadVals pv{0.0, 0.0};
-    const bool value_mode = pad_mode >= ATH_PAD_LINEAR_RAMP && pad > 0;
+    const bool value_mode =
+        !ADJ && pad_mode >= ATH_PAD_LINEAR_RAMP && pad > 0;
     if (value_mode)
         pv = row_pad_values<NT, Tin>(buf, src_row, ncol, pad, pad_mode,
                                      apply_log);
     auto val = [&](int k) {
-        return padded_sample<Tin, false>(src_row, k, width, ncol, pad,
-                                         pad_mode, apply_log, value_mode, pv);
+        return padded_sample<Tin, ADJ>(src_row, k, width, ncol, pad,
+                                       pad_mode, apply_log, value_mode, pv);
     };
     BigRegs r;
     big_forward_halves(buf, tw, val, r);
@@ -517,10 +529,24 @@ __device__ __forceinline__ void filter_row_big(
     }
     __syncthreads();
     // IFFT_H(V) gives the odd outputs y[2j+1], then IFFT_H(U) the even
-    // ones.  Output c is padded sample q = W + pad + c: component q & 1 of
-    // y[q >> 1], at buf[q >> 2] of the transform of parity (q >> 1) & 1.
-    const int off = width + pad;
+    // ones.  Real sample q is component q & 1 of y[q >> 1], at buf[q >> 2]
+    // of the transform of parity (q >> 1) & 1.
     const double* y = reinterpret_cast<const double*>(buf);
+    auto phase_of = [](int q) { return (q >> 1) & 1; };
+    auto at = [&](int q) { return y[2 * (q >> 2) + (q & 1)]; };
+    // ADJ: output c gets folds when c <= pad or c >= ncol - pad - 1; its
+    // interior value waits at slot c or pad + 1 + c - (ncol - pad - 1), a
+    // pad sample n at 2 pad + 2 + (n < pad ? n : n - ncol)
+    const int lo_end = pad, hi_beg = ncol - pad - 1;
+    auto folded = [&](int c) {
+        return pad_mode != ATH_PAD_CONSTANT && (c <= lo_end || c >= hi_beg);
+    };
+    auto int_slot = [&](int c) {
+        return c <= lo_end ? c : pad + 1 + (c - hi_beg);
+    };
+    auto pad_slot = [&](int n) {
+        return 2 * pad + 2 + (n < pad ? n : n - ncol);
+    };
     for (int h = 1; h >= 0; --h) {
         if (h == 0) {
 #pragma unroll
@@ -533,12 +559,57 @@ __device__ __forceinline__ void filter_row_big(
             __syncthreads();
         }
         lds_fft<1, kBigRowMaxb, false, NT, 2>(buf, H, tw);
-        for (int c = tid; c < ncol; c += NT) {
-            const int q = off + c;
-            if (((q >> 1) & 1) == h)
-                store_elem(dst + c, y[2 * (q >> 2) + (q & 1)]);
+        if constexpr (!ADJ) {
+            // output c is padded sample q = W + pad + c
+            for (int c = tid; c < ncol; c += NT) {
+                const int q = width + pad + c;
+                if (phase_of(q) == h) store_elem(dst + c, at(q));
+            }
+        } else {
+            // z[n] = y[W + n]; this phase's values of the unfolded outputs
+            // are stored, the odd phase's values the folds need wait
+            for (int n = tid; n < width; n += NT) {
+                const int q = width + n;
+                if (phase_of(q) != h) continue;
+                const bool interior = n >= pad && n < pad + ncol;
+                const int c = n - pad;
+                if (interior && !folded(c)) {
+                    store_elem(dst + c, at(q));
+                } else if (h == 1) {
+                    stash[interior ? int_slot(c) : pad_slot(n)] = at(q);
+                }
+            }
+            if (h == 0) {
+                // the folded outputs, in filter_row's order: the interior
+                // value, then the pad samples whose source is c
+                auto z = [&](int n) {
+                    const int q = width + n;
+                    if (phase_of(q) == 0) return at(q);
+                    const bool interior = n >= pad && n < pad + ncol;
+                    return stash[interior ? int_slot(n - pad) : pad_slot(n)];
+                };
+                for (int c = tid; c < ncol; c += NT) {
+                    if (!folded(c)) continue;
+                    double v = z(pad + c);
+                    if (pad_mode == ATH_PAD_EDGE) {
+                        if (c == 0)
+                            for (int n = 0; n < pad; ++n) v += z(n);
+                        if (c == ncol - 1)
+                            for (int n = pad + ncol; n < width; ++n)
+                                v += z(n);
+                    } else {
+                        for (int n = 0; n < pad; ++n)
+                            if (pad_source(n - pad, ncol, pad_mode) == c)
+                                v += z(n);
+                        for (int n = pad + ncol; n < width; ++n)
+                            if (pad_source(n - pad, ncol, pad_mode) == c)
+                                v += z(n);
+                    }
+                    store_elem(dst + c, v);
+                }
+            }
         }
-        __syncthreads();  // buf free for U, then for the next row
+        __syncthreads();  // buf and stash free for U, then the next row
     }
 }
 
@@ -556,6 +627,28 @@ __global__ __launch_bounds__(kBigRowNT) void ramp_filter_big_kernel(
         filter_row_big<Tin, Tout>(buf, sino + row * (size_t)ncol,
                                   out + row * (size_t)ncol, ncol, pad,
                                   pad_mode, apply_log, tw, G);
+    }
+}
+
+// Transpose of ramp_filter_big_kernel (float32 in and out), persistent;
+// the stash follows the row's LDS.
+constexpr size_t kBigAdjLds = kBigRowLds + (size_t)kAdjStash * sizeof(double);
+
+__global__ __launch_bounds__(kBigRowNT) void ramp_filter_adjoint_big_kernel(
+    const float* __restrict__ in, float* __restrict__ out, size_t rows,
+    int nslice, int ncol, int pad, int pad_mode,
+    const double2* __restrict__ tw, const double2* __restrict__ spec,
+    int nwin) {
+    extern __shared__ double2 buf[];
+    double* stash = reinterpret_cast<double*>(
+        reinterpret_cast<char*>(buf) + kBigRowLds);
+    constexpr int M = 2 * kBigH;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const double2* G =
+            spec + (nwin > 1 ? (row / nslice) : 0) * (size_t)(M + 1);
+        filter_row_big<float, float, true>(buf, in + row * (size_t)ncol,
+                                           out + row * (size_t)ncol, ncol,
+                                           pad, pad_mode, 0, tw, G, stash);
     }
 }
 
@@ -892,6 +985,24 @@ int launch_adjoint(const float* in, int nrow, int nslice, int ncol, int pad,
 int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                         int pad, int pad_mode, const void* plan, int fft_len,
                         int nwin, float* out, hipStream_t s) {
+    if (fft_len == 4 * kBigH) {
+        if (pad > kAdjMaxPad || pad >= ncol)
+            return set_error(ATH_ERR_UNSUPPORTED,
+                             "ramp filter adjoint at FFT length 32768: pad "
+                             "must be below ncol and at most 700");
+        const double2* tw = static_cast<const double2*>(plan);
+        const double2* spec = tw + fft_len;
+        auto kern = ramp_filter_adjoint_big_kernel;
+        prepare_lds(kern, kBigAdjLds);
+        const size_t rows = (size_t)nrow * nslice;
+        const size_t grid = std::min(
+            rows, (size_t)resident_blocks(kern, kBigAdjLds, kBigRowNT));
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kBigRowNT),
+                           kBigAdjLds, s, in, out, rows, nslice, ncol, pad,
+                           pad_mode, tw, spec, nwin);
+        return check_hip(hipGetLastError(),
+                         "ramp_filter_adjoint_big_kernel launch");
+    }
     switch (maxb_for(fft_len / 2)) {
     case 1: return launch_adjoint<1>(in, nrow, nslice, ncol, pad, pad_mode,
                                      plan, fft_len, nwin, out, s);
@@ -903,7 +1014,7 @@ int ramp_filter_adjoint(const float* in, int nrow, int nslice, int ncol,
                                      plan, fft_len, nwin, out, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "ramp filter adjoint: FFT length above 16384 is not "
+                     "ramp filter adjoint: FFT length above 32768 is not "
                      "supported");
 }
 

@@ -119,8 +119,9 @@ int ath_device_count(int* count);
  * evaluates the reference's length-W circular convolution exactly as a
  * linear convolution with the doubled kernel [g, g] on this length.  The
  * filter runs up to L = 32768 (ncol <= 16159 at pad 150);
- * ATH_ERR_UNSUPPORTED above.  ath_ramp_filter_adjoint and the circular
- * (BPF) filter stop at 16384. */
+ * ATH_ERR_UNSUPPORTED above.  ath_ramp_filter_adjoint also runs up to
+ * 32768 (there with pad <= 700); the circular (BPF) filter stops at
+ * 16384. */
 int ath_filter_fft_length(int ncol, int pad);
 
 /* Device bytes of a filter plan holding `nwin` kernel spectra. */

@@ -123,6 +123,43 @@ def test_filter_adjoint_is_dense_transpose(mode):
     assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("mode", ["edge", "constant", "reflect", "symmetric",
+                                  "wrap"])
+def test_filter_adjoint_fft_32768(mode):
+    """8000 columns, pad 150: the adjoint needs FFT length 32768 (the
+    two-half path with the folded outputs' values stashed in LDS).  Oracle:
+    the transpose of F = crop . C_g . pad in float64 -- u embedded at
+    [pad, pad + ncol) of a zero row, circular cross-correlation with the
+    taps g (numpy FFT), then folded through numpy.pad's index map."""
+    import torch
+    from algotom_amd.rec import autograd
+    rng = np.random.default_rng(8)
+    nrow, ncol, pad = 3, 8000, 150
+    geom = autograd.FBPGeometry(nrow, ncol, pad=pad, pad_mode=mode,
+                                ratio=None)
+    assert geom.fft_len_adj == 32768
+    u = rng.standard_normal((nrow, 2, ncol)).astype(np.float32)
+    got = geom.filter_adjoint(torch.from_numpy(u).cuda()).cpu().numpy()
+    W = ncol + 2 * pad
+    g = np.asarray(geom.taps, dtype=np.float64).reshape(-1, W)[0]
+    if mode == "constant":
+        src = np.pad(np.arange(ncol), pad, mode="constant",
+                     constant_values=-1)
+    else:
+        src = np.pad(np.arange(ncol), pad, mode=mode)
+    ghat = np.conj(np.fft.fft(g))
+    for r in range(nrow):
+        for j in range(2):
+            e = np.zeros(W)
+            e[pad:pad + ncol] = u[r, j].astype(np.float64)
+            z = np.real(np.fft.ifft(np.fft.fft(e) * ghat))
+            ref = np.zeros(ncol)
+            keep = src >= 0
+            np.add.at(ref, src[keep], z[keep])
+            err = np.abs(got[r, j] - ref).max() / np.abs(ref).max()
+            assert err <= 1e-6, (mode, r, j, err)
+
+
 def _fp64_loss(sino, center, angles, target, ratio=1.0):
     """float64 loss sum((FBP(s; c) - target)^2) with the oracle (centre not
     rounded, so finite differences below float32 resolution work)."""
