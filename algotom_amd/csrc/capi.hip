@@ -744,7 +744,7 @@ int ath_minus_log(const float* in, float* out, long long n, void* stream) {
 }
 
 static bool circular_len_ok(int L) {
-    return L >= 64 && L <= 16384 && (L & (L - 1)) == 0;
+    return L >= 64 && L <= 32768 && (L & (L - 1)) == 0;
 }
 
 size_t ath_circular_filter_plan_bytes(int fft_len) {
@@ -766,7 +766,7 @@ int ath_circular_filter_plan_build(const double* kernel2d, int width,
     if (width < 1 || !circular_len_ok(fft_len) || fft_len < width)
         return set_error(ATH_ERR_INVALID_ARG,
                          "circular filter: fft_len must be a power of two in "
-                         "[64, 16384] and at least the width");
+                         "[64, 32768] and at least the width");
     if (plan_bytes < circular_plan_bytes(fft_len) ||
         work_bytes < circular_plan_work_bytes(width, fft_len))
         return set_error(ATH_ERR_WORKSPACE,
@@ -806,7 +806,7 @@ int ath_circular_ramp_filter(const float* images, int nimg, int ncol,
         return set_error(ATH_ERR_INVALID_ARG,
                          "circular filter: fft_len must be a power of two, "
                          "at least ath_filter_fft_length(ncol, pad) and at "
-                         "most 16384");
+                         "most 32768");
     if (workspace_bytes < circular_filter_work_bytes(ncol, pad, fft_len, nimg))
         return set_error(ATH_ERR_WORKSPACE,
                          "circular filter: workspace too small");

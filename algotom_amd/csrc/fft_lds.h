@@ -374,6 +374,23 @@ __device__ __forceinline__ void big4_combine(double2 f0, double2 f1,
     x[3] = csub(b, d);
 }
 
+// Forward FFT of length 4H of x (LOAD as in conv_big4): STORE(k, X[k]) for
+// every k < 4H.  All kBigNT threads; ends synchronised.
+template <typename Load, typename Store>
+__device__ void fft_big4_forward(double2* buf, double2* scratch, int N,
+                                 const double2* __restrict__ tw, Load load,
+                                 Store store) {
+    fft_big4_quarters(buf, scratch, N, tw, load);
+    for (int k = threadIdx.x; k < kBigH; k += kBigNT) {
+        double2 x[4];
+        big4_combine(scratch[k], scratch[kBigH + k], scratch[2 * kBigH + k],
+                     buf[k], k, tw, x);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) store(k + q * kBigH, x[q]);
+    }
+    __syncthreads();
+}
+
 template <typename Load>
 __device__ void conv_big4(double2* buf, double2* scratch, int N,
                           const double2* __restrict__ tw,
@@ -414,6 +431,16 @@ __device__ __forceinline__ double2 big4_conv(const double2* buf,
                                              const double2* scratch, int n) {
     const int s = n & 3, j = n >> 2;
     return s == 0 ? buf[j] : scratch[s * kBigH + j];
+}
+
+// fft_big_forward (LV = 2) or fft_big4_forward (LV = 4)
+template <int LV, typename Load, typename Store>
+__device__ __forceinline__ void fft_lv_forward(double2* buf, double2* scratch,
+                                               int N,
+                                               const double2* __restrict__ tw,
+                                               Load load, Store store) {
+    if constexpr (LV == 2) fft_big_forward(buf, scratch, N, tw, load, store);
+    else fft_big4_forward(buf, scratch, N, tw, load, store);
 }
 
 // conv_big (LV = 2, M = 2H) or conv_big4 (LV = 4, M = 4H); scratch of
@@ -470,6 +497,166 @@ __device__ __forceinline__ void split_real(double2 zk, double2 zm, double2 wk,
     const double2 wo = cmul(wk, o);
     xk = cadd(e, wo);
     xm = conjg(csub(e, wo));
+}
+
+// The inverse of split_real for the pair (k, km = (M - k) mod M),
+// 0 <= k <= M/2: from the real spectrum's values Y[k], Y[M-k] the packed
+// input of the length-M inverse at k and km (zm untouched when k == km).
+// tw: the length-2M table.
+__device__ __forceinline__ void merge_pair(double2 yk, double2 ym, int k,
+                                           int M,
+                                           const double2* __restrict__ tw,
+                                           double2& zk, double2& zm) {
+    const int km = (M - k) & (M - 1);
+    const double2 wk = tw[k];
+    // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
+    const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
+    const double2 ok = rot<1>(cmul(conjg(wk),
+                                   cscale(csub(yk, conjg(ym)), 0.5)));
+    zk = cadd(ek, ok);
+    if (k != km) {
+        // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
+        const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
+        const double2 om = rot<-1>(cmul(wk,
+                                        cscale(csub(ym, conjg(yk)), 0.5)));
+        zm = cadd(em, om);
+    }
+}
+
+// ---- real transforms of length 4H = 32768 (M = 2H packed points) ---------
+// One workgroup of kBigNT threads per row, H = 8192 complex float64 in LDS
+// and no global scratch.  Forward, decimation in time: E = FFT_H(z[2j])
+// moves into registers (thread t keeps E[k] and E[H-k] for its
+// k = t + NT b < H/2, thread 0 also E[H/2]) while O = FFT_H(z[2j+1]) runs in
+// LDS; then Z[k] = E[k] + W^k O[k] and Z[k+H] = E[k] - W^k O[k]
+// (W = exp(-2 pi i/M)) at the four indices k, H-k, H+k, 2H-k, which are the
+// two pairs (k, M-k) and (H-k, H+k) of the real split.  Inverse,
+// decimation in frequency: U[k] = Y[k] + Y[k+H] stays in registers and
+// V[k] = (Y[k] - Y[k+H]) W^-k goes to LDS; IFFT_H(V) gives the odd outputs
+// y[2j+1], then IFFT_H(U) the even ones.  The H-point transforms read the
+// 4H-point table at stride 2.
+constexpr int kBigKB = kBigH / 2 / kBigNT;  // k slots per thread (8)
+
+struct BigRegs {
+    double2 v[kBigKB][2];
+    double2 mid;
+};
+
+// E = FFT_H of the packed samples val(4j), val(4j+1) into r; then O =
+// FFT_H of val(4j+2), val(4j+3) left in buf.  All threads; ends
+// synchronised.
+template <typename Val>
+__device__ __forceinline__ void big_forward_halves(
+    double2* buf, const double2* __restrict__ tw, Val val, BigRegs& r) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1) {
+#pragma unroll
+            for (int b = 0; b < kBigKB; ++b) {
+                const int k = tid + NT * b;
+                r.v[b][0] = buf[k];
+                r.v[b][1] = buf[(H - k) & (H - 1)];
+            }
+            if (tid == 0) r.mid = buf[H / 2];
+            __syncthreads();
+        }
+        for (int j = tid; j < H; j += NT)
+            buf[j] = make_double2(val(4 * j + 2 * h), val(4 * j + 2 * h + 1));
+        __syncthreads();
+        lds_fft<-1, kBigB, false, NT, 2>(buf, H, tw);
+    }
+}
+
+// Z[k] and Z[k + H] from E[k] (register) and O[k] (buf)
+__device__ __forceinline__ void big_z(const double2* buf,
+                                      const double2* __restrict__ tw, int k,
+                                      double2 ek, double2& zlo, double2& zhi) {
+    const double2 o = cmul(buf[k], tw[2 * k]);
+    zlo = cadd(ek, o);
+    zhi = csub(ek, o);
+}
+
+// After big_forward_halves: put(k, Z[k], Z[(M - k) mod M]) once for every
+// pair of the real split, 0 <= k <= M/2 (the self pairs 0 and H with the
+// same value twice).  Each pair is one thread's.
+template <typename Put>
+__device__ __forceinline__ void big_pairs(const double2* buf,
+                                          const double2* __restrict__ tw,
+                                          const BigRegs& r, Put put) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int b = 0; b < kBigKB; ++b) {
+        const int k = tid + NT * b;
+        double2 z0, z1;
+        big_z(buf, tw, k, r.v[b][0], z0, z1);
+        if (k == 0) {
+            put(0, z0, z0);
+            put(H, z1, z1);
+        } else {
+            double2 z2, z3;
+            big_z(buf, tw, H - k, r.v[b][1], z2, z3);
+            put(k, z0, z3);
+            put(H - k, z2, z1);
+        }
+    }
+    if (tid == 0) {
+        double2 z0, z1;
+        big_z(buf, tw, H / 2, r.mid, z0, z1);
+        put(H / 2, z0, z1);
+    }
+}
+
+// From the inverse's inputs Y at k, H+k (y0, y1) and, for k > 0, at H-k,
+// 2H-k (y2, y3): U into the slot's registers, V into buf.
+__device__ __forceinline__ void big_store_uv(double2* buf,
+                                             const double2* __restrict__ tw,
+                                             int k, double2 y0, double2 y1,
+                                             double2 y2, double2 y3,
+                                             double2 (&u)[2]) {
+    constexpr int H = kBigH;
+    u[0] = cadd(y0, y1);
+    buf[k] = cmul(csub(y0, y1), conjg(tw[2 * k]));
+    if (k != 0) {
+        u[1] = cadd(y2, y3);
+        buf[H - k] = cmul(csub(y2, y3), conjg(tw[2 * (H - k)]));
+    }
+}
+
+// the same for the index H/2 (y at H/2 and 3H/2), thread 0
+__device__ __forceinline__ void big_store_mid(double2* buf,
+                                              const double2* __restrict__ tw,
+                                              double2 y0, double2 y1,
+                                              double2& mid) {
+    mid = cadd(y0, y1);
+    buf[kBigH / 2] = cmul(csub(y0, y1), conjg(tw[kBigH]));
+}
+
+// The two inverse phases after big_store_uv / big_store_mid (callers
+// synchronise first): out(1) runs with the odd outputs in buf, out(0) with
+// the even ones.  Real sample q is component q & 1 of y[q >> 1], at
+// buf[q >> 2] in phase (q >> 1) & 1.  Ends synchronised.
+template <typename Out>
+__device__ __forceinline__ void big_inverse_phases(
+    double2* buf, const double2* __restrict__ tw, const BigRegs& r, Out out) {
+    constexpr int H = kBigH, NT = kBigNT;
+    const int tid = threadIdx.x;
+    for (int h = 1; h >= 0; --h) {
+        if (h == 0) {
+#pragma unroll
+            for (int b = 0; b < kBigKB; ++b) {
+                const int k = tid + NT * b;
+                buf[k] = r.v[b][0];
+                if (k != 0) buf[H - k] = r.v[b][1];
+            }
+            if (tid == 0) buf[H / 2] = r.mid;
+            __syncthreads();
+        }
+        lds_fft<1, kBigB, false, NT, 2>(buf, H, tw);
+        out(h);
+        __syncthreads();  // buf free for U, then for the caller
+    }
 }
 
 // tw[n] = exp(-2 pi i n / L), n < L

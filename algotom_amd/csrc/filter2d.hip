@@ -142,18 +142,18 @@ __global__ __launch_bounds__(F2Cfg<MAXB>::nt) void cols_kernel(
         dst[(size_t)y * (M + 1)] = buf[r0 + y];
 }
 
-// Column pass at L = 2H = 16384 (conv_big / fft_big_forward), persistent
-// over the (image, bin column) pairs; workgroup b uses scratch slot b.
-// tw: the row transforms' table, exp(-2 pi i n / L), which is the H-point
-// table the two-level transform needs.
-template <bool SPECTRUM>
+// Column pass at L = LV * H (16384: conv_big / fft_big_forward; 32768:
+// conv_big4 / fft_big4_forward), persistent over the (image, bin column)
+// pairs; workgroup b uses scratch slot b (L values).  tw: the plan's table
+// exp(-2 pi i n / L), which is the table both forms need.
+template <bool SPECTRUM, int LV>
 __global__ __launch_bounds__(kBigNT) void cols_big_kernel(
     const double2* __restrict__ T, int nrows_in, const double2* __restrict__ tw,
     double2* __restrict__ khat_out, const double2* __restrict__ khat,
     double2* __restrict__ U, int r0, int nout, int M, double scale,
     int nimg, double2* __restrict__ scratch_all) {
     extern __shared__ double2 buf[];
-    double2* scratch = scratch_all + (size_t)blockIdx.x * 2 * kBigH;
+    double2* scratch = scratch_all + (size_t)blockIdx.x * LV * kBigH;
     const int L = 2 * M;
     const long long ncols = (long long)(M + 1) * nimg;
     for (long long c = blockIdx.x; c < ncols; c += gridDim.x) {
@@ -163,13 +163,16 @@ __global__ __launch_bounds__(kBigNT) void cols_big_kernel(
         auto load = [&](int n) -> double2 { return col[(size_t)n * (M + 1)]; };
         if (SPECTRUM) {
             double2* dst = khat_out + (size_t)k * L;
-            fft_big_forward(buf, scratch, nrows_in, tw, load,
-                            [&](int i, double2 v) { dst[i] = cscale(v, scale); });
+            fft_lv_forward<LV>(
+                buf, scratch, nrows_in, tw, load,
+                [&](int i, double2 v) { dst[i] = cscale(v, scale); });
         } else {
-            conv_big(buf, scratch, nrows_in, tw, khat + (size_t)k * L, load);
+            conv_lv<LV>(buf, scratch, nrows_in, tw, khat + (size_t)k * L,
+                        load);
             double2* dst = U + (size_t)img * nout * (M + 1) + k;
             for (int y = threadIdx.x; y < nout; y += kBigNT)
-                dst[(size_t)y * (M + 1)] = big_conv(buf, scratch, r0 + y);
+                dst[(size_t)y * (M + 1)] =
+                    conv_lv_out<LV>(buf, scratch, r0 + y);
             __syncthreads();
         }
     }
@@ -189,8 +192,8 @@ inline int big_slots() {
 }
 
 inline size_t big_scratch_bytes(int L) {
-    return L == 2 * kBigH
-               ? (size_t)big_slots() * 2 * kBigH * sizeof(double2)
+    return L == 2 * kBigH || L == 4 * kBigH
+               ? (size_t)big_slots() * L * sizeof(double2)
                : 0;
 }
 
@@ -261,6 +264,119 @@ __global__ __launch_bounds__(F2Cfg<MAXB>::nt) void rows_inv_kernel(
     }
 }
 
+// mask and pi/(N-1) of output (x, y) in the reference's order and dtype
+template <typename Tout>
+__device__ __forceinline__ Tout finish_value(double v, int x, int y,
+                                             const Finish2d& f) {
+    bool keep = true;
+    if (f.use_mask) {
+        const long long dx = x - f.mask_c, dy = y - f.mask_c;
+        keep = (double)(dx * dx + dy * dy) <= f.mask_r2;
+    }
+    if constexpr (sizeof(Tout) == 8) {
+        double r = v * (keep ? 1.0 : 0.0);
+        if (f.scale) r = r * f.pi64 / f.denom64;
+        return r;
+    } else {
+        float r = (float)v * (keep ? 1.0f : 0.0f);
+        if (f.scale) {
+            r = r * f.pi32;
+            r = r / f.denom32;
+        }
+        return r;
+    }
+}
+
+// ---- L = 32768: rows as two-half transforms (fft_lds.h) -------------------
+// Forward real FFT of one row as rows_fwd_kernel, the packed 16384 points
+// as two 8192-point halves.  tw: exp(-2 pi i n / L).
+template <int SRC>
+__global__ __launch_bounds__(kBigNT) void rows_fwd_big_kernel(
+    const float* __restrict__ images, long long row_stride,
+    long long img_stride, const double* __restrict__ g2, int ncol, int pad,
+    int pad_mode, int width, const double2* __restrict__ tw,
+    double2* __restrict__ T) {
+    extern __shared__ double2 buf[];
+    constexpr int M = 2 * kBigH;
+    const int u = blockIdx.x, nrows = gridDim.x, img = blockIdx.y;
+    const float* src_row = nullptr;
+    const double* g_row = nullptr;
+    if (SRC == kSrcImage) {
+        const int su = pad_source(u - pad, ncol, pad_mode);
+        if (su >= 0)
+            src_row = images + img * img_stride + (long long)su * row_stride;
+    } else {
+        g_row = g2 + (size_t)(u % width) * width;
+    }
+    auto val = [&](int col) -> double {
+        if (SRC == kSrcImage) {
+            if (src_row && col < width) {
+                const int sv = pad_source(col - pad, ncol, pad_mode);
+                if (sv >= 0) return (double)src_row[sv];
+            }
+            return 0.0;
+        }
+        return col < 2 * width ? g_row[col < width ? col : col - width] : 0.0;
+    };
+    BigRegs r;
+    big_forward_halves(buf, tw, val, r);
+    double2* out = T + ((size_t)img * nrows + u) * (M + 1);
+    big_pairs(buf, tw, r, [&](int k, double2 zk, double2 zm) {
+        double2 xk, xm;
+        split_real(zk, zm, tw[k], xk, xm);
+        out[k] = xk;
+        out[M - k] = xm;
+    });
+}
+
+// Row inverse as rows_inv_kernel: the Hermitian merge of U[y][0..M] pair by
+// pair, the two-half inverse, crop, finish.
+template <typename Tout>
+__global__ __launch_bounds__(kBigNT) void rows_inv_big_kernel(
+    const double2* __restrict__ U, const double2* __restrict__ tw,
+    Tout* __restrict__ out, Finish2d f) {
+    extern __shared__ double2 buf[];
+    constexpr int H = kBigH, M = 2 * H, NT = kBigNT;
+    const int tid = threadIdx.x;
+    const int y = blockIdx.x;
+    const int img = blockIdx.y;
+    const double2* Y = U + ((size_t)img * gridDim.x + y) * (M + 1);
+    BigRegs r;
+#pragma unroll
+    for (int b = 0; b < kBigKB; ++b) {
+        const int k = tid + NT * b;
+        double2 z0, z1, z2, z3;  // the inverse's inputs at k, H+k, H-k, 2H-k
+        if (k == 0) {
+            // packed bin 0 takes the DC and Nyquist values; bin H pairs
+            // with itself
+            merge_pair(Y[0], Y[M], 0, M, tw, z0, z0);
+            merge_pair(Y[H], Y[H], H, M, tw, z1, z1);
+            z2 = z0;
+            z3 = z1;
+        } else {
+            merge_pair(Y[k], Y[M - k], k, M, tw, z0, z3);
+            merge_pair(Y[H - k], Y[H + k], H - k, M, tw, z2, z1);
+        }
+        big_store_uv(buf, tw, k, z0, z1, z2, z3, r.v[b]);
+    }
+    if (tid == 0) {
+        double2 z0, z1;
+        merge_pair(Y[H / 2], Y[M - H / 2], H / 2, M, tw, z0, z1);
+        big_store_mid(buf, tw, z0, z1, r.mid);
+    }
+    __syncthreads();
+    const double* v = reinterpret_cast<const double*>(buf);
+    Tout* row = out + img * f.out_img_stride + (long long)y * f.out_row_stride;
+    big_inverse_phases(buf, tw, r, [&](int h) {
+        for (int x = tid; x < f.ncol; x += NT) {
+            const int q = f.c0 + x;
+            if (((q >> 1) & 1) == h)
+                row[x] = finish_value<Tout>(v[2 * (q >> 2) + (q & 1)], x, y,
+                                            f);
+        }
+    });
+}
+
 template <int MAXB_R, int MAXB_C>
 int build_plan(const double* g2, int width, int L, void* plan, void* work,
                hipStream_t s) {
@@ -295,7 +411,10 @@ int build_plan(const double* g2, int width, int L, void* plan, void* work,
 }
 
 // L = 16384: the row passes as for L <= 8192 (class 8: 1024 threads), the
-// column pass two-level.  work: the row-spectrum rows, then the scratch.
+// column pass two-level.  L = 32768: the row passes as two-half
+// transforms, the column pass radix-4 two-level.  work: the row-spectrum
+// rows, then the scratch.
+template <int LV>
 int build_plan_big(const double* g2, int width, int L, void* plan,
                    void* work, hipStream_t s) {
     const int M = L / 2;
@@ -311,15 +430,23 @@ int build_plan_big(const double* g2, int width, int L, void* plan,
     auto* T = static_cast<double2*>(work);
     const int krows = std::min(2 * width, L);
     auto* scr = T + (size_t)krows * (M + 1);
-    auto rk = rows_fwd_kernel<8, kSrcKernel>;
-    const size_t lds_r = (size_t)M * sizeof(double2);
-    prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
-    hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(F2Cfg<8>::nt), lds_r, s,
-                       nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE, width,
-                       tw, T, M);
+    const size_t lds_r = (size_t)(LV == 4 ? kBigH : M) * sizeof(double2);
+    if constexpr (LV == 4) {
+        auto rk = rows_fwd_big_kernel<kSrcKernel>;
+        prepare_lds(rk, lds_r);
+        hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(kBigNT), lds_r, s,
+                           nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE,
+                           width, tw, T);
+    } else {
+        auto rk = rows_fwd_kernel<8, kSrcKernel>;
+        prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
+        hipLaunchKernelGGL(rk, dim3(krows, 1), dim3(F2Cfg<8>::nt), lds_r, s,
+                           nullptr, 0LL, 0LL, g2, width, 0, ATH_PAD_EDGE,
+                           width, tw, T, M);
+    }
     st = check_hip(hipGetLastError(), "rows_fwd_kernel (kernel) launch");
     if (st != ATH_OK) return st;
-    auto ck = cols_big_kernel<true>;
+    auto ck = cols_big_kernel<true, LV>;
     const size_t lds_c = (size_t)kBigH * sizeof(double2);
     prepare_lds(ck, lds_c);
     const int grid = std::min(M + 1, big_slots());
@@ -365,7 +492,7 @@ int run_filter(const float* images, int nimg, int ncol, long long row_stride,
     return check_hip(hipGetLastError(), "rows_inv_kernel launch");
 }
 
-template <typename Tout>
+template <int LV, typename Tout>
 int run_filter_big(const float* images, int nimg, int ncol,
                    long long row_stride, long long img_stride, int pad,
                    int pad_mode, const void* plan, int L, void* work,
@@ -376,15 +503,23 @@ int run_filter_big(const float* images, int nimg, int ncol,
     auto* T = static_cast<double2*>(work);
     auto* U = T + (size_t)nimg * width * (M + 1);
     auto* scr = U + (size_t)nimg * ncol * (M + 1);
-    auto rk = rows_fwd_kernel<8, kSrcImage>;
-    const size_t lds_r = (size_t)M * sizeof(double2);
-    prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
-    hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(F2Cfg<8>::nt), lds_r, s,
-                       images, row_stride, img_stride, nullptr, ncol, pad,
-                       pad_mode, width, p.tw, T, M);
+    const size_t lds_r = (size_t)(LV == 4 ? kBigH : M) * sizeof(double2);
+    if constexpr (LV == 4) {
+        auto rk = rows_fwd_big_kernel<kSrcImage>;
+        prepare_lds(rk, lds_r);
+        hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(kBigNT), lds_r, s,
+                           images, row_stride, img_stride, nullptr, ncol, pad,
+                           pad_mode, width, p.tw, T);
+    } else {
+        auto rk = rows_fwd_kernel<8, kSrcImage>;
+        prepare_lds(rk, 1024 * (size_t)8 * sizeof(double2));
+        hipLaunchKernelGGL(rk, dim3(width, nimg), dim3(F2Cfg<8>::nt), lds_r,
+                           s, images, row_stride, img_stride, nullptr, ncol,
+                           pad, pad_mode, width, p.tw, T, M);
+    }
     int st = check_hip(hipGetLastError(), "rows_fwd_kernel launch");
     if (st != ATH_OK) return st;
-    auto ck = cols_big_kernel<false>;
+    auto ck = cols_big_kernel<false, LV>;
     const size_t lds_c = (size_t)kBigH * sizeof(double2);
     prepare_lds(ck, lds_c);
     const long long ncols = (long long)(M + 1) * nimg;
@@ -394,10 +529,17 @@ int run_filter_big(const float* images, int nimg, int ncol,
                        scr);
     st = check_hip(hipGetLastError(), "cols_big_kernel launch");
     if (st != ATH_OK) return st;
-    auto ik = rows_inv_kernel<8, Tout>;
-    prepare_lds(ik, 1024 * (size_t)8 * sizeof(double2));
-    hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(F2Cfg<8>::nt), lds_r, s, U,
-                       p.tw, out, f, M);
+    if constexpr (LV == 4) {
+        auto ik = rows_inv_big_kernel<Tout>;
+        prepare_lds(ik, lds_r);
+        hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(kBigNT), lds_r, s, U,
+                           p.tw, out, f);
+    } else {
+        auto ik = rows_inv_kernel<8, Tout>;
+        prepare_lds(ik, 1024 * (size_t)8 * sizeof(double2));
+        hipLaunchKernelGGL(ik, dim3(ncol, nimg), dim3(F2Cfg<8>::nt), lds_r, s,
+                           U, p.tw, out, f, M);
+    }
     return check_hip(hipGetLastError(), "rows_inv_kernel launch");
 }
 
@@ -428,10 +570,11 @@ int circular_plan_build(const double* g2, int width, int L, void* plan,
     case 2048: return build_plan<1, 2>(g2, width, L, plan, work, s);
     case 4096: return build_plan<2, 4>(g2, width, L, plan, work, s);
     case 8192: return build_plan<4, 8>(g2, width, L, plan, work, s);
-    case 16384: return build_plan_big(g2, width, L, plan, work, s);
+    case 16384: return build_plan_big<2>(g2, width, L, plan, work, s);
+    case 32768: return build_plan_big<4>(g2, width, L, plan, work, s);
     }
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "circular ramp filter: FFT length above 16384");
+                     "circular ramp filter: FFT length above 32768");
 }
 
 template <typename Tout>
@@ -450,13 +593,17 @@ static int dispatch_filter(const float* images, int nimg, int ncol,
     case 4096: ATH_F2D(2, 4)
     case 8192: ATH_F2D(4, 8)
     case 16384:
-        return run_filter_big<Tout>(images, nimg, ncol, row_stride,
-                                    img_stride, pad, pad_mode, plan, L, work,
-                                    out, f, s);
+        return run_filter_big<2, Tout>(images, nimg, ncol, row_stride,
+                                       img_stride, pad, pad_mode, plan, L,
+                                       work, out, f, s);
+    case 32768:
+        return run_filter_big<4, Tout>(images, nimg, ncol, row_stride,
+                                       img_stride, pad, pad_mode, plan, L,
+                                       work, out, f, s);
     }
 #undef ATH_F2D
     return set_error(ATH_ERR_UNSUPPORTED,
-                     "circular ramp filter: FFT length above 16384");
+                     "circular ramp filter: FFT length above 32768");
 }
 
 int circular_ramp_filter(const float* images, int nimg, int ncol,

@@ -205,23 +205,9 @@ __device__ __forceinline__ void spectral_pair(double2& zk, double2& zm, int k,
                                               const double2* __restrict__ tw,
                                               const double2* __restrict__ G) {
     const int km = (M - k) & (M - 1);
-    const double2 wk = tw[k];
     double2 xk, xm;
-    split_real(zk, k == km ? zk : zm, wk, xk, xm);
-    const double2 yk = cmul(xk, G[k]);
-    const double2 ym = cmul(xm, G[M - k]);
-    // Zy[k] = (Yk + conj Ym)/2 + i conj(wk) (Yk - conj Ym)/2
-    const double2 ek = cscale(cadd(yk, conjg(ym)), 0.5);
-    const double2 ok = rot<1>(cmul(conjg(wk),
-                                   cscale(csub(yk, conjg(ym)), 0.5)));
-    zk = cadd(ek, ok);
-    if (k != km) {
-        // Zy[M-k] = (Ym + conj Yk)/2 - i wk (Ym - conj Yk)/2
-        const double2 em = cscale(cadd(ym, conjg(yk)), 0.5);
-        const double2 om = rot<-1>(cmul(wk,
-                                        cscale(csub(ym, conjg(yk)), 0.5)));
-        zm = cadd(em, om);
-    }
+    split_real(zk, k == km ? zk : zm, tw[k], xk, xm);
+    merge_pair(cmul(xk, G[k]), cmul(xm, G[M - k]), k, M, tw, zk, zm);
 }
 
 // Filters one row in LDS buffer `buf` (M double2): pad, rfft, multiply by
@@ -406,63 +392,13 @@ __global__ __launch_bounds__(kThreads) void kernel_spectrum_kernel(
 }
 
 // ---- FFT length L = 32768: M = 16384 = 2H packed points --------------------
-// One workgroup of kBigRowNT threads per row, H = 8192 complex float64 in
-// LDS (128 KB) and no global scratch.  Forward, decimation in time: E =
-// FFT_H(z[2j]) moves into registers (thread t keeps E[k] and E[H-k] for its
-// k = t + NT b <= H/2) while O = FFT_H(z[2j+1]) runs in LDS; then Z[k] =
-// E[k] + W^k O[k] and Z[k+H] = E[k] - W^k O[k] (W = exp(-2 pi i/M)) at the
-// four indices k, H-k, H+k, 2H-k, which are the two pairs (k, M-k) and
-// (H-k, H+k) of the kernel product.  Inverse, decimation in frequency: U[k]
-// = Y[k] + Y[k+H] stays in registers and V[k] = (Y[k] - Y[k+H]) W^-k goes to
-// LDS; IFFT_H(V) gives the odd outputs y[2j+1] (stored at once), IFFT_H(U)
-// the even ones.  The H-point transforms read the L-point table at stride 2.
-constexpr int kBigRowNT = 512;
-constexpr int kBigRowKB = kBigH / 2 / kBigRowNT;  // k slots per thread (8)
-constexpr int kBigRowMaxb = 4;   // lds_fft butterflies per thread
+// One workgroup of kBigNT threads per row through the two-half transforms of
+// fft_lds.h (BigRegs, big_forward_halves, big_z, big_store_uv,
+// big_inverse_phases): H = 8192 complex float64 in LDS and no global
+// scratch.
 // the row's LDS plus room for the median's histogram past ncol keys
 constexpr size_t kBigRowLds = (size_t)kBigH * sizeof(double2) + 256;
-
-// Registers of one thread across the two halves: for its k = t + NT b
-// (0 <= k < H/2) the values at k and H-k, and thread 0's value at H/2.
-struct BigRegs {
-    double2 v[kBigRowKB][2];
-    double2 mid;
-};
-
-// E = FFT_H of the packed samples val(4j), val(4j+1) into r; then O =
-// FFT_H of val(4j+2), val(4j+3) left in buf.  All threads; ends
-// synchronised.
-template <typename Val>
-__device__ __forceinline__ void big_forward_halves(
-    double2* buf, const double2* __restrict__ tw, Val val, BigRegs& r) {
-    constexpr int H = kBigH, NT = kBigRowNT;
-    const int tid = threadIdx.x;
-    for (int h = 0; h < 2; ++h) {
-        if (h == 1) {
-#pragma unroll
-            for (int b = 0; b < kBigRowKB; ++b) {
-                const int k = tid + NT * b;
-                r.v[b][0] = buf[k];
-                r.v[b][1] = buf[(H - k) & (H - 1)];
-            }
-            if (tid == 0) r.mid = buf[H / 2];
-            __syncthreads();
-        }
-        for (int j = tid; j < H; j += NT)
-            buf[j] = make_double2(val(4 * j + 2 * h), val(4 * j + 2 * h + 1));
-        __syncthreads();
-        lds_fft<-1, kBigRowMaxb, false, NT, 2>(buf, H, tw);
-    }
-}
-
-// Z[k] and Z[k + H] from E[k] (register) and O[k] (buf)
-__device__ __forceinline__ void big_z(const double2* buf,
-                                      const double2* __restrict__ tw, int k,
-                                      double2 ek, double2& zlo, double2& zhi) {
-    const double2 o = cmul(buf[k], tw[2 * k]);
-    zlo = cadd(ek, o);
-    zhi = csub(ek, o);
-}
+constexpr int kBigRowNT = kBigNT;
 
 // ADJ (the filter's transpose, as in filter_row): the W outputs
 // z[n] = y[W + n] fold back through the transpose of the pad index map.
@@ -499,33 +435,27 @@ __device__ __forceinline__ void filter_row_big(
     big_forward_halves(buf, tw, val, r);
     // kernel product: U into r, V into buf (every index has one owner)
 #pragma unroll
-    for (int b = 0; b < kBigRowKB; ++b) {
+    for (int b = 0; b < kBigKB; ++b) {
         const int k = tid + NT * b;
         double2 z0, z1;  // Z[k], Z[H+k]
         big_z(buf, tw, k, r.v[b][0], z0, z1);
+        double2 z2 = z0, z3 = z1;  // Z[H-k], Z[2H-k]
         if (k == 0) {
             // the self pairs Z[0] and Z[H]
             spectral_pair(z0, z0, 0, M, tw, G);
             spectral_pair(z1, z1, H, M, tw, G);
-            r.v[b][0] = cadd(z0, z1);
-            buf[0] = csub(z0, z1);
         } else {
-            double2 z2, z3;  // Z[H-k], Z[2H-k]
             big_z(buf, tw, H - k, r.v[b][1], z2, z3);
             spectral_pair(z0, z3, k, M, tw, G);      // (k, M-k)
             spectral_pair(z2, z1, H - k, M, tw, G);  // (H-k, H+k)
-            r.v[b][0] = cadd(z0, z1);
-            r.v[b][1] = cadd(z2, z3);
-            buf[k] = cmul(csub(z0, z1), conjg(tw[2 * k]));
-            buf[H - k] = cmul(csub(z2, z3), conjg(tw[2 * (H - k)]));
         }
+        big_store_uv(buf, tw, k, z0, z1, z2, z3, r.v[b]);
     }
     if (tid == 0) {  // the one pair (H/2, 3H/2)
         double2 z0, z1;
         big_z(buf, tw, H / 2, r.mid, z0, z1);
         spectral_pair(z0, z1, H / 2, M, tw, G);
-        r.mid = cadd(z0, z1);
-        buf[H / 2] = cmul(csub(z0, z1), conjg(tw[H]));
+        big_store_mid(buf, tw, z0, z1, r.mid);
     }
     __syncthreads();
     // IFFT_H(V) gives the odd outputs y[2j+1], then IFFT_H(U) the even
@@ -547,18 +477,7 @@ __device__ __forceinline__ void filter_row_big(
     auto pad_slot = [&](int n) {
         return 2 * pad + 2 + (n < pad ? n : n - ncol);
     };
-    for (int h = 1; h >= 0; --h) {
-        if (h == 0) {
-#pragma unroll
-            for (int b = 0; b < kBigRowKB; ++b) {
-                const int k = tid + NT * b;
-                buf[k] = r.v[b][0];
-                if (k != 0) buf[H - k] = r.v[b][1];
-            }
-            if (tid == 0) buf[H / 2] = r.mid;
-            __syncthreads();
-        }
-        lds_fft<1, kBigRowMaxb, false, NT, 2>(buf, H, tw);
+    big_inverse_phases(buf, tw, r, [&](int h) {
         if constexpr (!ADJ) {
             // output c is padded sample q = W + pad + c
             for (int c = tid; c < ncol; c += NT) {
@@ -609,8 +528,7 @@ __device__ __forceinline__ void filter_row_big(
                 }
             }
         }
-        __syncthreads();  // buf and stash free for U, then the next row
-    }
+    });
 }
 
 template <typename Tin, typename Tout>
@@ -685,8 +603,7 @@ __global__ __launch_bounds__(kBigRowNT) void kernel_spectrum_big_kernel(
     const double* __restrict__ taps, int width,
     const double2* __restrict__ tw, double2* __restrict__ spec) {
     extern __shared__ double2 buf[];
-    constexpr int H = kBigH, M = 2 * H, NT = kBigRowNT;
-    const int tid = threadIdx.x;
+    constexpr int M = 2 * kBigH;
     const double* g = taps + (size_t)blockIdx.x * width;
     double2* G = spec + (size_t)blockIdx.x * (M + 1);
     auto val = [&](int k) {
@@ -695,32 +612,12 @@ __global__ __launch_bounds__(kBigRowNT) void kernel_spectrum_big_kernel(
     BigRegs r;
     big_forward_halves(buf, tw, val, r);
     const double inv = 1.0 / (double)M;
-    auto put = [&](int k, double2 zk, double2 zm) {
+    big_pairs(buf, tw, r, [&](int k, double2 zk, double2 zm) {
         double2 xk, xm;
         split_real(zk, zm, tw[k], xk, xm);
         G[k] = cscale(xk, inv);
         G[M - k] = cscale(xm, inv);
-    };
-#pragma unroll
-    for (int b = 0; b < kBigRowKB; ++b) {
-        const int k = tid + NT * b;
-        double2 z0, z1;
-        big_z(buf, tw, k, r.v[b][0], z0, z1);
-        if (k == 0) {
-            put(0, z0, z0);
-            put(H, z1, z1);
-        } else {
-            double2 z2, z3;
-            big_z(buf, tw, H - k, r.v[b][1], z2, z3);
-            put(k, z0, z3);
-            put(H - k, z2, z1);
-        }
-    }
-    if (tid == 0) {
-        double2 z0, z1;
-        big_z(buf, tw, H / 2, r.mid, z0, z1);
-        put(H / 2, z0, z1);
-    }
+    });
 }
 
 // Size class of the row kernels for a packed length M: maxb_for(M) up to

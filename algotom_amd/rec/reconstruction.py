@@ -569,7 +569,7 @@ def _circular_plan(ramp_win, dev, name_key=None):
     with g2 = Re(ifft2(ifftshift(R2))) (reconstruction.py:555-559),
     evaluated by ath_circular_ramp_filter.  One window is kept resident
     (a plan holds the kernel's 2D spectrum: 537 MB at FFT length 8192,
-    2.1 GB at 16384).
+    2.1 GB at 16384, 8.6 GB at 32768).
     Default windows are keyed by (width, filter_name); a caller's window by
     a hash of its values."""
     import hashlib
@@ -605,7 +605,7 @@ def _circular_plan_for(entry, fft_len, dev):
         nwork = lib.ath_circular_filter_plan_work_bytes(width, fft_len)
         if nbytes == 0 or nwork == 0:
             raise NotImplementedError(
-                "the circular ramp filter supports FFT lengths up to 16384 "
+                "the circular ramp filter supports FFT lengths up to 32768 "
                 "(width %d needs %d)" % (width, fft_len))
         plan = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         work = torch.empty(nwork, dtype=torch.uint8, device=dev)

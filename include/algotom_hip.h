@@ -119,9 +119,8 @@ int ath_device_count(int* count);
  * evaluates the reference's length-W circular convolution exactly as a
  * linear convolution with the doubled kernel [g, g] on this length.  The
  * filter runs up to L = 32768 (ncol <= 16159 at pad 150);
- * ATH_ERR_UNSUPPORTED above.  ath_ramp_filter_adjoint also runs up to
- * 32768 (there with pad <= 700); the circular (BPF) filter stops at
- * 16384. */
+ * ATH_ERR_UNSUPPORTED above.  ath_ramp_filter_adjoint (with pad <= 700) and
+ * the circular (BPF) filter run up to 32768 too. */
 int ath_filter_fft_length(int ncol, int pad);
 
 /* Device bytes of a filter plan holding `nwin` kernel spectra. */
@@ -345,7 +344,7 @@ int ath_minus_log(const float* in, float* out, long long n, void* stream);
 
 /* Bytes of a circular-filter plan for FFT length fft_len (twiddles + the
  * 2D spectrum of the tiled kernel, float64); 0 if fft_len is not a power
- * of two in [64, 16384].  fft_len comes from ath_filter_fft_length(ncol,
+ * of two in [64, 32768].  fft_len comes from ath_filter_fft_length(ncol,
  * pad), the same rule as the 1D filter. */
 size_t ath_circular_filter_plan_bytes(int fft_len);
 

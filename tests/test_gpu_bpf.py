@@ -1,7 +1,7 @@
 """BPF on the GPU against the oracle (oracle/bpf_np.py).
 
 * apply_circular_ramp_filter: float64 output within 1e-11 of the oracle's
-  complex128 spectral product (FFT lengths 64 .. 16384, every pad mode);
+  complex128 spectral product (FFT lengths 64 .. 32768, every pad mode);
 * bpf_reconstruction, one sinogram: float64 output within 1e-10 (the
   back-projection is bit-identical; the filters differ by float64 rounding);
 * stacks: float32 output within 2e-6 of max (float32 rounding flips);
@@ -20,7 +20,9 @@ pytestmark = pytest.mark.gpu
     (65, 7, "reflect"), (65, 7, "symmetric"), (65, 7, "wrap"),
     (257, None, "edge"), (512, None, "edge"), (1024, None, "edge"),
     (2560, None, "edge"),
-    (4096, None, "edge")])  # W = 4396: FFT length 16384, two-level columns
+    (4096, None, "edge"),   # W = 4396: FFT length 16384, two-level columns
+    # FFT length 32768: two-half row transforms, radix-4 two-level columns
+    (7968, 150, "reflect"), (8000, None, "edge")])
 def test_circular_filter_vs_oracle(ncol, pad, mode):
     from oracle import bpf_np
     from algotom_amd.rec import reconstruction as rec
