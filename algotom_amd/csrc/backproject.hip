@@ -651,21 +651,27 @@ int launch_batched(const ath_bp_job* jobs, int njobs, long long total,
     return check_hip(hipGetLastError(), "bp_batched_kernel launch");
 }
 
-// Angles staged per LDS block.  Larger blocks mean fewer barriers and more
-// staging loads in flight per phase, as long as the records still leave
-// room for three workgroups per CU.  Measured on MI355X (the angle order,
-// hence every result bit, does not depend on it):
-//   one slice (SB = 1): 64 angles; the 256-sinogram uneven batch took
-//     299.3 / 222.9 / 211.9 ms (F64 / fp32 / fp16) against 323.7 / 246.6 /
-//     234.9 ms with 16 and 243.9 ms (fp16) with 128;
-//   16-bit records, SB >= 2: 32 angles; the 64 x 1801 x 2560 fp16 stack
-//     117.1 ms against 122.0 with 16 and 136.7 with 64;
-//   float32 and float64 records, SB >= 2: 16 (32 leaves two workgroups
-//     per CU).
-template <int MODE, int SB>
+// Angles staged per LDS block: the largest of 64 / 32 / 16 whose shared
+// arrays stay within 53 KB (three workgroups per CU).  Larger blocks mean
+// fewer barriers and more staging loads in flight per phase.  The angle
+// order, hence every result bit, does not depend on it.  Measured on MI355X:
+//   one slice (SB = 1, 64 angles): the 256-sinogram uneven batch 299.3 /
+//     222.9 / 211.9 ms (F64 / fp32 / fp16) against 323.7 / 246.6 / 234.9 ms
+//     with 16 angles and 243.9 ms (fp16) with 128;
+//   16-bit records, 4 slices x 32 rows (32 angles): the 64 x 1801 x 2560
+//     fp16 stack 117.1 ms against 122.0 with 16 and 136.7 with 64;
+//   image + centre derivative, 2 slices x 16 rows (32 angles): 4.97 against
+//     5.22 ms at 16 x 901 x 1024, 447.6 against 478.4 ms at 16 x 1801 x 7192;
+//   the centre derivative alone stays at 16 (3.35 against 3.46 ms with 32);
+//   float32 and float64 records, 4 slices: 16 (32 would not fit).
+template <int MODE, int SB, int PY>
 constexpr int angle_block() {
-    if (SB == 1) return 64;
-    return (MODE == kAccH16 || MODE == kAccB16) ? 32 : 16;
+    if (MODE == ATH_ACC_DERIV) return 16;
+    using Rec = typename Mode<MODE>::Rec;
+    constexpr int span = mode_span<Mode<MODE>>(kTileX, 4 * PY);
+    constexpr size_t per_angle =
+        (size_t)span * SB * sizeof(Rec) + 24 + 8 * SB;
+    return per_angle * 64 <= 53000 ? 64 : (per_angle * 32 <= 53000 ? 32 : 16);
 }
 
 template <int MODE, typename Tin>
@@ -673,7 +679,7 @@ int batched_config(const ath_bp_job* jobs, int njobs, long long total,
                    int sb, int py, int nblocks, int* counter, hipStream_t s) {
 #define ATH_BPB_CASE(SB_, PY_)                                             \
     if (sb == SB_ && py == PY_)                                            \
-        return launch_batched<MODE, SB_, PY_, angle_block<MODE, SB_>(),    \
+        return launch_batched<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(), \
                               Tin>(jobs, njobs, total, nblocks, counter, s);
     ATH_BPB_CASE(1, 4) ATH_BPB_CASE(1, 8) ATH_BPB_CASE(1, 16)
     ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
@@ -692,7 +698,8 @@ int launch_config(const BpArgs& a, int sb, int py, hipStream_t s) {
     constexpr bool wide = (MODE == kAccH16 || MODE == kAccB16);
 #define ATH_BP_CASE(SB_, PY_)                                              \
     if (sb == SB_ && py == PY_)                                            \
-        return launch_tiled<MODE, SB_, PY_, angle_block<MODE, SB_>(), Tin>( \
+        return launch_tiled<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(),  \
+                            Tin>(                                          \
             a, s);
     ATH_BP_CASE(1, 4) ATH_BP_CASE(1, 8) ATH_BP_CASE(1, 16)
     ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
