@@ -13,13 +13,14 @@
 // Kernel design (bp_tiled_kernel): a 256-thread workgroup owns a 64 x (4*PY)
 // pixel tile of SB slices.  Lane l of wave w owns pixel column x0+l and the
 // PY rows y0 + w*PY + k, so one wave instruction touches 64 consecutive x
-// and its LDS gathers hit consecutive columns (no bank conflicts).  Angles
+// and its LDS gathers hit neighbouring or shared detector columns.  Angles
 // are processed in blocks of A: for each angle the workgroup computes the
 // tile's detector footprint [dlo, dlo+SPAN) exactly from its four corners
 // and stages one record per (column, slice) into LDS -- the sample and its
 // forward difference, interleaved over the SB slices so one pixel reads
-// all of them with one or two ds_read_b128.  f, d and the weight are
-// computed once per pixel and angle and reused by the SB slices.  Angles
+// all of them with one or two ds_read_b128 (RecPlanes below).  f, d and
+// the weight are computed once per pixel and angle and reused by the SB
+// slices.  Angles
 // whose footprint lies inside [0, ncol-1] for the whole tile (every interior
 // tile) skip the range test; the rest take the exact per-pixel test.  Each
 // pixel sums its angles in ascending order, as the reference does, so
@@ -96,6 +97,7 @@ template <>
 struct Mode<ATH_ACC_F64> {
     static constexpr bool kF32Pos = false;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
     using Rec = float2;
     using W = double;
     using Acc = double;
@@ -116,6 +118,7 @@ template <>
 struct Mode<ATH_ACC_F32SEQ> {
     static constexpr bool kF32Pos = false;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
     using Rec = float2;
     using W = double;
     using Acc = float;
@@ -136,6 +139,7 @@ template <>
 struct Mode<ATH_ACC_F32> {
     static constexpr bool kF32Pos = false;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = true;
     using Rec = float2;
     using W = float;
     using Acc = float;
@@ -159,6 +163,7 @@ template <>
 struct Mode<ATH_ACC_DERIV> {
     static constexpr bool kF32Pos = false;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
     using Rec = float2;
     using W = double;
     using Acc = double;
@@ -184,6 +189,7 @@ template <>
 struct Mode<ATH_ACC_F64_DERIV> {
     static constexpr bool kF32Pos = false;
     static constexpr bool kTwoOut = true;
+    static constexpr bool kPlanes = false;
     using Rec = float2;
     using W = double;
     using Acc = AccPair;
@@ -211,6 +217,7 @@ struct Mode<kAccH16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
     using Rec = half2_t;
     using W = half2_t;
     using Acc = float;
@@ -242,6 +249,7 @@ struct Mode<kAccB16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
     static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
     using Rec = bf162_t;
     using W = bf162_t;
     using Acc = float;
@@ -282,6 +290,28 @@ __device__ __forceinline__ float finalize(const BpArgs& a, float v, int x,
     return v;
 }
 
+// LDS record layout of one angle: SB slices per column, SPAN columns.
+// By default a column's SB records are contiguous (one or two ds_read_b128
+// per pixel).  Modes with kPlanes split them into NP planes of P slices (16
+// bytes of records per column and plane), each plane SPAN columns long, so
+// that neighbouring lanes on neighbouring columns read neighbouring 16-byte
+// words.  Measured on MI355X with the 64 x 1801 x 2560 stack, 4 slices:
+// the float32 mode takes 105.6 ms in planes with 16-row tiles against
+// 116.1 ms (32 rows) and 123.8 ms (16 rows) contiguous; the float64-sum
+// modes are VALU-bound and 2-4 % slower in planes, so they stay contiguous.
+template <class Rec, int SB, bool PLANES>
+struct RecPlanes {
+    static constexpr int kPer16 = (int)(16 / sizeof(Rec));
+    static constexpr int P = PLANES && kPer16 >= 1 && kPer16 < SB ? kPer16 : SB;
+    static constexpr int NP = SB / P;
+    static_assert(NP * P == SB, "slice batch must be a multiple of the plane");
+    // records from the column's first record to slice js's
+    template <int SPAN>
+    static constexpr int slice(int js) {
+        return (js / P) * SPAN * P + js % P;
+    }
+};
+
 // One angle over the thread's PY x SB pixels when the whole tile is in
 // range: f >= 0, so the truncation is the floor and fract(f) = f - floor(f).
 // The record address is one shift-add on a workgroup-uniform offset.
@@ -291,7 +321,11 @@ __device__ __forceinline__ void inner_angle(
     double xc, const double (&yc)[PY], double cen,
     typename M::Acc (&acc)[PY][SB]) {
     using Rec = typename M::Rec;
-    const int off = __builtin_amdgcn_readfirstlane((al * SPAN - dlo) * SB);
+    using L = RecPlanes<Rec, SB, M::kPlanes>;
+    // (a multiple of P records, so every plane read is one aligned
+    // ds_read_b128)
+    const int off =
+        __builtin_amdgcn_readfirstlane(al * SPAN * L::NP - dlo) * L::P;
     const Rec* qbase = s_rec + off;
     if constexpr (M::kF32Pos) {
         // f = (y - ic) s + [(x - ic) c + center]; the bracket in float64,
@@ -303,9 +337,10 @@ __device__ __forceinline__ void inner_angle(
             const float f = fmaf((float)yc[k], s32, xcc);
             const int d = floor_to_int(f);
             const auto w = M::weight32(__builtin_amdgcn_fractf(f));
-            const Rec* q = qbase + d * SB;
+            const Rec* q = qbase + d * L::P;
 #pragma unroll
-            for (int js = 0; js < SB; ++js) M::add(acc[k][js], q[js], w);
+            for (int js = 0; js < SB; ++js)
+                M::add(acc[k][js], q[L::template slice<SPAN>(js)], w);
         }
         return;
     }
@@ -315,9 +350,10 @@ __device__ __forceinline__ void inner_angle(
         const double f = det_pos(xcc, yc[k], s, cen);
         const int d = (int)f;
         const auto w = M::weight(__builtin_amdgcn_fract(f));
-        const Rec* q = qbase + d * SB;
+        const Rec* q = qbase + d * L::P;
 #pragma unroll
-        for (int js = 0; js < SB; ++js) M::add(acc[k][js], q[js], w);
+        for (int js = 0; js < SB; ++js)
+            M::add(acc[k][js], q[L::template slice<SPAN>(js)], w);
     }
 }
 
@@ -338,6 +374,7 @@ __device__ __forceinline__ void bp_tile(
              mode_span<Mode<MODE>>(kTileX, 4 * PY)>& sh) {
     using M = Mode<MODE>;
     using Rec = typename M::Rec;
+    using L = RecPlanes<Rec, SB, M::kPlanes>;
     constexpr int TY = 4 * PY;
     constexpr int SPAN = mode_span<M>(kTileX, TY);
     Rec* s_rec = sh.rec;
@@ -414,9 +451,10 @@ __device__ __forceinline__ void bp_tile(
                 }
                 r[js] = M::make(yd, yu);
             }
-            Rec* dst = s_rec + e * SB;
+            Rec* dst = s_rec + al * SPAN * SB + k * L::P;
 #pragma unroll
-            for (int js = 0; js < SB; ++js) dst[js] = r[js];
+            for (int js = 0; js < SB; ++js)
+                dst[L::template slice<SPAN>(js)] = r[js];
         }
         if (a.edge_pad) {
             for (int e = tid; e < na * SB; e += kThreads) {
@@ -472,11 +510,12 @@ __device__ __forceinline__ void bp_tile(
                         w = M::weight(f - (double)d);
                     }
                     const int idx = min(max(d - dlo, 0), SPAN - 1);
-                    const Rec* q = base + idx * SB;
+                    const Rec* q = base + idx * L::P;
                     if (in) {
 #pragma unroll
                         for (int js = 0; js < SB; ++js)
-                            M::add(acc[k][js], q[js], w);
+                            M::add(acc[k][js],
+                                   q[L::template slice<SPAN>(js)], w);
                     } else if (a.edge_pad) {
                         const int side = f < 0.0 ? 0 : 1;
 #pragma unroll
@@ -720,7 +759,10 @@ template <int MODE>
 void default_config(int nslice, int& sb, int& py) {
     sb = nslice <= 2 ? nslice : 4;
     py = 8;
-    if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == ATH_ACC_F64_DERIV))
+    // float64 sums: 4 x 16 (164.9 against 193.9 ms for 4 x 32); float32
+    // records in planes: 4 x 16 (105.6 against 121.3 ms)
+    if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == ATH_ACC_F64_DERIV ||
+                       MODE == ATH_ACC_F32))
         py = 4;
     // the centre derivative reads one value per record: 2 slices x 16 rows
     // (16 x 901 x 1024: 3.47 ms against 4.84 ms for 4 x 32; 16 x 1801 x

@@ -161,6 +161,26 @@ def bp_tile_footprints(ncol, angles, center, py=8, tx=64):
     return worst, span
 
 
+def bp_rec_planes(rec_bytes, sb, planes):
+    """RecPlanes<Rec, SB, PLANES>: (P, NP) = slices per plane, planes."""
+    per16 = 16 // rec_bytes
+    p = per16 if planes and 1 <= per16 < sb else sb
+    return p, sb // p
+
+
+def bp_rec_stage_index(al, k, js, span, sb, p):
+    """Record index bp_tile's staging loop writes for (angle, column, slice)."""
+    return al * span * sb + k * p + (js // p) * span * p + js % p
+
+
+def bp_rec_read_index(al, d, dlo, js, span, sb, p):
+    """Record index inner_angle reads: the wave-uniform offset (a multiple of
+    P), plus d * P, plus the slice's offset within the column."""
+    npl = sb // p
+    off = (al * span * npl - dlo) * p
+    return off + d * p + (js // p) * span * p + js % p
+
+
 # ---- forward_project_hat_kernel's staging ---------------------------------
 def fp_block(p, nrow, ncol, nslice, fsb):
     """forward_project_hat_kernel's workgroup id -> (bin block, angle block,

@@ -64,6 +64,37 @@ def test_bp_staging_span_bound(ncol, nang, center):
     assert worst <= span - 2
 
 
+@pytest.mark.parametrize("rec_bytes,sb,planes", [(8, 1, True), (8, 2, True),
+                                                 (8, 4, True), (8, 4, False),
+                                                 (4, 8, True), (4, 4, True),
+                                                 (4, 8, False)])
+def test_bp_record_planes_layout(rec_bytes, sb, planes):
+    # staging writes every (angle, column, slice) to its own record of the
+    # block; inner_angle reads back the record staged for column d - dlo;
+    # each plane of a column starts on a 16-byte boundary when it holds more
+    # than one record
+    span, na = 74, 3
+    p, npl = km.bp_rec_planes(rec_bytes, sb, planes)
+    assert p * npl == sb
+    if planes and sb * rec_bytes > 16:
+        assert p * rec_bytes == 16 and npl > 1
+    else:
+        assert npl == 1
+    seen = set()
+    for al in range(na):
+        for k in range(span):
+            for js in range(sb):
+                i = km.bp_rec_stage_index(al, k, js, span, sb, p)
+                assert 0 <= i < na * span * sb
+                seen.add(i)
+                if js % p == 0 and p * rec_bytes == 16:
+                    assert (i * rec_bytes) % 16 == 0
+                for dlo in (-7, 0, 913):
+                    assert km.bp_rec_read_index(al, dlo + k, dlo, js, span,
+                                                sb, p) == i
+    assert len(seen) == na * span * sb
+
+
 @pytest.mark.parametrize("ncol,ratio", [(64, 1.0), (101, 0.8), (256, None),
                                         (300, 0.0)])
 def test_fp_empty_chunks_hold_no_kept_pixel(ncol, ratio):
