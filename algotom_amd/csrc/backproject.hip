@@ -433,28 +433,58 @@ __device__ __forceinline__ void bp_tile(
         const int all_inner = __syncthreads_and(inner);
         // one record per (angle, column): the SB slices' (s, ds)
         const int nrec = na * SPAN;
-        for (int e = tid; e < nrec; e += kThreads) {
-            const int al = e / SPAN;
-            const int k = e - al * SPAN;
-            const int col = s_dlo[al] + k;
-            const bool in0 = col >= 0 && col < C;
-            const bool in1 = col + 1 >= 0 && col + 1 < C;
-            const Tin* rows = sino + ((size_t)(a0 + al) * S + j0) * C;
-            Rec r[SB];
+        // two records per thread and step, both loaded before either is
+        // stored; a record whose two columns are inside the row and whose
+        // slice group is full loads without per-sample tests.  Against one
+        // record per step with a test per sample: fp16 stack 117.4 ->
+        // 90.7 ms, fp32 106.0 -> 103.6 ms, f32seq 185.9 -> 185.1 ms
+        // (64 x 1801 x 2560)
+        for (int e0 = tid; e0 < nrec; e0 += 2 * kThreads) {
+            float yd[2][SB], yu[2][SB];
+            int at[2];
 #pragma unroll
-            for (int js = 0; js < SB; ++js) {
-                float yd = 0.0f, yu = 0.0f;
-                if (js < nsb) {
-                    const Tin* row = rows + (size_t)js * C;
-                    if (in0) yd = load_f32(row + col);
-                    if (in1) yu = load_f32(row + col + 1);
+            for (int u = 0; u < 2; ++u) {
+                const int e = e0 + u * kThreads;
+                const int al = e / SPAN;
+                const int k = e - al * SPAN;
+                at[u] = e < nrec ? al * SPAN * SB + k * L::P : -1;
+#pragma unroll
+                for (int js = 0; js < SB; ++js) {
+                    yd[u][js] = 0.0f;
+                    yu[u][js] = 0.0f;
                 }
-                r[js] = M::make(yd, yu);
-            }
-            Rec* dst = s_rec + al * SPAN * SB + k * L::P;
+                if (e >= nrec) continue;
+                const int col = s_dlo[al] + k;
+                const Tin* rows =
+                    sino + ((size_t)(a0 + al) * S + j0) * C + col;
+                if (col >= 0 && col + 1 < C && nsb == SB) {
 #pragma unroll
-            for (int js = 0; js < SB; ++js)
-                dst[L::template slice<SPAN>(js)] = r[js];
+                    for (int js = 0; js < SB; ++js) {
+                        yd[u][js] = load_f32(rows + (size_t)js * C);
+                        yu[u][js] = load_f32(rows + (size_t)js * C + 1);
+                    }
+                } else {
+                    const bool in0 = col >= 0 && col < C;
+                    const bool in1 = col + 1 >= 0 && col + 1 < C;
+#pragma unroll
+                    for (int js = 0; js < SB; ++js) {
+                        if (js < nsb) {
+                            const Tin* row = rows + (size_t)js * C;
+                            if (in0) yd[u][js] = load_f32(row);
+                            if (in1) yu[u][js] = load_f32(row + 1);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (at[u] < 0) continue;
+                Rec* dst = s_rec + at[u];
+#pragma unroll
+                for (int js = 0; js < SB; ++js)
+                    dst[L::template slice<SPAN>(js)] =
+                        M::make(yd[u][js], yu[u][js]);
+            }
         }
         if (a.edge_pad) {
             for (int e = tid; e < na * SB; e += kThreads) {

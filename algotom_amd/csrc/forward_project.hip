@@ -239,6 +239,78 @@ __device__ __forceinline__ FpBlock fp_block(const FpArgs& a) {
     return b;
 }
 
+// Stages lines m0 .. m0 + nrows - 1 and minor columns ka .. ka + wid - 1
+// of the chunk into the tile: the thread's kSteps pixels (mask and bounds
+// test once per pixel, pixels outside staged as zeros) are loaded before any
+// is stored, so all their loads are in flight together.  FULL: the slice
+// group holds FSB slices (no per-slice test).  The former loop, one pixel
+// per step with a test per slice, waited for each step's loads before the
+// next: 5.45 against 4.91 ms at 16 x 901 x 1024, 519 against 459 ms at
+// 16 x 1801 x 7192.
+template <int FSB, int kRows, int RS, bool FULL>
+__device__ __forceinline__ void stage_chunk(const FpArgs& a, float* tile,
+                                            int tid, bool rowwise,
+                                            bool by_rows, int m0, int nrows,
+                                            int ka, int wid, int j0,
+                                            int ns) {
+    constexpr int kSteps = kRows * kMaxMinor / 256;
+    static_assert(kSteps * 256 == kRows * kMaxMinor, "whole steps");
+    const int C = a.ncol, S = a.nslice;
+    float v[kSteps][FSB];
+    int dst[kSteps];
+#pragma unroll
+    for (int it = 0; it < kSteps; ++it) {
+        const int e = tid + it * 256;
+        int r, k;
+        if (rowwise) {
+            r = e / kMaxMinor;
+            k = e % kMaxMinor;
+        } else {  // lanes along the lines (x) first
+            r = e % kRows;
+            k = e / kRows;
+        }
+        const bool in = r < nrows && k < wid;
+        dst[it] = in ? (r * kMaxMinor + k) * RS : -1;
+        const int kk = ka + k;
+        const int x = by_rows ? kk : m0 + r;
+        const int y = by_rows ? m0 + r : kk;
+        bool keep = in && kk >= 0 && kk < C;
+        if (keep && a.use_mask) {
+            // integer squares and their sum are exact in float64
+            const double dx = x - a.mask_c, dy = y - a.mask_c;
+            keep = fma(dx, dx, dy * dy) <= a.mask_r2;
+        }
+        // zeros first, then the loads under the branch: 5 % faster than
+        // zeros written in a separate branch of the store
+#pragma unroll
+        for (int js = 0; js < FSB; ++js) v[it][js] = 0.0f;
+        if (keep) {
+            const float* src =
+                by_rows || !a.image_t
+                    ? a.image + ((size_t)y * S + j0) * C + x
+                    : a.image_t + ((size_t)x * S + j0) * C + y;
+#pragma unroll
+            for (int js = 0; js < FSB; ++js)
+                if (FULL || js < ns) v[it][js] = src[(size_t)js * C];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kSteps; ++it) {
+        if (dst[it] < 0) continue;
+        float* d = tile + dst[it];
+        if constexpr (FSB % 4 == 0) {
+#pragma unroll
+            for (int q = 0; q < FSB / 4; ++q)
+                reinterpret_cast<float4*>(d)[q] =
+                    make_float4(v[it][4 * q], v[it][4 * q + 1],
+                                v[it][4 * q + 2], v[it][4 * q + 3]);
+        } else {
+#pragma unroll
+            for (int js = 0; js < FSB; ++js) d[js] = v[it][js];
+        }
+    }
+}
+
 template <int FSB, int kRows>
 __global__ __launch_bounds__(256) void forward_project_hat_kernel(
     FpArgs a, int by_rows_block) {
@@ -356,56 +428,18 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
         if (staged && !empty) {
-            // one pixel per thread and step: the mask test once, then its
-            // FSB slices.  The trip count is a constant, so the loads of
-            // successive steps are in flight together (a loop over exactly
-            // nrows x wid entries measured 6 % slower).  Lanes run along
-            // the staged columns when those are contiguous in memory: rows
-            // of the image, or columns of its transposed copy.
+            // lanes run along the staged columns when those are contiguous
+            // in memory: rows of the image, or columns of its transposed
+            // copy
             const int nrows = m1 - m0 + 1;
-            for (int e = tid; e < kRows * kMaxMinor; e += 256) {
-                int r, k;
-                if (rowwise) {
-                    r = e / kMaxMinor;
-                    k = e % kMaxMinor;
-                } else {        // lanes along the lines (x) first
-                    r = e % kRows;
-                    k = e / kRows;
-                }
-                if (r >= nrows || k >= wid) continue;
-                const int kk = ka + k;
-                const int x = by_rows ? kk : m0 + r;
-                const int y = by_rows ? m0 + r : kk;
-                bool keep = kk >= 0 && kk < C;
-                if (keep && a.use_mask) {
-                    // integer squares and their sum are exact in float64
-                    const double dx = x - a.mask_c, dy = y - a.mask_c;
-                    keep = fma(dx, dx, dy * dy) <= a.mask_r2;
-                }
-                float v[FSB];
-                if (keep) {
-                    const float* src =
-                        by_rows || !a.image_t
-                            ? a.image + ((size_t)y * S + j0) * C + x
-                            : a.image_t + ((size_t)x * S + j0) * C + y;
-#pragma unroll
-                    for (int js = 0; js < FSB; ++js)
-                        v[js] = js < ns ? src[(size_t)js * C] : 0.0f;
-                } else {
-#pragma unroll
-                    for (int js = 0; js < FSB; ++js) v[js] = 0.0f;
-                }
-                float* dst = tile + (r * kMaxMinor + k) * RS;
-                if constexpr (FSB % 4 == 0) {
-#pragma unroll
-                    for (int q = 0; q < FSB / 4; ++q)
-                        reinterpret_cast<float4*>(dst)[q] = make_float4(
-                            v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
-                } else {
-#pragma unroll
-                    for (int js = 0; js < FSB; ++js) dst[js] = v[js];
-                }
-            }
+            if (ns == FSB)
+                stage_chunk<FSB, kRows, RS, true>(a, tile, tid, rowwise,
+                                                  by_rows, m0, nrows, ka, wid,
+                                                  j0, ns);
+            else
+                stage_chunk<FSB, kRows, RS, false>(a, tile, tid, rowwise,
+                                                   by_rows, m0, nrows, ka,
+                                                   wid, j0, ns);
         }
         __syncthreads();
         if (valid && !empty) {
