@@ -217,7 +217,7 @@ struct Mode<kAccH16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
     static constexpr bool kTwoOut = false;
-    static constexpr bool kPlanes = false;
+    static constexpr bool kPlanes = true;
     using Rec = half2_t;
     using W = half2_t;
     using Acc = float;
@@ -249,7 +249,7 @@ struct Mode<kAccB16> {
     // detector position in float32 from a float64 per-angle offset
     static constexpr bool kF32Pos = true;
     static constexpr bool kTwoOut = false;
-    static constexpr bool kPlanes = false;
+    static constexpr bool kPlanes = true;
     using Rec = bf162_t;
     using W = bf162_t;
     using Acc = float;
@@ -295,10 +295,14 @@ __device__ __forceinline__ float finalize(const BpArgs& a, float v, int x,
 // per pixel).  Modes with kPlanes split them into NP planes of P slices (16
 // bytes of records per column and plane), each plane SPAN columns long, so
 // that neighbouring lanes on neighbouring columns read neighbouring 16-byte
-// words.  Measured on MI355X with the 64 x 1801 x 2560 stack, 4 slices:
-// the float32 mode takes 105.6 ms in planes with 16-row tiles against
-// 116.1 ms (32 rows) and 123.8 ms (16 rows) contiguous; the float64-sum
-// modes are VALU-bound and 2-4 % slower in planes, so they stay contiguous.
+// words.  Measured on MI355X with the 64 x 1801 x 2560 stack: the float32
+// mode (4 slices, two planes) takes 105.6 ms in planes with 16-row tiles
+// against 116.1 ms (32 rows) and 123.8 ms (16 rows) contiguous; the 16-bit
+// modes (8 slices, two planes) 82.7 ms with 16 rows against 90.2 ms
+// contiguous (and 102.6 ms in planes with 32 rows; padding the plane
+// stride to other residues modulo 16 columns changed nothing).  The
+// float64-sum modes are VALU-bound and 2-4 % slower in planes, so they
+// stay contiguous.
 template <class Rec, int SB, bool PLANES>
 struct RecPlanes {
     static constexpr int kPer16 = (int)(16 / sizeof(Rec));
@@ -794,6 +798,12 @@ void default_config(int nslice, int& sb, int& py) {
     if (nslice > 2 && (MODE == ATH_ACC_F64 || MODE == ATH_ACC_F64_DERIV ||
                        MODE == ATH_ACC_F32))
         py = 4;
+    // 16-bit records, 8 slices in two planes x 16 rows: 82.7 against
+    // 90.2 ms for 4 x 32
+    if (nslice >= 8 && (MODE == kAccH16 || MODE == kAccB16)) {
+        sb = 8;
+        py = 4;
+    }
     // the centre derivative reads one value per record: 2 slices x 16 rows
     // (16 x 901 x 1024: 3.47 ms against 4.84 ms for 4 x 32; 16 x 1801 x
     // 7192: 308 against 429 ms)
