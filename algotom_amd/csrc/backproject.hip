@@ -43,6 +43,7 @@ constexpr int kThreads = 256;
 constexpr int kTileX = 64;
 constexpr int kAccH16 = 16;  // internal modes: 16-bit storage + dot2
 constexpr int kAccB16 = 17;
+constexpr int kAccF64W = 18;  // ATH_ACC_F64 on single slices, float64 records
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
@@ -107,6 +108,31 @@ struct Mode<ATH_ACC_F64> {
     static __device__ __forceinline__ W weight(double fr) { return fr; }
     static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
         a += (double)q.x + (double)q.y * w;
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a += (double)e;
+    }
+    static __device__ __forceinline__ float result(Acc a) { return (float)a; }
+};
+
+// ATH_ACC_F64 for one slice per workgroup with the record staged as the
+// exactly converted float64 pair: the two converts per update move to the
+// staging loop and the sums are bit-identical.  A 16-byte record is one
+// ds_read_b128 per pixel (the float32 pair of 4 slices takes two).
+template <>
+struct Mode<kAccF64W> {
+    static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
+    using Rec = double2;
+    using W = double;
+    using Acc = double;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_double2((double)yd, (double)(yu - yd));
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a += q.x + q.y * w;
     }
     static __device__ __forceinline__ void add_value(Acc& a, float e) {
         a += (double)e;
@@ -754,8 +780,15 @@ int batched_config(const ath_bp_job* jobs, int njobs, long long total,
     if (sb == SB_ && py == PY_)                                            \
         return launch_batched<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(), \
                               Tin>(jobs, njobs, total, nblocks, counter, s);
+    if constexpr (MODE == ATH_ACC_F64) {
+        if (sb == 1)
+            return batched_config<kAccF64W, Tin>(jobs, njobs, total, sb, py,
+                                                 nblocks, counter, s);
+    }
     ATH_BPB_CASE(1, 4) ATH_BPB_CASE(1, 8) ATH_BPB_CASE(1, 16)
-    ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
+    if constexpr (MODE != kAccF64W) {
+        ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
+    }
 #undef ATH_BPB_CASE
     return set_error(ATH_ERR_UNSUPPORTED,
                      "batched back-projection: no kernel for slice_batch " +
@@ -774,9 +807,14 @@ int launch_config(const BpArgs& a, int sb, int py, hipStream_t s) {
         return launch_tiled<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(),  \
                             Tin>(                                          \
             a, s);
+    if constexpr (MODE == ATH_ACC_F64) {
+        if (sb == 1) return launch_config<kAccF64W, Tin>(a, sb, py, s);
+    }
     ATH_BP_CASE(1, 4) ATH_BP_CASE(1, 8) ATH_BP_CASE(1, 16)
-    ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
-    ATH_BP_CASE(4, 4) ATH_BP_CASE(4, 8) ATH_BP_CASE(4, 16)
+    if constexpr (MODE != kAccF64W) {
+        ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
+        ATH_BP_CASE(4, 4) ATH_BP_CASE(4, 8) ATH_BP_CASE(4, 16)
+    }
     if constexpr (wide) {
         ATH_BP_CASE(8, 4) ATH_BP_CASE(8, 8)
     }
