@@ -239,6 +239,51 @@ __device__ __forceinline__ FpBlock fp_block(const FpArgs& a) {
     return b;
 }
 
+// The column window of chunk lines m0..m1: the union over the workgroup's 4
+// angles of the minor coordinates whose f lies in bins [d0 - 2, d0 + 65] on
+// lines m0 and m1 (windows are linear in m).  Lanes 0-15 of wave 0 each
+// evaluate one end, reduce min / max across those 16 lanes, and lane 0
+// stores the staged column range and the skip flag into ab.
+__device__ __forceinline__ void chunk_window(const FpArgs& a, int tid, int m0,
+                                             int m1, int d0, bool by_rows,
+                                             const double (&cs)[4][2],
+                                             const double (&inv)[4],
+                                             int* ab) {
+    if (tid >= 64) return;
+    const double ic = a.icenter, cen = a.center;
+    double lo = 1e300, hi = -1e300;
+    if (tid < 16) {
+        const int q = tid >> 2, e = (tid >> 1) & 1, b = tid & 1;
+        const double mc = (double)(e ? m1 : m0) - ic;
+        const double base = (by_rows ? mc * cs[q][1] : mc * cs[q][0]) + cen;
+        const double bin = b ? (double)d0 + 65.0 : (double)d0 - 2.0;
+        // a near-parallel line (inverse 0) takes the whole line
+        const double u = inv[q] != 0.0 ? (bin - base) * inv[q]
+                                       : (b ? 1e300 : -1e300);
+        lo = u;
+        hi = u;
+    }
+    // lanes 0-15 are closed under these xor offsets
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off));
+        hi = fmax(hi, __shfl_xor(hi, off));
+    }
+    // a negative inverse swaps the two bin ends: min / max over all sixteen
+    // covers both orders.  The range is not clipped to the image: columns
+    // outside it are staged as zeros.
+    if (tid == 0) {
+        const int C = a.ncol;
+        const bool whole =
+            lo <= -1e299 || hi >= 1e299 || hi - lo > 2.0 * kMaxMinor;
+        const int ka = whole ? 0 : (int)floor(lo + ic) - 1;
+        const int kb = whole ? C - 1 + kMaxMinor : (int)ceil(hi + ic) + 1;
+        ab[0] = ka;
+        ab[1] = kb;
+        ab[2] = chunk_is_empty(a, m0, m1, ka, kb) ? 1 : 0;
+    }
+}
+
 // Stages lines m0 .. m0 + nrows - 1 and minor columns ka .. ka + wid - 1
 // of the chunk into the tile: the thread's kSteps pixels (mask and bounds
 // test once per pixel, pixels outside staged as zeros) are loaded before any
@@ -320,7 +365,7 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
     __shared__ __align__(16) float tile[(kRows * kMaxMinor + kMaxK) * RS];
     __shared__ double s_cs[4][2];
     __shared__ double s_inv[4];   // 1 / slope, 0 for near-parallel lines
-    __shared__ int s_ab[3];       // the chunk's staged column range, skip
+    __shared__ int s_ab[2][3];    // chunk column range and skip, 2 slots
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int C = a.ncol, S = a.nslice, N = a.nrow;
     const FpBlock blk = fp_block<FSB>(a);
@@ -379,52 +424,19 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
         acc[j] = 0.0;
     }
 
-    for (int m0 = 0; m0 < C; m0 += kRows) {
+    // the first chunk's window; each later one is computed by wave 0
+    // during the previous chunk's arithmetic into the other slot, and
+    // published by the barrier that ends that chunk
+    chunk_window(a, tid, 0, min(kRows, C) - 1, d0, by_rows, s_cs, s_inv,
+                 s_ab[0]);
+    __syncthreads();
+    for (int m0 = 0, slot = 0; m0 < C; m0 += kRows, slot ^= 1) {
         const int m1 = min(m0 + kRows, C) - 1;
-        // union window of the minor coordinate over 4 angles, bins
-        // [d0 - 2, d0 + 65] and lines m0, m1 (windows are linear in m):
-        // lanes 0-15 of wave 0 each evaluate one end, reduce min / max
-        // across those 16 lanes and lane 0 publishes the column range
-        if (tid < 64) {
-            double lo = 1e300, hi = -1e300;
-            if (tid < 16) {
-                const int q = tid >> 2, e = (tid >> 1) & 1, b = tid & 1;
-                const double mc = (double)(e ? m1 : m0) - ic;
-                const double base =
-                    (by_rows ? mc * s_cs[q][1] : mc * s_cs[q][0]) + cen;
-                const double bin = b ? (double)d0 + 65.0 : (double)d0 - 2.0;
-                // a near-parallel line (inverse 0) takes the whole line
-                const double u = s_inv[q] != 0.0 ? (bin - base) * s_inv[q]
-                                                 : (b ? 1e300 : -1e300);
-                lo = u;
-                hi = u;
-            }
-            // lanes 0-15 are closed under these xor offsets
-#pragma unroll
-            for (int off = 8; off > 0; off >>= 1) {
-                lo = fmin(lo, __shfl_xor(lo, off));
-                hi = fmax(hi, __shfl_xor(hi, off));
-            }
-            // a negative inverse swaps the two bin ends: min / max over
-            // all sixteen covers both orders.  The range is not clipped to
-            // the image: columns outside it are staged as zeros.
-            if (tid == 0) {
-                const bool whole = lo <= -1e299 || hi >= 1e299 ||
-                                   hi - lo > 2.0 * kMaxMinor;
-                const int ka = whole ? 0 : (int)floor(lo + ic) - 1;
-                const int kb = whole ? C - 1 + kMaxMinor
-                                     : (int)ceil(hi + ic) + 1;
-                s_ab[0] = ka;
-                s_ab[1] = kb;
-                s_ab[2] = chunk_is_empty(a, m0, m1, ka, kb) ? 1 : 0;
-            }
-        }
-        __syncthreads();
-        const int ka = s_ab[0], kb = s_ab[1];
+        const int ka = s_ab[slot][0], kb = s_ab[slot][1];
         // every pixel the chunk's runs (and their margins) can reach is
         // outside the mask or the image: its terms are all w * 0, so the
         // chunk is skipped (the flush schedule below is unchanged)
-        const bool empty = s_ab[2] != 0;
+        const bool empty = s_ab[slot][2] != 0;
         const int wid = kb - ka + 1;
         const bool staged = wid <= kMaxMinor;
         if (staged && !empty) {
@@ -442,6 +454,9 @@ __global__ __launch_bounds__(256) void forward_project_hat_kernel(
                                                    wid, j0, ns);
         }
         __syncthreads();
+        if (m0 + kRows < C)
+            chunk_window(a, tid, m0 + kRows, min(m0 + 2 * kRows, C) - 1, d0,
+                         by_rows, s_cs, s_inv, s_ab[slot ^ 1]);
         if (valid && !empty) {
             if (staged && fast_wave && wid >= K) {
                 const double u0 =
