@@ -387,13 +387,48 @@ __device__ __forceinline__ void inner_angle(
     }
 }
 
+// Per-angle data of an angle block in two slots: the next block's is
+// written while the current one's records are in use.
 template <class Rec, int SB, int A, int SPAN>
 struct alignas(16) BpShared {
     alignas(16) Rec rec[A * SPAN * SB];  // 16 B aligned: ds_read_b128 records
-    double cos_[A], sin_[A];
-    int dlo[A], inner[A];
+    double cos_[2][A], sin_[2][A];
+    int dlo[2][A], inner[2][A];
+    int all_inner[2];
     float edge[A * SB * 2];
 };
+
+// Angles b0 .. b0 + nb - 1 (nb <= A <= 64, so all in wave 0): float32
+// cos / sin as the reference rounds them, the tile's detector footprint from
+// its four corners, whether the footprint is inside [0, ncol-1], and
+// whether that holds for every angle of the block.
+template <class M, class Sh>
+__device__ __forceinline__ void angle_setup(const BpArgs& a, Sh& sh, int sl,
+                                            int b0, int nb, int tid,
+                                            double cx0, double cx1,
+                                            double cy0, double cy1,
+                                            double cen, double w1) {
+    if (tid >= 64) return;
+    int inner = 1;
+    if (tid < nb) {
+        const float th = -a.angles[b0 + tid];
+        const double c = (double)(float)cos((double)th);
+        const double s = (double)(float)sin((double)th);
+        const double f00 = det_pos(cx0 * c, cy0, s, cen);
+        const double f01 = det_pos(cx1 * c, cy0, s, cen);
+        const double f10 = det_pos(cx0 * c, cy1, s, cen);
+        const double f11 = det_pos(cx1 * c, cy1, s, cen);
+        const double fmn = fmin(fmin(f00, f01), fmin(f10, f11));
+        const double fmx = fmax(fmax(f00, f01), fmax(f10, f11));
+        sh.cos_[sl][tid] = c;
+        sh.sin_[sl][tid] = s;
+        sh.dlo[sl][tid] = (int)floor(fmn) - (M::kF32Pos ? 1 : 0);
+        inner = (fmn >= 0.0) && (fmx <= w1);
+        sh.inner[sl][tid] = inner;
+    }
+    const unsigned long long out = __ballot(!inner);
+    if (tid == 0) sh.all_inner[sl] = out == 0ull ? 1 : 0;
+}
 
 // Back-projects tile (tx, ty) of slice group grp of one stack.  Called by
 // every thread of the workgroup; starts and ends with all LDS consumed.
@@ -408,10 +443,6 @@ __device__ __forceinline__ void bp_tile(
     constexpr int TY = 4 * PY;
     constexpr int SPAN = mode_span<M>(kTileX, TY);
     Rec* s_rec = sh.rec;
-    double* s_cos = sh.cos_;
-    double* s_sin = sh.sin_;
-    int* s_dlo = sh.dlo;
-    int* s_inner = sh.inner;
     float* s_edge = sh.edge;
 
     const Tin* __restrict__ sino = static_cast<const Tin*>(a.sino);
@@ -438,29 +469,18 @@ __device__ __forceinline__ void bp_tile(
 #pragma unroll
         for (int js = 0; js < SB; ++js) acc[k][js] = typename M::Acc{};
 
-    for (int a0 = 0; a0 < N; a0 += A) {
+    // the first block's angles; each later block's are set up by wave 0
+    // during the previous block's arithmetic, into the other slot, and
+    // published by the barrier that ends that block
+    angle_setup<M>(a, sh, 0, 0, min(A, N), tid, cx0, cx1, cy0, cy1, cen, w1);
+    __syncthreads();
+    for (int a0 = 0, sl = 0; a0 < N; a0 += A, sl ^= 1) {
         const int na = min(A, N - a0);
-        __syncthreads();  // previous block's LDS fully consumed
-        int inner = 1;
-        if (tid < na) {
-            const float th = -a.angles[a0 + tid];
-            const double c = (double)(float)cos((double)th);
-            const double s = (double)(float)sin((double)th);
-            const double f00 = det_pos(cx0 * c, cy0, s, cen);
-            const double f01 = det_pos(cx1 * c, cy0, s, cen);
-            const double f10 = det_pos(cx0 * c, cy1, s, cen);
-            const double f11 = det_pos(cx1 * c, cy1, s, cen);
-            const double fmn = fmin(fmin(f00, f01), fmin(f10, f11));
-            const double fmx = fmax(fmax(f00, f01), fmax(f10, f11));
-            s_cos[tid] = c;
-            s_sin[tid] = s;
-            s_dlo[tid] = (int)floor(fmn) - (M::kF32Pos ? 1 : 0);
-            inner = (fmn >= 0.0) && (fmx <= w1);
-            s_inner[tid] = inner;
-        }
-        // barrier that also tells every thread whether all na angles are
-        // in range for the whole tile
-        const int all_inner = __syncthreads_and(inner);
+        const double* s_cos = sh.cos_[sl];
+        const double* s_sin = sh.sin_[sl];
+        const int* s_dlo = sh.dlo[sl];
+        const int* s_inner = sh.inner[sl];
+        const int all_inner = sh.all_inner[sl];
         // one record per (angle, column): the SB slices' (s, ds)
         const int nrec = na * SPAN;
         // two records per thread and step, both loaded before either is
@@ -531,6 +551,9 @@ __device__ __forceinline__ void bp_tile(
             }
         }
         __syncthreads();
+        if (a0 + A < N)
+            angle_setup<M>(a, sh, sl ^ 1, a0 + A, min(A, N - a0 - A), tid,
+                           cx0, cx1, cy0, cy1, cen, w1);
 
         if (all_inner) {
             // common case: the whole block is in range for the whole tile
@@ -586,6 +609,9 @@ __device__ __forceinline__ void bp_tile(
                 }
             }
         }
+        // the block's records and angles are consumed; the next block's
+        // angles are published
+        __syncthreads();
     }
 
     if (x < C) {
