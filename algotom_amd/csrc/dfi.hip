@@ -335,11 +335,13 @@ __device__ __forceinline__ void bluestein_core(double2* buf, int M,
                                                const double2* __restrict__ tw,
                                                const double2* tq,
                                                const double2* __restrict__ bhat) {
-    lds_fft<-1, MAXB, QT, NT>(buf, M, tw, tq);
+    // the rotated first radix-8 pass (conflict-free LDS writes; without the
+    // quarter table only)
+    lds_fft<-1, MAXB, QT, NT, 1, true>(buf, M, tw, tq);
     for (int k = threadIdx.x; k < M; k += NT)
         buf[k] = cmul(buf[k], bhat[k]);
     __syncthreads();
-    lds_fft<+1, MAXB, QT, NT>(buf, M, tw, tq);
+    lds_fft<+1, MAXB, QT, NT, 1, true>(buf, M, tw, tq);
 }
 
 // The quarter twiddle table sits in LDS beside the buffer for M <= 4096.

@@ -113,7 +113,7 @@ __device__ __forceinline__ void dft8(double2 (&v)[8]) {
 // TWS: the table's index stride (tw[n * TWS] = exp(-2 pi i n / L)), so a
 // transform can read the table of a multiple of its length.
 template <int DIR, int MAXB, bool QT = false, int NT = kFftThreads,
-          int TWS = 1>
+          int TWS = 1, bool ROT = false>
 __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
                         const double2* tq = nullptr) {
     static_assert(TWS == 1 || !QT, "the quarter table has stride 1");
@@ -124,11 +124,17 @@ __device__ void lds_fft(double2* buf, int M, const double2* __restrict__ tw,
     // radix-4 and one radix-2 pass for the remaining factor
     constexpr int B8 = (MAXB + 1) / 2;
     // The first radix-8 pass writes 8 consecutive elements per lane (8-way
-    // bank conflicts); rotating each lane pair's outputs removes them at the
-    // cost of 14 complex products.  Only without the LDS quarter table: with
-    // it the extra registers push the row kernels past 128 VGPRs (one
-    // workgroup per CU instead of two), which costs more than it saves.
-    constexpr bool kRotate = !QT;
+    // bank conflicts); with ROT each lane pair's outputs are rotated, which
+    // removes them at the cost of 14 complex products and 10-90 more VGPRs.
+    // Taken by the DFI Bluestein transforms (1.75 -> 1.68 ms per 1801 x 2560
+    // slice), the two-level transforms below (DFI 16384-point columns 1.10
+    // -> 1.00 ms, BPF 16384-point columns 3.31 -> 3.11 ms) and the 1D
+    // filter's row kernels without the quarter table (7192-wide volume rows
+    // 4.54 -> 4.22 ms).  Not by the 2D filter's row and column kernels:
+    // there the registers cost occupancy (rows 62 -> 148 VGPRs, 8 -> 3
+    // waves per SIMD; 0.43 -> 0.66 ms per 8-image row pass).  Never with the
+    // LDS quarter table.
+    constexpr bool kRotate = ROT && !QT;
     while (ns * 8 <= M) {
         const int nb = M >> 3;
         const int tstep = L / (ns * 8);
@@ -295,13 +301,13 @@ __device__ void conv_big(double2* buf, double2* scratch, int N,
     for (int j = tid; j < H; j += NT)
         buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<-1, kBigB, false, NT, 1, true>(buf, H, tw);
     for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
     __syncthreads();
     for (int j = tid; j < H; j += NT)
         buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<-1, kBigB, false, NT, 1, true>(buf, H, tw);
     for (int k = tid; k < H; k += NT) {
         const double2 w = tw[k];
         const double2 e = scratch[k];
@@ -312,12 +318,12 @@ __device__ void conv_big(double2* buf, double2* scratch, int N,
         buf[k] = cmul(csub(ylo, yhi), conjg(w));
     }
     __syncthreads();
-    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<+1, kBigB, false, NT, 1, true>(buf, H, tw);
     for (int q = tid; q < H; q += NT) scratch[H + q] = buf[q];
     __syncthreads();
     for (int k = tid; k < H; k += NT) buf[k] = scratch[k];
     __syncthreads();
-    lds_fft<+1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<+1, kBigB, false, NT, 1, true>(buf, H, tw);
 }
 
 __device__ __forceinline__ double2 big_conv(const double2* buf,
@@ -349,7 +355,7 @@ __device__ void fft_big4_quarters(double2* buf, double2* scratch, int N,
             buf[j] = n < N ? load(n) : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        lds_fft<-1, kBigB, false, NT, 2>(buf, H, tw);
+        lds_fft<-1, kBigB, false, NT, 2, true>(buf, H, tw);
         if (r < 3) {
             for (int k = tid; k < H; k += NT) scratch[r * H + k] = buf[k];
             __syncthreads();
@@ -419,7 +425,7 @@ __device__ void conv_big4(double2* buf, double2* scratch, int N,
             for (int k = tid; k < H; k += NT) buf[k] = scratch[s * H + k];
             __syncthreads();
         }
-        lds_fft<1, kBigB, false, NT, 2>(buf, H, tw);
+        lds_fft<1, kBigB, false, NT, 2, true>(buf, H, tw);
         if (s > 0) {
             for (int j = tid; j < H; j += NT) scratch[s * H + j] = buf[j];
             __syncthreads();
@@ -473,13 +479,13 @@ __device__ void fft_big_forward(double2* buf, double2* scratch, int N,
     for (int j = tid; j < H; j += NT)
         buf[j] = 2 * j < N ? load(2 * j) : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<-1, kBigB, false, NT, 1, true>(buf, H, tw);
     for (int k = tid; k < H; k += NT) scratch[k] = buf[k];
     __syncthreads();
     for (int j = tid; j < H; j += NT)
         buf[j] = 2 * j + 1 < N ? load(2 * j + 1) : make_double2(0.0, 0.0);
     __syncthreads();
-    lds_fft<-1, kBigB, false, NT>(buf, H, tw);
+    lds_fft<-1, kBigB, false, NT, 1, true>(buf, H, tw);
     for (int k = tid; k < H; k += NT) {
         const double2 e = scratch[k];
         const double2 o = cmul(buf[k], tw[k]);
@@ -564,7 +570,7 @@ __device__ __forceinline__ void big_forward_halves(
         for (int j = tid; j < H; j += NT)
             buf[j] = make_double2(val(4 * j + 2 * h), val(4 * j + 2 * h + 1));
         __syncthreads();
-        lds_fft<-1, kBigB, false, NT, 2>(buf, H, tw);
+        lds_fft<-1, kBigB, false, NT, 2, true>(buf, H, tw);
     }
 }
 
@@ -653,7 +659,7 @@ __device__ __forceinline__ void big_inverse_phases(
             if (tid == 0) buf[H / 2] = r.mid;
             __syncthreads();
         }
-        lds_fft<1, kBigB, false, NT, 2>(buf, H, tw);
+        lds_fft<1, kBigB, false, NT, 2, true>(buf, H, tw);
         out(h);
         __syncthreads();  // buf free for U, then for the caller
     }

@@ -242,7 +242,7 @@ __device__ __forceinline__ void filter_row(
             padded_sample<Tin, ADJ>(src_row, 2 * n + 1, width, ncol, pad,
                                     pad_mode, apply_log, value_mode, pv));
     __syncthreads();
-    lds_fft<-1, B, QT, NT>(buf, M, tw, tq);
+    lds_fft<-1, B, QT, NT, 1, true>(buf, M, tw, tq);
 
     // rfft split, multiply by the kernel spectrum, merge for the inverse
     for (int k = tid; k <= M / 2; k += NT) {
@@ -253,7 +253,7 @@ __device__ __forceinline__ void filter_row(
         if (km != k) buf[km] = zm;
     }
     __syncthreads();
-    lds_fft<1, B, QT, NT>(buf, M, tw, tq);
+    lds_fft<1, B, QT, NT, 1, true>(buf, M, tw, tq);
 
     const double* y = reinterpret_cast<const double*>(buf);
     if (!ADJ) {

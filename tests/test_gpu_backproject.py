@@ -25,6 +25,11 @@ CASES = [
     (96, 40, 4, 3.0, 2 * np.pi, False),
     (96, 40, 1, 94.9, np.pi, True),
     (512, 64, 1, 256.0, np.pi, False),
+    # one angle past a whole angle block (single slices stage 32 or 64
+    # angles per block, stacks 16 or 32): the last block holds one angle
+    (97, 65, 1, 48.2, np.pi, True),
+    (97, 33, 1, 48.0, 2 * np.pi, False),
+    (97, 33, 4, 47.6, np.pi, True),
     # the widths of the uneven batch and of the stitched volume, with few
     # angles: the largest staging spans and 64-bit offsets
     (4096, 23, 1, 2047.5 + 13.7, np.pi, False),
