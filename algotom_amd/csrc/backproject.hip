@@ -440,6 +440,7 @@ __device__ __forceinline__ void bp_tile(
     using M = Mode<MODE>;
     using Rec = typename M::Rec;
     using L = RecPlanes<Rec, SB, M::kPlanes>;
+    static_assert(A <= 64, "angle_setup runs one block's angles in wave 0");
     constexpr int TY = 4 * PY;
     constexpr int SPAN = mode_span<M>(kTileX, TY);
     Rec* s_rec = sh.rec;
