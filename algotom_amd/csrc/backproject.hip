@@ -332,7 +332,8 @@ __device__ __forceinline__ float finalize(const BpArgs& a, float v, int x,
 template <class Rec, int SB, bool PLANES>
 struct RecPlanes {
     static constexpr int kPer16 = (int)(16 / sizeof(Rec));
-    static constexpr int P = PLANES && kPer16 >= 1 && kPer16 < SB ? kPer16 : SB;
+    static constexpr int P =
+        PLANES && kPer16 >= 1 && kPer16 < SB ? kPer16 : SB;
     static constexpr int NP = SB / P;
     static_assert(NP * P == SB, "slice batch must be a multiple of the plane");
     // records from the column's first record to slice js's
@@ -795,8 +796,9 @@ constexpr int angle_block() {
     if (MODE == ATH_ACC_DERIV) return 16;
     using Rec = typename Mode<MODE>::Rec;
     constexpr int span = mode_span<Mode<MODE>>(kTileX, 4 * PY);
+    // records, two slots of cos / sin / dlo / inner, edge values
     constexpr size_t per_angle =
-        (size_t)span * SB * sizeof(Rec) + 24 + 8 * SB;
+        (size_t)span * SB * sizeof(Rec) + 48 + 8 * SB;
     return per_angle * 64 <= 53000 ? 64 : (per_angle * 32 <= 53000 ? 32 : 16);
 }
 
