@@ -44,6 +44,7 @@ constexpr int kTileX = 64;
 constexpr int kAccH16 = 16;  // internal modes: 16-bit storage + dot2
 constexpr int kAccB16 = 17;
 constexpr int kAccF64W = 18;  // ATH_ACC_F64 on single slices, float64 records
+constexpr int kAccF32SeqW = 19;  // ATH_ACC_F32SEQ likewise
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
@@ -138,6 +139,29 @@ struct Mode<kAccF64W> {
         a += (double)e;
     }
     static __device__ __forceinline__ float result(Acc a) { return (float)a; }
+};
+
+// ATH_ACC_F32SEQ for one slice per workgroup with float64 records, as
+// kAccF64W: the same bits, the converts moved into staging.
+template <>
+struct Mode<kAccF32SeqW> {
+    static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
+    using Rec = double2;
+    using W = double;
+    using Acc = float;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_double2((double)yd, (double)(yu - yd));
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W w) {
+        a = (float)((double)a + (q.x + q.y * w));
+    }
+    static __device__ __forceinline__ void add_value(Acc& a, float e) {
+        a = (float)((double)a + (double)e);
+    }
+    static __device__ __forceinline__ float result(Acc a) { return a; }
 };
 
 template <>
@@ -814,8 +838,13 @@ int batched_config(const ath_bp_job* jobs, int njobs, long long total,
             return batched_config<kAccF64W, Tin>(jobs, njobs, total, sb, py,
                                                  nblocks, counter, s);
     }
+    if constexpr (MODE == ATH_ACC_F32SEQ) {
+        if (sb == 1)
+            return batched_config<kAccF32SeqW, Tin>(jobs, njobs, total, sb,
+                                                    py, nblocks, counter, s);
+    }
     ATH_BPB_CASE(1, 4) ATH_BPB_CASE(1, 8) ATH_BPB_CASE(1, 16)
-    if constexpr (MODE != kAccF64W) {
+    if constexpr (MODE != kAccF64W && MODE != kAccF32SeqW) {
         ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
     }
 #undef ATH_BPB_CASE
@@ -839,8 +868,11 @@ int launch_config(const BpArgs& a, int sb, int py, hipStream_t s) {
     if constexpr (MODE == ATH_ACC_F64) {
         if (sb == 1) return launch_config<kAccF64W, Tin>(a, sb, py, s);
     }
+    if constexpr (MODE == ATH_ACC_F32SEQ) {
+        if (sb == 1) return launch_config<kAccF32SeqW, Tin>(a, sb, py, s);
+    }
     ATH_BP_CASE(1, 4) ATH_BP_CASE(1, 8) ATH_BP_CASE(1, 16)
-    if constexpr (MODE != kAccF64W) {
+    if constexpr (MODE != kAccF64W && MODE != kAccF32SeqW) {
         ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
         ATH_BP_CASE(4, 4) ATH_BP_CASE(4, 8) ATH_BP_CASE(4, 16)
     }
