@@ -45,6 +45,21 @@ constexpr int kAccH16 = 16;  // internal modes: 16-bit storage + dot2
 constexpr int kAccB16 = 17;
 constexpr int kAccF64W = 18;  // ATH_ACC_F64 on single slices, float64 records
 constexpr int kAccF32SeqW = 19;  // ATH_ACC_F32SEQ likewise
+constexpr int kAccDerivW = 20;     // ATH_ACC_DERIV likewise
+// The single-slice float64-record form of a mode, or the mode itself.
+// Measured on single 1801-angle slices: f64 2560 columns 244.7 -> 237.7 ms
+// for 64 slices, f32seq 282.0 -> 239.6 ms, the centre derivative 3.23 ->
+// 2.86 ms (22.4 -> 19.9 ms at 7192 columns).  Image + derivative in one
+// pass (ATH_ACC_F64_DERIV) keeps float32 records: 4.91 against 5.25 ms.
+constexpr int wide1_of(int mode) {
+    return mode == ATH_ACC_F64 ? kAccF64W
+         : mode == ATH_ACC_F32SEQ ? kAccF32SeqW
+         : mode == ATH_ACC_DERIV ? kAccDerivW
+         : mode;
+}
+constexpr bool is_wide1(int mode) {
+    return mode == kAccF64W || mode == kAccF32SeqW || mode == kAccDerivW;
+}
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
@@ -260,6 +275,27 @@ struct Mode<ATH_ACC_F64_DERIV> {
     static __device__ __forceinline__ float result2(Acc a) {
         return (float)a.dv;
     }
+};
+
+// ATH_ACC_DERIV for one slice per workgroup with float64 records (as
+// kAccF64W; the same bits).
+template <>
+struct Mode<kAccDerivW> {
+    static constexpr bool kF32Pos = false;
+    static constexpr bool kTwoOut = false;
+    static constexpr bool kPlanes = false;
+    using Rec = double2;
+    using W = double;
+    using Acc = double;
+    static __device__ __forceinline__ Rec make(float yd, float yu) {
+        return make_double2((double)yd, (double)(yu - yd));
+    }
+    static __device__ __forceinline__ W weight(double fr) { return fr; }
+    static __device__ __forceinline__ void add(Acc& a, Rec q, W) {
+        a += q.y;
+    }
+    static __device__ __forceinline__ void add_value(Acc&, float) {}
+    static __device__ __forceinline__ float result(Acc a) { return (float)a; }
 };
 
 template <>
@@ -833,18 +869,15 @@ int batched_config(const ath_bp_job* jobs, int njobs, long long total,
     if (sb == SB_ && py == PY_)                                            \
         return launch_batched<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(), \
                               Tin>(jobs, njobs, total, nblocks, counter, s);
-    if constexpr (MODE == ATH_ACC_F64) {
+    // single slices of the float64-sum modes: float64 records
+    if constexpr (wide1_of(MODE) != MODE) {
         if (sb == 1)
-            return batched_config<kAccF64W, Tin>(jobs, njobs, total, sb, py,
-                                                 nblocks, counter, s);
-    }
-    if constexpr (MODE == ATH_ACC_F32SEQ) {
-        if (sb == 1)
-            return batched_config<kAccF32SeqW, Tin>(jobs, njobs, total, sb,
-                                                    py, nblocks, counter, s);
+            return batched_config<wide1_of(MODE), Tin>(jobs, njobs, total, sb,
+                                                       py, nblocks, counter,
+                                                       s);
     }
     ATH_BPB_CASE(1, 4) ATH_BPB_CASE(1, 8) ATH_BPB_CASE(1, 16)
-    if constexpr (MODE != kAccF64W && MODE != kAccF32SeqW) {
+    if constexpr (!is_wide1(MODE)) {
         ATH_BPB_CASE(4, 4) ATH_BPB_CASE(4, 8)
     }
 #undef ATH_BPB_CASE
@@ -865,14 +898,12 @@ int launch_config(const BpArgs& a, int sb, int py, hipStream_t s) {
         return launch_tiled<MODE, SB_, PY_, angle_block<MODE, SB_, PY_>(),  \
                             Tin>(                                          \
             a, s);
-    if constexpr (MODE == ATH_ACC_F64) {
-        if (sb == 1) return launch_config<kAccF64W, Tin>(a, sb, py, s);
-    }
-    if constexpr (MODE == ATH_ACC_F32SEQ) {
-        if (sb == 1) return launch_config<kAccF32SeqW, Tin>(a, sb, py, s);
+    // single slices of the float64-sum modes: float64 records
+    if constexpr (wide1_of(MODE) != MODE) {
+        if (sb == 1) return launch_config<wide1_of(MODE), Tin>(a, sb, py, s);
     }
     ATH_BP_CASE(1, 4) ATH_BP_CASE(1, 8) ATH_BP_CASE(1, 16)
-    if constexpr (MODE != kAccF64W && MODE != kAccF32SeqW) {
+    if constexpr (!is_wide1(MODE)) {
         ATH_BP_CASE(2, 4) ATH_BP_CASE(2, 8) ATH_BP_CASE(2, 16)
         ATH_BP_CASE(4, 4) ATH_BP_CASE(4, 8) ATH_BP_CASE(4, 16)
     }
