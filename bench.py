@@ -813,8 +813,8 @@ def main():
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs no multi-GPU "
-              "path (slices shard with no exchange; see DESIGN.md)",
+        print("bench.py measures one GPU: there is no multi-GPU path "
+              "(slices would shard across GPUs with no exchange)",
               file=sys.stderr)
         sys.exit(2)
 
