@@ -1,7 +1,7 @@
 """Oracle for DFI (direct Fourier inversion) -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
-``cpu_baseline`` leg may import this module, as the checker.
+``cpu_baseline`` leg may import this module, to check results against.
 
 Restates algotom/rec/reconstruction.py:
 * ``mapping_coordinates``  :687-728 (generate_mapping_coordinate);

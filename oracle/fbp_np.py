@@ -1,7 +1,7 @@
 """CPU oracle for the FBP path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
-leg may import this module, and only as the checker.  The product path
+leg may import this module, and only to check results against.  The product path
 (``algotom_amd``) never imports it and fails loudly when its HIP library is
 missing.
 
