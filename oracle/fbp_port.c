@@ -2,8 +2,8 @@
  * CPU port of algotom's FBP path -- TEST INFRASTRUCTURE ONLY (the oracle).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * this library, and only to check results against or as the CPU baseline.  The product
- * (algotom_amd) never links it.
+ * this library, and only to check results against or as the CPU baseline.
+ * The product (algotom_amd) never links it.
  *
  * Restates, with numba's typing (algotom 1.6.0):
  *   orc_backproject  <- algotom/rec/reconstruction.py:307-359 (CPU kernel),
