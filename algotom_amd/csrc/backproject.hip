@@ -567,13 +567,14 @@ __device__ __forceinline__ void bp_tile(
                 }
                 if (e >= nrec) continue;
                 const int col = s_dlo[al] + k;
-                const Tin* rows =
-                    sino + ((size_t)(a0 + al) * S + j0) * C + col;
+                // column offsets are added only where they are in range
+                const Tin* rows = sino + ((size_t)(a0 + al) * S + j0) * C;
                 if (col >= 0 && col + 1 < C && nsb == SB) {
 #pragma unroll
                     for (int js = 0; js < SB; ++js) {
-                        yd[u][js] = load_f32(rows + (size_t)js * C);
-                        yu[u][js] = load_f32(rows + (size_t)js * C + 1);
+                        const Tin* p = rows + (size_t)js * C + col;
+                        yd[u][js] = load_f32(p);
+                        yu[u][js] = load_f32(p + 1);
                     }
                 } else {
                     const bool in0 = col >= 0 && col < C;
@@ -582,8 +583,8 @@ __device__ __forceinline__ void bp_tile(
                     for (int js = 0; js < SB; ++js) {
                         if (js < nsb) {
                             const Tin* row = rows + (size_t)js * C;
-                            if (in0) yd[u][js] = load_f32(row);
-                            if (in1) yu[u][js] = load_f32(row + 1);
+                            if (in0) yd[u][js] = load_f32(row + col);
+                            if (in1) yu[u][js] = load_f32(row + (col + 1));
                         }
                     }
                 }
