@@ -850,7 +850,9 @@ int launch_batched(const ath_bp_job* jobs, int njobs, long long total,
 //     fp16 stack 117.1 ms against 122.0 with 16 and 136.7 with 64;
 //   image + centre derivative, 2 slices x 16 rows (32 angles): 4.97 against
 //     5.22 ms at 16 x 901 x 1024, 447.6 against 478.4 ms at 16 x 1801 x 7192;
-//   the centre derivative alone stays at 16 (3.35 against 3.46 ms with 32);
+//   the centre derivative alone stays at 16 (3.35 against 3.46 ms with 32;
+//     3.16 against 3.29 ms with the current staging, 308 against 316 ms at
+//     16 x 1801 x 7192);
 //   float32 and float64 records, 4 slices: 16 (32 would not fit).
 template <int MODE, int SB, int PY>
 constexpr int angle_block() {
